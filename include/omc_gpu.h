@@ -1,0 +1,229 @@
+/*
+ * omc_gpu.h -- C ABI of the MI355X transport engine.
+ *
+ * The reference (e0404/ompMC) is driven by two user codes that each run the
+ * whole simulation inside one process with global state: the CLI
+ * ucodes/omc_dosxyz/omc_dosxyz.c (main, :1072-1307) and the MATLAB mex
+ * ucodes/omc_matrad/omc_matrad.c (mexFunction, :1417-1789; legacy
+ * matRad_ompInterface.c:41-652).  Between them and the engine sits the
+ * per-step plugin API of src/ompmc.h:39-41,422 (ausgab / howfar / hownear /
+ * initRegions over the thread-private particle stack), which a GPU cannot
+ * call back into.  This header therefore replaces the engine + ucode pair at
+ * the level those front ends need: build the tables once, upload a phantom,
+ * run a dose or a dose-influence (Dij) simulation, read the results.  All
+ * arguments are plain C pointers and sizes; every call returns 0 or an OMC_ERR_*
+ * code (never exits) and omc_last_error() describes the failure.
+ *
+ * Threading: a context may be used from one thread at a time, except the
+ * serving calls (omc_server_*), which are thread safe.
+ */
+#ifndef OMC_GPU_H
+#define OMC_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OMC_API_VERSION 1
+
+/* error codes (also returned by the host layer) */
+#define OMC_OK 0
+#define OMC_ERR_IO 1
+#define OMC_ERR_FORMAT 2
+#define OMC_ERR_MEDIUM 3
+#define OMC_ERR_LIMIT 4
+#define OMC_ERR_ARG 5
+#define OMC_ERR_DEVICE 6
+#define OMC_ERR_NOMEM 7
+#define OMC_ERR_BUDGET 8
+#define OMC_ERR_STACK 9
+#define OMC_ERR_CANCELLED 10
+
+/* transport precision: FP32 transport with FP64 tallies, or FP64 throughout */
+#define OMC_FP32 32
+#define OMC_FP64 64
+/* execution form: one lane per history (plain), or event queues (batched) */
+#define OMC_FORM_HISTORY 0
+#define OMC_FORM_EVENT 1
+/* random numbers: counter-based Philox streams keyed by (seed, history), or a
+ * replayed 24-bit stream (RANMAR output) with per-history offsets */
+#define OMC_RNG_PHILOX 0
+#define OMC_RNG_REPLAY 1
+
+typedef struct omc_ctx omc_ctx;
+
+/* Last error message of the calling thread. */
+const char *omc_last_error(void);
+int omc_api_version(void);
+
+/* Builds all physics tables on the host (reference initMediaData,
+ * src/ompmc.c:5468-5505: PEGS4, XCOM, pgs4form, msnew, spinms) and uploads
+ * FP32 and FP64 copies to device `device`.  media: nmed PEGS4 names
+ * (reference media.med_names, filled from the egsphant header or mcGeo.material). */
+int omc_ctx_create(omc_ctx **ctx, int nmed, const char *const *media, const char *pegs_file,
+                   const char *pgs4form_file, const char *data_folder, int device);
+void omc_ctx_destroy(omc_ctx *ctx);
+/* Host FP64 view of a physics table by name (names of the reference's
+ * arrays, e.g. "gmfp0", "esig1", "spin_rej"); for verification. */
+int omc_ctx_table(omc_ctx *ctx, const char *name, const double **data, long *n);
+
+/* Voxel phantom (reference struct Geom + initRegions, omc_dosxyz.c:48-59,
+ * 890-962; omc_matrad.c:364-453, 1103-1175).  med is 1-based per voxel (0 =
+ * vacuum), x fastest; ecut/pcut are the global cuts (total energies, MeV). */
+typedef struct omc_phantom_desc {
+    int nx, ny, nz;
+    const double *xb, *yb, *zb;
+    const int *med;
+    const double *rho;
+    double ecut, pcut;
+} omc_phantom_desc;
+
+int omc_phantom_upload(omc_ctx *ctx, const omc_phantom_desc *desc, int *phantom_id);
+int omc_phantom_release(omc_ctx *ctx, int phantom_id);
+
+/* Energy source: mono energy or the spectrum inverse CDF of the reference
+ * (omc_dosxyz.c:396-510): cdfinv1/cdfinv2 hold OMC_INVDIM = 1000 entries. */
+typedef struct omc_spectrum_desc {
+    int spectrum;
+    double energy;
+    const double *cdfinv1, *cdfinv2;
+} omc_spectrum_desc;
+
+/* omc_dosxyz collimated point source (omc_dosxyz.c:521-617, 964-1068). */
+typedef struct omc_dosxyz_source_desc {
+    omc_spectrum_desc spec;
+    int charge;
+    double ssd;
+    double collimator[4];   /* xl xu yl yu */
+} omc_dosxyz_source_desc;
+
+/* Options of a run (precision, form, random numbers, launch shape). */
+typedef struct omc_run_opts {
+    int precision;          /* OMC_FP32 | OMC_FP64 */
+    int form;               /* OMC_FORM_HISTORY | OMC_FORM_EVENT */
+    int rng;                /* OMC_RNG_PHILOX | OMC_RNG_REPLAY */
+    uint64_t seed;          /* Philox key */
+    int nsplit;             /* photon splitting; the GPU forms accept 1 */
+    const uint32_t *replay;       /* OMC_RNG_REPLAY: draws (24-bit integers) */
+    const int64_t *replay_offsets;/* first draw of each history, nhist+1 entries */
+    int64_t replay_len;
+    int grid, block;        /* 0 = automatic */
+    int count_events;       /* fill omc_run_stats counters */
+} omc_run_opts;
+
+typedef struct omc_run_stats {
+    double gpu_seconds;             /* device time of the transport kernels */
+    double wall_seconds;            /* whole call */
+    double histories;
+    uint64_t n_howfar, n_hownear, n_cross, n_dep, n_rn, n_photon, n_electron;
+    int errors;                     /* bit 0 stack overflow, 1 replay underrun, 2 step guard */
+} omc_run_stats;
+
+/* Defaults: FP32, event form, Philox, seed 1802, nsplit 1, automatic launch. */
+void omc_run_opts_default(omc_run_opts *o);
+
+/* Dose run with omc_dosxyz semantics (main, omc_dosxyz.c:1206-1263):
+ * nbatch batches of nperbatch histories; after each batch the per-region
+ * energy e is folded into sum += e, sum2 += e^2 (accumEndep, :696-717).
+ * sum/sum2 are host arrays of nvox+1 doubles (region 0 = outside). */
+int omc_run_dosxyz(omc_ctx *ctx, int phantom_id, const omc_dosxyz_source_desc *src, int64_t nperbatch,
+                   int nbatch, const omc_run_opts *opts, double *sum, double *sum2, omc_run_stats *stats);
+
+/* Dose and relative uncertainty from the batch sums (accumulateResults,
+ * omc_dosxyz.c:719-799), and the .3ddose writer (:801-886). */
+int omc_dosxyz_dose(omc_ctx *ctx, int phantom_id, const double *sum, const double *sum2, int64_t nperbatch,
+                    int nbatch, double *dose, double *unc);
+int omc_write_3ddose_file(omc_ctx *ctx, int phantom_id, const char *path, const double *dose, const double *unc);
+
+/* matRad beamlet set (mcSrc of omc_matrad.c:780-829). */
+typedef struct omc_beamlets_desc {
+    int nbixels, nbeams;
+    const int *ibeam;              /* 0-based beam of each bixel */
+    const double *src;             /* [nbeams*3] source points */
+    const double *corner, *side1, *side2;   /* [nbixels*3] */
+    int gaussian;                  /* sourceGeometry 'gaussian' */
+    double sigma;                  /* sourceGaussianWidth */
+    const int64_t *nhist;          /* histories per bixel, or NULL for nhist_each */
+    int64_t nhist_each;
+} omc_beamlets_desc;
+
+/* Compressed sparse column matrix (MATLAB layout): column j holds rows
+ * ir[jc[j]..jc[j+1]-1] with values val[]; allocated by the library, released
+ * with omc_csc_free. */
+typedef struct omc_csc {
+    int64_t nrows, ncols, nnz;
+    int64_t *jc, *ir;
+    double *val;
+} omc_csc;
+void omc_csc_free(omc_csc *m);
+
+/* Dose-influence matrix with omc_matrad semantics (mexFunction,
+ * omc_matrad.c:1576-1763): per bixel nbatch batches, dose per history in Gy,
+ * entries strictly above rel_threshold times the bixel maximum, rows are
+ * voxel indices (x fastest, 0-based).  var (optional) receives the variance
+ * matrix of the same pattern, including the reference's carry-over of the
+ * previous bixel's variance array. */
+int omc_run_dij(omc_ctx *ctx, int phantom_id, const omc_spectrum_desc *spec, const omc_beamlets_desc *bix,
+                int nbatch, double rel_threshold, const omc_run_opts *opts, omc_csc *dij, omc_csc *var,
+                omc_run_stats *stats);
+
+/* Injected-random-number sampler evaluation for verification: ncase cases of
+ * the golden-vector layout (kinds 1..16: compton, pair, photo, rayleigh,
+ * brems, moller, bhabha, annih, rannih, msdist, mscat, sscat, drange, eloss,
+ * photon, electron), inputs in[ncase*in_stride], draws rn with offsets
+ * rn_off[ncase+1]; outputs out[ncase*out_stride] (see the test suite for the
+ * layout) and rn_used[ncase].  Uses the phantom for geometry lookups. */
+int omc_sampler_eval(omc_ctx *ctx, int phantom_id, int precision, int ncase, const int *kind, const double *in,
+                     int in_stride, const uint32_t *rn, const int64_t *rn_off, double *out, int out_stride,
+                     int64_t *rn_used);
+
+/* ----------------------------------------------------------------- serving */
+/* A resident request loop on the context's device: requests (dose or Dij
+ * over their own phantom) are admitted while their device memory fits the
+ * HBM budget, and the transport of admitted requests shares the device. */
+typedef struct omc_server omc_server;
+
+#define OMC_REQ_DOSE 0
+#define OMC_REQ_DIJ 1
+#define OMC_REQ_QUEUED 0
+#define OMC_REQ_RUNNING 1
+#define OMC_REQ_DONE 2
+#define OMC_REQ_FAILED 3
+
+typedef struct omc_request {
+    int kind;                         /* OMC_REQ_DOSE | OMC_REQ_DIJ */
+    omc_phantom_desc phantom;         /* copied at submit */
+    omc_dosxyz_source_desc dose_src;  /* OMC_REQ_DOSE */
+    int64_t nperbatch;                /* OMC_REQ_DOSE */
+    omc_spectrum_desc spec;           /* OMC_REQ_DIJ */
+    omc_beamlets_desc bix;            /* OMC_REQ_DIJ, copied at submit */
+    double rel_threshold;             /* OMC_REQ_DIJ */
+    int nbatch;
+    uint64_t seed;
+    int precision;
+} omc_request;
+
+typedef struct omc_result {
+    int state;
+    int error;
+    double *sum, *sum2;               /* OMC_REQ_DOSE: nvox+1 each (library-owned until omc_result_free) */
+    int64_t nreg;
+    omc_csc dij;                      /* OMC_REQ_DIJ */
+    double submit_t, start_t, end_t;  /* seconds on the server clock */
+    double histories;
+} omc_result;
+
+int omc_server_create(omc_ctx *ctx, int64_t hbm_budget_bytes, int max_inflight, omc_server **srv);
+int omc_server_submit(omc_server *srv, const omc_request *req, int64_t *req_id);
+int omc_server_poll(omc_server *srv, int64_t req_id, int *state);
+int omc_server_wait(omc_server *srv, int64_t req_id, omc_result *res);
+void omc_result_free(omc_result *res);
+void omc_server_destroy(omc_server *srv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,310 @@
+"""MI355X transport engine for voxelized photon/electron Monte Carlo dose.
+
+Python host mirror of the reference's two user codes over the C ABI in
+include/omc_gpu.h (ompmc_amd/libompmc.so):
+
+* ``Context``  -- physics tables for a set of PEGS4 media, resident on the GPU
+                  (reference initMediaData, src/ompmc.c:5468-5505);
+* ``Context.phantom`` -- a voxel phantom (reference initPhantom/initRegions);
+* ``Context.run_dosxyz`` -- omc_dosxyz dose run (ucodes/omc_dosxyz/omc_dosxyz.c:1072-1307);
+* ``Context.run_dij`` -- omc_matrad dose-influence matrix
+                  (ucodes/omc_matrad/omc_matrad.c:1417-1789);
+* ``read_inp`` / ``DosxyzJob`` -- the .inp front end of omc_dosxyz.
+
+The library must be built (``__graft_entry__.build()``); importing never falls
+back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import phantoms  # noqa: F401  (re-export)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libompmc.so")
+REPO = os.path.dirname(HERE)
+DATA = os.path.join(REPO, "data")
+
+FP32, FP64 = 32, 64
+FORM_HISTORY, FORM_EVENT = 0, 1
+RNG_PHILOX, RNG_REPLAY = 0, 1
+INVDIM = 1000
+
+_d = C.POINTER(C.c_double)
+_i = C.POINTER(C.c_int)
+_i64 = C.POINTER(C.c_int64)
+_u32 = C.POINTER(C.c_uint32)
+
+
+class SpectrumDesc(C.Structure):
+    _fields_ = [("spectrum", C.c_int), ("energy", C.c_double), ("cdfinv1", _d), ("cdfinv2", _d)]
+
+
+class PhantomDesc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("xb", _d), ("yb", _d), ("zb", _d),
+                ("med", _i), ("rho", _d), ("ecut", C.c_double), ("pcut", C.c_double)]
+
+
+class DosxyzSourceDesc(C.Structure):
+    _fields_ = [("spec", SpectrumDesc), ("charge", C.c_int), ("ssd", C.c_double),
+                ("collimator", C.c_double * 4)]
+
+
+class RunOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("form", C.c_int), ("rng", C.c_int), ("seed", C.c_uint64),
+                ("nsplit", C.c_int), ("replay", _u32), ("replay_offsets", _i64), ("replay_len", C.c_int64),
+                ("grid", C.c_int), ("block", C.c_int), ("count_events", C.c_int)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("gpu_seconds", C.c_double), ("wall_seconds", C.c_double), ("histories", C.c_double),
+                ("n_howfar", C.c_uint64), ("n_hownear", C.c_uint64), ("n_cross", C.c_uint64),
+                ("n_dep", C.c_uint64), ("n_rn", C.c_uint64), ("n_photon", C.c_uint64),
+                ("n_electron", C.c_uint64), ("errors", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BeamletsDesc(C.Structure):
+    _fields_ = [("nbixels", C.c_int), ("nbeams", C.c_int), ("ibeam", _i), ("src", _d), ("corner", _d),
+                ("side1", _d), ("side2", _d), ("gaussian", C.c_int), ("sigma", C.c_double),
+                ("nhist", _i64), ("nhist_each", C.c_int64)]
+
+
+class CSC(C.Structure):
+    _fields_ = [("nrows", C.c_int64), ("ncols", C.c_int64), ("nnz", C.c_int64), ("jc", _i64), ("ir", _i64),
+                ("val", _d)]
+
+
+_lib = None
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"omc error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Loads ompmc_amd/libompmc.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.omc_last_error.restype = C.c_char_p
+    L.omc_api_version.restype = C.c_int
+    L.omc_ctx_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(C.c_char_p), C.c_char_p, C.c_char_p,
+                                 C.c_char_p, C.c_int]
+    L.omc_ctx_destroy.argtypes = [vp]
+    L.omc_ctx_table.argtypes = [vp, C.c_char_p, C.POINTER(_d), C.POINTER(C.c_long)]
+    L.omc_phantom_upload.argtypes = [vp, C.POINTER(PhantomDesc), _i]
+    L.omc_phantom_release.argtypes = [vp, C.c_int]
+    L.omc_run_opts_default.argtypes = [C.POINTER(RunOpts)]
+    L.omc_run_dosxyz.argtypes = [vp, C.c_int, C.POINTER(DosxyzSourceDesc), C.c_int64, C.c_int,
+                                 C.POINTER(RunOpts), _d, _d, C.POINTER(RunStats)]
+    L.omc_dosxyz_dose.argtypes = [vp, C.c_int, _d, _d, C.c_int64, C.c_int, _d, _d]
+    L.omc_write_3ddose_file.argtypes = [vp, C.c_int, C.c_char_p, _d, _d]
+    L.omc_sampler_eval.argtypes = [vp, C.c_int, C.c_int, C.c_int, _i, _d, C.c_int, _u32, _i64, _d, C.c_int, _i64]
+    L.omc_run_dij.argtypes = [vp, C.c_int, C.POINTER(SpectrumDesc), C.POINTER(BeamletsDesc), C.c_int, C.c_double,
+                              C.POINTER(RunOpts), C.POINTER(CSC), C.POINTER(CSC), C.POINTER(RunStats)]
+    L.omc_csc_free.argtypes = [C.POINTER(CSC)]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise EngineError(rc, lib().omc_last_error().decode(errors="replace"))
+
+
+def _dp(a):
+    return a.ctypes.data_as(_d)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_i)
+
+
+def default_paths():
+    return (os.path.join(DATA, "pegs4", "521icru.pegs4dat"), os.path.join(DATA, "pegs4", "pgs4form.dat"),
+            DATA + "/")
+
+
+def run_opts(precision=FP32, form=FORM_EVENT, seed=1802, rng=RNG_PHILOX, replay=None, replay_offsets=None,
+             grid=0, block=0, count_events=False, nsplit=1):
+    o = RunOpts()
+    lib().omc_run_opts_default(C.byref(o))
+    o.precision, o.form, o.seed, o.rng, o.nsplit = precision, form, seed, rng, nsplit
+    o.grid, o.block, o.count_events = grid, block, int(bool(count_events))
+    keep = []
+    if replay is not None:
+        rp = np.ascontiguousarray(replay, np.uint32)
+        ro = np.ascontiguousarray(replay_offsets, np.int64)
+        keep = [rp, ro]
+        o.replay = rp.ctypes.data_as(_u32)
+        o.replay_offsets = ro.ctypes.data_as(_i64)
+        o.replay_len = len(rp)
+    return o, keep
+
+
+@dataclass
+class Spectrum:
+    """Mono energy (``cdfinv1 is None``) or the 1000-bin inverse CDF of a
+    spectrum file (reference omc_dosxyz.c:396-510)."""
+    energy: float = 0.0
+    cdfinv1: np.ndarray = None
+    cdfinv2: np.ndarray = None
+
+    @staticmethod
+    def mono(e):
+        return Spectrum(energy=float(e))
+
+    @staticmethod
+    def from_histogram(e_upper, weights, enmin=0.0, imode=0):
+        """Inverse CDF exactly as the reference builds it."""
+        e_upper = np.asarray(e_upper, float)
+        pdf = np.asarray(weights, float).copy()
+        if imode == 1:
+            pdf[0] *= e_upper[0] - enmin
+            pdf[1:] *= e_upper[1:] - e_upper[:-1]
+        cdf = np.cumsum(pdf)
+        cdf = cdf * (1.0 / cdf[-1])
+        c1 = np.zeros(INVDIM)
+        c2 = np.zeros(INVDIM)
+        gridsz = 1.0 / INVDIM
+        for k in range(INVDIM):
+            ak = k * gridsz
+            i = int(np.searchsorted(cdf, ak, side="left"))
+            i = min(i, len(cdf) - 1)
+            c1[k] = e_upper[i - 1] if i != 0 else enmin
+            c2[k] = e_upper[i] - c1[k]
+        return Spectrum(cdfinv1=c1, cdfinv2=c2)
+
+    @staticmethod
+    def read(path):
+        with open(path) as f:
+            f.readline()
+            n, enmin, imode = f.readline().split()[:3]
+            rows = [f.readline().split()[:2] for _ in range(int(n))]
+        e = [float(r[0]) for r in rows]
+        w = [float(r[1]) for r in rows]
+        return Spectrum.from_histogram(e, w, float(enmin), int(imode))
+
+    def desc(self):
+        d = SpectrumDesc()
+        if self.cdfinv1 is None:
+            d.spectrum, d.energy = 0, self.energy
+        else:
+            self._c1 = np.ascontiguousarray(self.cdfinv1, float)
+            self._c2 = np.ascontiguousarray(self.cdfinv2, float)
+            d.spectrum = 1
+            d.cdfinv1, d.cdfinv2 = _dp(self._c1), _dp(self._c2)
+        return d
+
+
+class Phantom:
+    """A phantom resident on the context's device."""
+
+    def __init__(self, ctx, ph: "phantoms.Phantom", ecut=0.7, pcut=0.01):
+        self.ctx, self.ph = ctx, ph
+        self.xb = np.ascontiguousarray(ph.xb, float)
+        self.yb = np.ascontiguousarray(ph.yb, float)
+        self.zb = np.ascontiguousarray(ph.zb, float)
+        self.med = np.ascontiguousarray(ph.med.reshape(-1), np.int32)
+        self.rho = np.ascontiguousarray(ph.rho.reshape(-1), float)
+        d = PhantomDesc(len(self.xb) - 1, len(self.yb) - 1, len(self.zb) - 1, _dp(self.xb), _dp(self.yb),
+                        _dp(self.zb), _ip(self.med), _dp(self.rho), ecut, pcut)
+        pid = C.c_int(0)
+        _check(lib().omc_phantom_upload(ctx.h, C.byref(d), C.byref(pid)))
+        self.id = pid.value
+        self.nreg = self.med.size + 1
+
+    def release(self):
+        if self.id is not None and self.ctx.h:
+            lib().omc_phantom_release(self.ctx.h, self.id)
+            self.id = None
+
+
+class Context:
+    """Physics tables for the given PEGS4 media, uploaded to GPU `device`."""
+
+    def __init__(self, media, pegs_file=None, pgs4form_file=None, data_folder=None, device=0):
+        pf, ff, df = default_paths()
+        arr = (C.c_char_p * len(media))(*[m.encode() for m in media])
+        h = C.c_void_p()
+        _check(lib().omc_ctx_create(C.byref(h), len(media), arr, (pegs_file or pf).encode(),
+                                    (pgs4form_file or ff).encode(), (data_folder or df).encode(), device))
+        self.h = h
+        self.media = list(media)
+
+    def close(self):
+        if self.h:
+            lib().omc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def table(self, name):
+        p = _d()
+        n = C.c_long(0)
+        _check(lib().omc_ctx_table(self.h, name.encode(), C.byref(p), C.byref(n)))
+        return np.ctypeslib.as_array(p, shape=(n.value,)).copy()
+
+    def phantom(self, ph, ecut=0.7, pcut=0.01):
+        return Phantom(self, ph, ecut, pcut)
+
+    def run_dosxyz(self, phantom: Phantom, spectrum: Spectrum, nperbatch, nbatch, collimator=(-5, 5, -5, 5),
+                   ssd=1.0e10, charge=0, **opts):
+        """Returns (sum, sum2, stats): per-region batch sums of deposited
+        energy (MeV, region 0 = outside) and run statistics."""
+        src = DosxyzSourceDesc()
+        src.spec = spectrum.desc()
+        src.charge, src.ssd = charge, ssd
+        for k in range(4):
+            src.collimator[k] = collimator[k]
+        o, keep = run_opts(**opts)
+        s = np.zeros(phantom.nreg)
+        s2 = np.zeros(phantom.nreg)
+        st = RunStats()
+        _check(lib().omc_run_dosxyz(self.h, phantom.id, C.byref(src), nperbatch, nbatch, C.byref(o), _dp(s),
+                                    _dp(s2), C.byref(st)))
+        del keep
+        return s, s2, st
+
+    def dose(self, phantom: Phantom, s, s2, nperbatch, nbatch):
+        nv = phantom.nreg - 1
+        dose, unc = np.zeros(nv), np.zeros(nv)
+        _check(lib().omc_dosxyz_dose(self.h, phantom.id, _dp(np.ascontiguousarray(s)), _dp(np.ascontiguousarray(s2)),
+                                     nperbatch, nbatch, _dp(dose), _dp(unc)))
+        return dose, unc
+
+    def sampler_eval(self, phantom: Phantom, precision, kinds, inputs, rn_lists, in_stride=24, out_stride=128):
+        """Evaluates golden-vector cases with injected random numbers on the
+        GPU.  rn_lists: per case, the draws as floats k*2^-24."""
+        n = len(kinds)
+        kind = np.ascontiguousarray(kinds, np.int32)
+        inp = np.zeros((n, in_stride))
+        for i, v in enumerate(inputs):
+            inp[i, :len(v)] = v
+        off = np.zeros(n + 1, np.int64)
+        off[1:] = np.cumsum([len(r) for r in rn_lists])
+        rn = np.concatenate([np.rint(np.asarray(r) * 16777216.0) for r in rn_lists]).astype(np.uint32) \
+            if off[-1] else np.zeros(1, np.uint32)
+        out = np.zeros((n, out_stride))
+        used = np.zeros(n, np.int64)
+        _check(lib().omc_sampler_eval(self.h, phantom.id, precision, n, _ip(kind), _dp(inp), in_stride,
+                                      rn.ctypes.data_as(_u32), off.ctypes.data_as(_i64), _dp(out), out_stride,
+                                      used.ctypes.data_as(_i64)))
+        return out, used

@@ -1,0 +1,529 @@
+// Host implementation of the C ABI in include/omc_gpu.h: device context,
+// table and phantom upload, dose and sampler entry points.  The Dij and
+// serving entry points live in omc_dij.hip and omc_server.hip.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "omc_capi_internal.hpp"
+
+using namespace omc;
+
+// ------------------------------------------------------------------ errors
+extern "C" const char *omc_last_error(void) { return omc_host_last_error(); }
+extern "C" int omc_api_version(void) { return OMC_API_VERSION; }
+
+int omc_seterr(int code, const char *fmt, ...) {
+    char buf[480];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    return omc_host_set_error(code, "%s", buf);
+}
+
+// ------------------------------------------------------------------ device memory
+void DevArena::release() {
+    for (void *p : blocks) (void)hipFree(p);
+    blocks.clear();
+    bytes = 0;
+}
+
+void *DevArena::alloc(size_t n) {
+    void *p = nullptr;
+    if (hipMalloc(&p, n ? n : 1) != hipSuccess) return nullptr;
+    blocks.push_back(p);
+    bytes += n;
+    return p;
+}
+
+// ------------------------------------------------------------------ tables
+template <class R> static int upload_tables(omc_ctx *c, DevArena &A, Tables<R> &T) {
+    const omc_tables &h = c->host;
+    int nm = h.nmed;
+    T.nmed = nm;
+    std::vector<Med<R>> med(nm);
+    for (int i = 0; i < nm; i++) {
+        Med<R> &m = med[i];
+        memset(&m, 0, sizeof m);
+        m.ge0 = (R)h.ge0[i]; m.ge1 = (R)h.ge1[i]; m.eke0 = (R)h.eke0[i]; m.eke1 = (R)h.eke1[i];
+        m.rho = (R)h.rho[i]; m.te = (R)h.te[i]; m.thmoll = (R)h.thmoll[i]; m.ap = (R)h.ap[i];
+        m.delcm = (R)h.delcm[i]; m.zbrang = (R)h.zbrang[i]; m.bpar0 = (R)h.bpar0[i]; m.bpar1 = (R)h.bpar1[i];
+        m.blcc = (R)h.blcc[i]; m.xcc = (R)h.xcc[i]; m.esig_e = (R)h.esig_e[i]; m.psig_e = (R)h.psig_e[i];
+        m.e_array0 = (R)h.e_array[(size_t)i * OMC_MXEKE];
+        for (int k = 0; k < 8; k++) {
+            m.dl1[k] = (R)h.dl1[8 * i + k]; m.dl2[k] = (R)h.dl2[8 * i + k]; m.dl3[k] = (R)h.dl3[8 * i + k];
+            m.dl4[k] = (R)h.dl4[8 * i + k]; m.dl5[k] = (R)h.dl5[8 * i + k]; m.dl6[k] = (R)h.dl6[8 * i + k];
+        }
+        m.sig_mono[0] = h.sig_ismonotone[i];
+        m.sig_mono[1] = h.sig_ismonotone[nm + i];
+        m.meke = h.meke[i];
+    }
+    T.med = A.upload(med);
+    auto pw = [&](const double *c0, const double *c1, size_t n) {
+        std::vector<PW<R>> v(n);
+        for (size_t k = 0; k < n; k++) { v[k].c1 = (R)c1[k]; v[k].c0 = (R)c0[k]; }
+        return A.upload(v);
+    };
+    auto vec = [&](const double *p, size_t n) {
+        std::vector<R> v(n);
+        for (size_t k = 0; k < n; k++) v[k] = (R)p[k];
+        return A.upload(v);
+    };
+    size_t nge = (size_t)nm * OMC_MXGE, nek = (size_t)nm * OMC_MXEKE;
+    T.gmfp = pw(h.gmfp0, h.gmfp1, nge);
+    T.gbr1 = pw(h.gbr10, h.gbr11, nge);
+    T.gbr2 = pw(h.gbr20, h.gbr21, nge);
+    T.cohe = pw(h.cohe0, h.cohe1, nge);
+    T.pmax = pw(h.ray_pmax0, h.ray_pmax1, nge);
+    // the sampler reads the form-factor CDF of medium 0 only (reference quirk Q1)
+    T.ray_xgrid = vec(h.ray_xgrid, OMC_MXRAYFF);
+    T.ray_fcum = vec(h.ray_fcum, OMC_MXRAYFF);
+    T.ray_b = vec(h.ray_b, OMC_MXRAYFF);
+    T.ray_c = vec(h.ray_c, OMC_MXRAYFF);
+    T.ray_iarr = A.upload(std::vector<int>(h.ray_iarr, h.ray_iarr + OMC_RAYCDFSIZE));
+    T.esig = pw(h.esig0, h.esig1, nek);
+    T.psig = pw(h.psig0, h.psig1, nek);
+    T.ededx = pw(h.ededx0, h.ededx1, nek);
+    T.pdedx = pw(h.pdedx0, h.pdedx1, nek);
+    T.ebr1 = pw(h.ebr10, h.ebr11, nek);
+    T.pbr1 = pw(h.pbr10, h.pbr11, nek);
+    T.pbr2 = pw(h.pbr20, h.pbr21, nek);
+    T.tmxs = pw(h.tmxs0, h.tmxs1, nek);
+    T.blcce = pw(h.blcce0, h.blcce1, nek);
+    T.etae = pw(h.etae_ms0, h.etae_ms1, nek);
+    T.etap = pw(h.etap_ms0, h.etap_ms1, nek);
+    T.q1ce = pw(h.q1ce_ms0, h.q1ce_ms1, nek);
+    T.q1cp = pw(h.q1cp_ms0, h.q1cp_ms1, nek);
+    T.q2ce = pw(h.q2ce_ms0, h.q2ce_ms1, nek);
+    T.q2cp = pw(h.q2cp_ms0, h.q2cp_ms1, nek);
+    T.range_ep = vec(h.range_ep, 2 * nek);
+    T.e_array = vec(h.e_array, nek);
+    T.spin_rej = vec(h.spin_rej, (size_t)nm * OMC_NSPIN_MED);
+    T.ums = vec(h.ums, OMC_NMS);
+    T.fms = vec(h.fms, OMC_NMS);
+    T.wms = vec(h.wms, OMC_NMS);
+    T.ims = A.upload(std::vector<int>(h.ims, h.ims + OMC_NMS));
+    T.b2spin_min = (R)h.b2spin_min; T.dbeta2i = (R)h.dbeta2i; T.espml = (R)h.espml; T.dleneri = (R)h.dleneri;
+    T.dqq1i = (R)h.dqq1i; T.dllambi = (R)h.dllambi; T.dqmsi = (R)h.dqmsi;
+    if (!A.ok) return omc_seterr(OMC_ERR_DEVICE, "table upload failed: %s", hipGetErrorString(hipGetLastError()));
+    return OMC_OK;
+}
+
+extern "C" int omc_ctx_create(omc_ctx **out, int nmed, const char *const *media, const char *pegs_file,
+                              const char *pgs4form_file, const char *data_folder, int device) {
+    if (!out || !media || nmed < 1 || nmed > OMC_MXMED) return omc_seterr(OMC_ERR_ARG, "omc_ctx_create: bad arguments");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return omc_seterr(OMC_ERR_DEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev) return omc_seterr(OMC_ERR_ARG, "device %d of %d", device, ndev);
+    omc_ctx *c = new omc_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return omc_seterr(OMC_ERR_DEVICE, "hipSetDevice(%d)", device); }
+    char names[OMC_MXMED][60];
+    for (int i = 0; i < nmed; i++) snprintf(names[i], 60, "%s", media[i]);
+    omc_paths p = {pegs_file, pgs4form_file, data_folder};
+    int rc = omc_tables_build(&c->host, nmed, (const char(*)[60])names, &p);
+    if (rc) { delete c; return rc; }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        omc_tables_free(&c->host);
+        delete c;
+        return omc_seterr(OMC_ERR_DEVICE, "stream creation failed");
+    }
+    rc = upload_tables<float>(c, c->tabmem, c->t32);
+    if (!rc) rc = upload_tables<double>(c, c->tabmem, c->t64);
+    if (rc) { omc_ctx_destroy(c); return rc; }
+    *out = c;
+    return OMC_OK;
+}
+
+extern "C" void omc_ctx_destroy(omc_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    for (auto &kv : c->phantoms) {
+        kv.second->mem.release();
+        omc_phantom_free(&kv.second->host);
+        delete kv.second;
+    }
+    c->phantoms.clear();
+    c->tabmem.release();
+    c->scratch.release();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    omc_tables_free(&c->host);
+    delete c;
+}
+
+extern "C" int omc_ctx_table(omc_ctx *c, const char *name, const double **data, long *n) {
+    if (!c || !name || !data || !n) return omc_seterr(OMC_ERR_ARG, "omc_ctx_table: bad arguments");
+    const omc_tables &h = c->host;
+    long nm = h.nmed;
+    struct E { const char *nm; const double *p; long n; };
+    const E tab[] = {
+        {"ge0", h.ge0, nm}, {"ge1", h.ge1, nm},
+        {"gmfp0", h.gmfp0, nm * OMC_MXGE}, {"gmfp1", h.gmfp1, nm * OMC_MXGE},
+        {"gbr10", h.gbr10, nm * OMC_MXGE}, {"gbr11", h.gbr11, nm * OMC_MXGE},
+        {"gbr20", h.gbr20, nm * OMC_MXGE}, {"gbr21", h.gbr21, nm * OMC_MXGE},
+        {"cohe0", h.cohe0, nm * OMC_MXGE}, {"cohe1", h.cohe1, nm * OMC_MXGE},
+        {"ray_xgrid", h.ray_xgrid, nm * OMC_MXRAYFF}, {"ray_fcum", h.ray_fcum, nm * OMC_MXRAYFF},
+        {"ray_b", h.ray_b, nm * OMC_MXRAYFF}, {"ray_c", h.ray_c, nm * OMC_MXRAYFF},
+        {"ray_pmax0", h.ray_pmax0, nm * OMC_MXGE}, {"ray_pmax1", h.ray_pmax1, nm * OMC_MXGE},
+        {"dl1", h.dl1, nm * 8}, {"dl2", h.dl2, nm * 8}, {"dl3", h.dl3, nm * 8},
+        {"dl4", h.dl4, nm * 8}, {"dl5", h.dl5, nm * 8}, {"dl6", h.dl6, nm * 8},
+        {"bpar0", h.bpar0, nm}, {"bpar1", h.bpar1, nm}, {"pair_delcm", h.delcm, nm}, {"zbrang", h.zbrang, nm},
+        {"esig0", h.esig0, nm * OMC_MXEKE}, {"esig1", h.esig1, nm * OMC_MXEKE},
+        {"psig0", h.psig0, nm * OMC_MXEKE}, {"psig1", h.psig1, nm * OMC_MXEKE},
+        {"ededx0", h.ededx0, nm * OMC_MXEKE}, {"ededx1", h.ededx1, nm * OMC_MXEKE},
+        {"pdedx0", h.pdedx0, nm * OMC_MXEKE}, {"pdedx1", h.pdedx1, nm * OMC_MXEKE},
+        {"ebr10", h.ebr10, nm * OMC_MXEKE}, {"ebr11", h.ebr11, nm * OMC_MXEKE},
+        {"pbr10", h.pbr10, nm * OMC_MXEKE}, {"pbr11", h.pbr11, nm * OMC_MXEKE},
+        {"pbr20", h.pbr20, nm * OMC_MXEKE}, {"pbr21", h.pbr21, nm * OMC_MXEKE},
+        {"tmxs0", h.tmxs0, nm * OMC_MXEKE}, {"tmxs1", h.tmxs1, nm * OMC_MXEKE},
+        {"blcce0", h.blcce0, nm * OMC_MXEKE}, {"blcce1", h.blcce1, nm * OMC_MXEKE},
+        {"etae_ms0", h.etae_ms0, nm * OMC_MXEKE}, {"etae_ms1", h.etae_ms1, nm * OMC_MXEKE},
+        {"etap_ms0", h.etap_ms0, nm * OMC_MXEKE}, {"etap_ms1", h.etap_ms1, nm * OMC_MXEKE},
+        {"q1ce_ms0", h.q1ce_ms0, nm * OMC_MXEKE}, {"q1ce_ms1", h.q1ce_ms1, nm * OMC_MXEKE},
+        {"q1cp_ms0", h.q1cp_ms0, nm * OMC_MXEKE}, {"q1cp_ms1", h.q1cp_ms1, nm * OMC_MXEKE},
+        {"q2ce_ms0", h.q2ce_ms0, nm * OMC_MXEKE}, {"q2ce_ms1", h.q2ce_ms1, nm * OMC_MXEKE},
+        {"q2cp_ms0", h.q2cp_ms0, nm * OMC_MXEKE}, {"q2cp_ms1", h.q2cp_ms1, nm * OMC_MXEKE},
+        {"e_array", h.e_array, nm * OMC_MXEKE}, {"range_ep", h.range_ep, 2 * nm * OMC_MXEKE},
+        {"eke0", h.eke0, nm}, {"eke1", h.eke1, nm}, {"esig_e", h.esig_e, nm}, {"psig_e", h.psig_e, nm},
+        {"xcc", h.xcc, nm}, {"blcc", h.blcc, nm}, {"expeke1", h.expeke1, nm},
+        {"spin_rej", h.spin_rej, nm * OMC_NSPIN_MED},
+        {"ums", h.ums, OMC_NMS}, {"fms", h.fms, OMC_NMS}, {"wms", h.wms, OMC_NMS},
+        {"rho", h.rho, nm}, {"ae", h.ae, nm}, {"ap", h.ap, nm}, {"ue", h.ue, nm}, {"up", h.up, nm},
+        {"te", h.te, nm}, {"thmoll", h.thmoll, nm}, {"rlc", h.rlc, nm}, {"delcm", h.delcm, nm},
+    };
+    for (const E &e : tab)
+        if (!strcmp(e.nm, name)) { *data = e.p; *n = e.n; return OMC_OK; }
+    return omc_seterr(OMC_ERR_ARG, "no table named %s", name);
+}
+
+// ------------------------------------------------------------------ phantoms
+template <class R> static void fill_geom(omc_ctx *c, PhantomDev *pd, Geom<R> &G) {
+    const omc_phantom &ph = pd->host;
+    const omc_tables &h = c->host;
+    long nvox = (long)ph.nx * ph.ny * ph.nz;
+    G.nx = ph.nx; G.ny = ph.ny; G.nz = ph.nz;
+    auto vec = [&](const double *p, size_t n) {
+        std::vector<R> v(n);
+        for (size_t k = 0; k < n; k++) v[k] = (R)p[k];
+        return pd->mem.upload(v);
+    };
+    G.xb = vec(ph.xb, ph.nx + 1);
+    G.yb = vec(ph.yb, ph.ny + 1);
+    G.zb = vec(ph.zb, ph.nz + 1);
+    std::vector<R> rhof(nvox);
+    for (long v = 0; v < nvox; v++) {
+        int im = ph.med[v] - 1;
+        rhof[v] = im < 0 ? (R)0 : (ph.rho[v] == 0.0F ? (R)1.0 : (R)(ph.rho[v] / h.rho[im]));
+    }
+    G.rhof = pd->mem.upload(rhof);
+    G.med = pd->dmed;
+    for (int m = 0; m < OMC_MXMED; m++) {
+        G.pcut[m] = m < h.nmed ? (R)(h.ap[m] <= pd->pcut ? pd->pcut : h.ap[m]) : (R)0;
+        G.ecut[m] = m < h.nmed ? (R)(h.ae[m] <= pd->ecut ? pd->ecut : h.ae[m]) : (R)0;
+    }
+}
+
+size_t omc_phantom_bytes(const omc_phantom_desc *d) {
+    size_t nvox = (size_t)d->nx * d->ny * d->nz;
+    return nvox * (1 + 4 + 8) + 16 * (size_t)(d->nx + d->ny + d->nz + 3);
+}
+
+int omc_phantom_make(omc_ctx *c, const omc_phantom_desc *d, PhantomDev **out) {
+    if (!c || !d || !d->xb || !d->yb || !d->zb || !d->med || !d->rho || d->nx < 1 || d->ny < 1 || d->nz < 1)
+        return omc_seterr(OMC_ERR_ARG, "omc_phantom_upload: bad descriptor");
+    long nvox = (long)d->nx * d->ny * d->nz;
+    for (long v = 0; v < nvox; v++)
+        if (d->med[v] < 0 || d->med[v] > c->host.nmed)
+            return omc_seterr(OMC_ERR_FORMAT, "voxel %ld has medium %d (context has %d)", v, d->med[v], c->host.nmed);
+    PhantomDev *pd = new PhantomDev();
+    omc_phantom &ph = pd->host;
+    ph.nmed = c->host.nmed;
+    ph.nx = d->nx; ph.ny = d->ny; ph.nz = d->nz;
+    ph.owns = 1;
+    ph.xb = (double *)malloc(sizeof(double) * (d->nx + 1));
+    ph.yb = (double *)malloc(sizeof(double) * (d->ny + 1));
+    ph.zb = (double *)malloc(sizeof(double) * (d->nz + 1));
+    ph.med = (int *)malloc(sizeof(int) * nvox);
+    ph.rho = (double *)malloc(sizeof(double) * nvox);
+    memcpy(ph.xb, d->xb, sizeof(double) * (d->nx + 1));
+    memcpy(ph.yb, d->yb, sizeof(double) * (d->ny + 1));
+    memcpy(ph.zb, d->zb, sizeof(double) * (d->nz + 1));
+    memcpy(ph.med, d->med, sizeof(int) * nvox);
+    memcpy(ph.rho, d->rho, sizeof(double) * nvox);
+    pd->ecut = d->ecut;
+    pd->pcut = d->pcut;
+    pd->nreg = nvox + 1;
+    std::vector<uint8_t> m8(nvox);
+    for (long v = 0; v < nvox; v++) m8[v] = d->med[v] == 0 ? VACUUM : (uint8_t)(d->med[v] - 1);
+    pd->dmed = pd->mem.upload(m8);
+    fill_geom<float>(c, pd, pd->g32);
+    fill_geom<double>(c, pd, pd->g64);
+    if (!pd->mem.ok) {
+        pd->mem.release();
+        omc_phantom_free(&pd->host);
+        delete pd;
+        return omc_seterr(OMC_ERR_NOMEM, "phantom upload failed (%ld voxels)", nvox);
+    }
+    *out = pd;
+    return OMC_OK;
+}
+
+void omc_phantom_destroy(PhantomDev *pd) {
+    if (!pd) return;
+    pd->mem.release();
+    omc_phantom_free(&pd->host);
+    delete pd;
+}
+
+extern "C" int omc_phantom_upload(omc_ctx *c, const omc_phantom_desc *d, int *id) {
+    if (!id) return omc_seterr(OMC_ERR_ARG, "omc_phantom_upload: null id");
+    (void)hipSetDevice(c->device);
+    PhantomDev *pd = nullptr;
+    int rc = omc_phantom_make(c, d, &pd);
+    if (rc) return rc;
+    *id = c->next_id++;
+    c->phantoms[*id] = pd;
+    return OMC_OK;
+}
+
+extern "C" int omc_phantom_release(omc_ctx *c, int id) {
+    auto it = c ? c->phantoms.find(id) : decltype(c->phantoms.end())();
+    if (!c || it == c->phantoms.end()) return omc_seterr(OMC_ERR_ARG, "no phantom %d", id);
+    omc_phantom_destroy(it->second);
+    c->phantoms.erase(it);
+    return OMC_OK;
+}
+
+PhantomDev *omc_get_phantom(omc_ctx *c, int id) {
+    auto it = c->phantoms.find(id);
+    return it == c->phantoms.end() ? nullptr : it->second;
+}
+
+// ------------------------------------------------------------------ runs
+extern "C" void omc_run_opts_default(omc_run_opts *o) {
+    memset(o, 0, sizeof *o);
+    o->precision = OMC_FP32;
+    o->form = OMC_FORM_EVENT;
+    o->rng = OMC_RNG_PHILOX;
+    o->seed = 1802;
+    o->nsplit = 1;
+}
+
+int omc_check_opts(const omc_run_opts *o) {
+    if (!o) return omc_seterr(OMC_ERR_ARG, "null run options");
+    if (o->precision != OMC_FP32 && o->precision != OMC_FP64) return omc_seterr(OMC_ERR_ARG, "precision %d", o->precision);
+    if (o->nsplit != 1)
+        return omc_seterr(OMC_ERR_ARG, "nsplit %d: the GPU transport supports photon splitting factor 1 only", o->nsplit);
+    if (o->form != OMC_FORM_HISTORY && o->form != OMC_FORM_EVENT) return omc_seterr(OMC_ERR_ARG, "form %d", o->form);
+    if (o->rng == OMC_RNG_REPLAY && (!o->replay || !o->replay_offsets))
+        return omc_seterr(OMC_ERR_ARG, "replay random numbers requested without a stream");
+    return OMC_OK;
+}
+
+template <class R> static void fill_spectrum(DevArena &A, const omc_spectrum_desc &sp, int &spectrum, R &energy,
+                                             R &deltak, const R *&c1, const R *&c2) {
+    spectrum = sp.spectrum;
+    energy = (R)sp.energy;
+    deltak = (R)OMC_INVDIM;
+    c1 = c2 = nullptr;
+    if (sp.spectrum) {
+        std::vector<R> a(OMC_INVDIM), b(OMC_INVDIM);
+        for (int k = 0; k < OMC_INVDIM; k++) { a[k] = (R)sp.cdfinv1[k]; b[k] = (R)sp.cdfinv2[k]; }
+        c1 = A.upload(a);
+        c2 = A.upload(b);
+    }
+}
+
+template <class R> static int make_dosxyz_src(DevArena &A, const PhantomDev *pd, const omc_dosxyz_source_desc *src,
+                                              SrcDosxyz<R> &S) {
+    omc_dosxyz_source hs;
+    memset(&hs, 0, sizeof hs);
+    int rc = omc_dosxyz_source_init(&hs, &pd->host, src->collimator, src->ssd, src->charge);
+    if (rc) return rc;
+    if (src->spec.spectrum && (!src->spec.cdfinv1 || !src->spec.cdfinv2))
+        return omc_seterr(OMC_ERR_ARG, "spectrum source without inverse CDF");
+    fill_spectrum<R>(A, src->spec, S.spectrum, S.energy, S.deltak, S.cdfinv1, S.cdfinv2);
+    S.charge = hs.charge;
+    S.ssd = (R)hs.ssd;
+    S.xinl = (R)hs.xinl;
+    S.yinl = (R)hs.yinl;
+    S.xsize = (R)hs.xsize;
+    S.ysize = (R)hs.ysize;
+    S.ixinl = hs.ixinl;
+    S.iyinl = hs.iyinl;
+    return OMC_OK;
+}
+
+#define HIPCHK(x)                                                                                         \
+    do {                                                                                                  \
+        hipError_t e_ = (x);                                                                              \
+        if (e_ != hipSuccess) { rc = omc_seterr(OMC_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); goto done; } \
+    } while (0)
+
+template <class R>
+static int run_dosxyz_history(omc_ctx *c, PhantomDev *pd, const omc_dosxyz_source_desc *src, int64_t nper, int nbatch,
+                              const omc_run_opts *o, double *sum, double *sum2, omc_run_stats *st) {
+    int rc = OMC_OK;
+    DevArena A;
+    HistArgs<R> a;
+    memset(&a, 0, sizeof a);
+    a.t = tables_of<R>(c);
+    a.g = geom_of<R>(pd);
+    long nreg = pd->nreg;
+    double *d_e = A.alloc_n<double>(nreg), *d_s = A.alloc_n<double>(nreg), *d_s2 = A.alloc_n<double>(nreg);
+    unsigned long long *d_cnt = A.alloc_n<unsigned long long>(8), *d_next = A.alloc_n<unsigned long long>(1);
+    int *d_err = A.alloc_n<int>(4);
+    uint32_t *d_rn = nullptr;
+    long *d_off = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float ms = 0.f;
+    auto wall0 = std::chrono::steady_clock::now();
+    if (!A.ok) { rc = omc_seterr(OMC_ERR_NOMEM, "device allocation failed"); goto done; }
+    rc = make_dosxyz_src<R>(A, pd, src, a.dos);
+    if (rc) goto done;
+    if (o->rng == OMC_RNG_REPLAY) {
+        long nh = (long)nper * nbatch;
+        d_rn = A.alloc_n<uint32_t>(o->replay_len);
+        d_off = A.alloc_n<long>(nh + 1);
+        if (!A.ok) { rc = omc_seterr(OMC_ERR_NOMEM, "replay stream allocation failed"); goto done; }
+        HIPCHK(hipMemcpy(d_rn, o->replay, sizeof(uint32_t) * o->replay_len, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_off, o->replay_offsets, sizeof(long) * (nh + 1), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemsetAsync(d_e, 0, sizeof(double) * nreg, c->stream));
+    HIPCHK(hipMemsetAsync(d_s, 0, sizeof(double) * nreg, c->stream));
+    HIPCHK(hipMemsetAsync(d_s2, 0, sizeof(double) * nreg, c->stream));
+    HIPCHK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long) * 8, c->stream));
+    HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int) * 4, c->stream));
+    a.endep = d_e;
+    a.counters = d_cnt;
+    a.err = d_err;
+    a.next = d_next;
+    a.seed = o->seed;
+    a.replay = d_rn;
+    a.chunk = 1;
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+    HIPCHK(hipEventRecord(ev0, c->stream));
+    for (int b = 0; b < nbatch; b++) {
+        a.nhist = nper;
+        a.hist_base = (long)b * nper;
+        a.replay_off = d_off ? d_off + (long)b * nper : nullptr;
+        HIPCHK(hipMemsetAsync(d_next, 0, sizeof(unsigned long long), c->stream));
+        int block = o->block > 0 ? o->block : 256;
+        long want = (nper + block - 1) / block;
+        int grid = o->grid > 0 ? o->grid : (int)(want < 2048 ? want : 2048);
+        HIPCHK(launch_history<R>(a, o->rng == OMC_RNG_REPLAY ? RNG_REPLAY : RNG_PHILOX, o->count_events != 0, grid,
+                                 block, c->stream));
+        HIPCHK(launch_accumulate(d_e, d_s, d_s2, nreg, 1.0, c->stream));
+    }
+    HIPCHK(hipEventRecord(ev1, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+    HIPCHK(hipMemcpy(sum, d_s, sizeof(double) * nreg, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sum2, d_s2, sizeof(double) * nreg, hipMemcpyDeviceToHost));
+    if (st) {
+        unsigned long long cnt[8];
+        int err[4];
+        HIPCHK(hipMemcpy(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(err, d_err, sizeof err, hipMemcpyDeviceToHost));
+        st->gpu_seconds = ms * 1e-3;
+        st->histories = (double)nper * nbatch;
+        st->n_howfar = cnt[0]; st->n_hownear = cnt[1]; st->n_cross = cnt[2]; st->n_dep = cnt[3];
+        st->n_rn = cnt[4]; st->n_photon = cnt[5]; st->n_electron = cnt[6];
+        st->errors = (err[0] ? 1 : 0) | (err[1] ? 2 : 0) | (err[2] ? 4 : 0);
+        if (err[0]) rc = omc_seterr(OMC_ERR_STACK, "particle stack overflow on the device");
+    }
+done:
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    A.release();
+    if (st) st->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
+    return rc;
+}
+
+extern "C" int omc_run_dosxyz(omc_ctx *c, int phantom_id, const omc_dosxyz_source_desc *src, int64_t nperbatch,
+                              int nbatch, const omc_run_opts *o, double *sum, double *sum2, omc_run_stats *st) {
+    if (!c || !src || !sum || !sum2 || nperbatch < 1 || nbatch < 1) return omc_seterr(OMC_ERR_ARG, "omc_run_dosxyz: bad arguments");
+    int rc = omc_check_opts(o);
+    if (rc) return rc;
+    PhantomDev *pd = omc_get_phantom(c, phantom_id);
+    if (!pd) return omc_seterr(OMC_ERR_ARG, "no phantom %d", phantom_id);
+    (void)hipSetDevice(c->device);
+    if (st) memset(st, 0, sizeof *st);
+    if (o->form == OMC_FORM_EVENT)
+        return o->precision == OMC_FP32 ? omc_run_dosxyz_event<float>(c, pd, src, nperbatch, nbatch, o, sum, sum2, st)
+                                        : omc_run_dosxyz_event<double>(c, pd, src, nperbatch, nbatch, o, sum, sum2, st);
+    return o->precision == OMC_FP32 ? run_dosxyz_history<float>(c, pd, src, nperbatch, nbatch, o, sum, sum2, st)
+                                    : run_dosxyz_history<double>(c, pd, src, nperbatch, nbatch, o, sum, sum2, st);
+}
+
+extern "C" int omc_dosxyz_dose(omc_ctx *c, int phantom_id, const double *sum, const double *sum2, int64_t nperbatch,
+                               int nbatch, double *dose, double *unc) {
+    PhantomDev *pd = c ? omc_get_phantom(c, phantom_id) : nullptr;
+    if (!pd || !sum || !sum2 || !dose || !unc) return omc_seterr(OMC_ERR_ARG, "omc_dosxyz_dose: bad arguments");
+    omc_dosxyz_finish(&pd->host, sum, sum2, (int)nperbatch, nbatch, dose, unc);
+    return OMC_OK;
+}
+
+extern "C" int omc_write_3ddose_file(omc_ctx *c, int phantom_id, const char *path, const double *dose, const double *unc) {
+    PhantomDev *pd = c ? omc_get_phantom(c, phantom_id) : nullptr;
+    if (!pd || !path) return omc_seterr(OMC_ERR_ARG, "omc_write_3ddose_file: bad arguments");
+    return omc_write_3ddose(path, &pd->host, dose, unc);
+}
+
+// ------------------------------------------------------------------ sampler parity
+template <class R>
+static int sampler_eval(omc_ctx *c, PhantomDev *pd, int ncase, const int *kind, const double *in, int in_stride,
+                        const uint32_t *rn, const int64_t *rn_off, double *out, int out_stride, int64_t *rn_used) {
+    int rc = OMC_OK;
+    DevArena A;
+    SampArgs<R> a;
+    memset(&a, 0, sizeof a);
+    a.t = tables_of<R>(c);
+    a.g = geom_of<R>(pd);
+    a.ncase = ncase;
+    long nrn = rn_off[ncase];
+    int *d_kind = A.alloc_n<int>(ncase);
+    double *d_in = A.alloc_n<double>((size_t)ncase * in_stride);
+    uint32_t *d_rn = A.alloc_n<uint32_t>(nrn > 0 ? nrn : 1);
+    long *d_off = A.alloc_n<long>(ncase + 1);
+    double *d_out = A.alloc_n<double>((size_t)ncase * out_stride);
+    long *d_used = A.alloc_n<long>(ncase);
+    int *d_err = A.alloc_n<int>(1);
+    if (!A.ok) { rc = omc_seterr(OMC_ERR_NOMEM, "sampler buffers"); goto done; }
+    HIPCHK(hipMemcpy(d_kind, kind, sizeof(int) * ncase, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_in, in, sizeof(double) * ncase * in_stride, hipMemcpyHostToDevice));
+    if (nrn > 0) HIPCHK(hipMemcpy(d_rn, rn, sizeof(uint32_t) * nrn, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_off, rn_off, sizeof(long) * (ncase + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(d_out, 0, sizeof(double) * ncase * out_stride));
+    HIPCHK(hipMemset(d_err, 0, sizeof(int)));
+    a.kind = d_kind; a.in = d_in; a.in_stride = in_stride; a.rn = d_rn; a.rn_off = d_off;
+    a.out = d_out; a.out_stride = out_stride; a.rn_used = d_used; a.err = d_err;
+    HIPCHK(launch_sampler<R>(a, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(out, d_out, sizeof(double) * ncase * out_stride, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rn_used, d_used, sizeof(long) * ncase, hipMemcpyDeviceToHost));
+done:
+    A.release();
+    return rc;
+}
+
+extern "C" int omc_sampler_eval(omc_ctx *c, int phantom_id, int precision, int ncase, const int *kind, const double *in,
+                                int in_stride, const uint32_t *rn, const int64_t *rn_off, double *out, int out_stride,
+                                int64_t *rn_used) {
+    PhantomDev *pd = c ? omc_get_phantom(c, phantom_id) : nullptr;
+    if (!pd || ncase < 1 || !kind || !in || !rn_off || !out || !rn_used || in_stride < 17 || out_stride < 64)
+        return omc_seterr(OMC_ERR_ARG, "omc_sampler_eval: bad arguments");
+    (void)hipSetDevice(c->device);
+    return precision == OMC_FP64 ? sampler_eval<double>(c, pd, ncase, kind, in, in_stride, rn, rn_off, out, out_stride, rn_used)
+                                 : sampler_eval<float>(c, pd, ncase, kind, in, in_stride, rn, rn_off, out, out_stride, rn_used);
+}

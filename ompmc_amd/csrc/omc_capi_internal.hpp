@@ -1,0 +1,72 @@
+// Internal host-side types shared by the C ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <map>
+#include <vector>
+
+#include "../../include/omc_gpu.h"
+#include "omc_host.h"
+#include "omc_launch.hpp"
+
+// Device allocations owned together and released together.
+struct DevArena {
+    std::vector<void *> blocks;
+    size_t bytes = 0;
+    bool ok = true;
+    void *alloc(size_t n);
+    template <class T> T *alloc_n(size_t n) {
+        T *p = (T *)alloc(sizeof(T) * n);
+        if (!p) ok = false;
+        return p;
+    }
+    template <class T> const T *upload(const std::vector<T> &v) {
+        T *p = alloc_n<T>(v.size() ? v.size() : 1);
+        if (!p) return nullptr;
+        if (!v.empty() && hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice) != hipSuccess) ok = false;
+        return p;
+    }
+    void release();
+};
+
+struct PhantomDev {
+    omc_phantom host;            // owned copy (dose conversion, source set-up)
+    double ecut = 0, pcut = 0;
+    long nreg = 0;               // voxels + 1
+    DevArena mem;
+    const uint8_t *dmed = nullptr;
+    omc::Geom<float> g32;
+    omc::Geom<double> g64;
+};
+
+struct omc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    omc_tables host;
+    DevArena tabmem;
+    DevArena scratch;
+    omc::Tables<float> t32;
+    omc::Tables<double> t64;
+    std::map<int, PhantomDev *> phantoms;
+    int next_id = 1;
+};
+
+template <class R> const omc::Tables<R> &tables_of(const omc_ctx *c);
+template <> inline const omc::Tables<float> &tables_of<float>(const omc_ctx *c) { return c->t32; }
+template <> inline const omc::Tables<double> &tables_of<double>(const omc_ctx *c) { return c->t64; }
+template <class R> const omc::Geom<R> &geom_of(const PhantomDev *p);
+template <> inline const omc::Geom<float> &geom_of<float>(const PhantomDev *p) { return p->g32; }
+template <> inline const omc::Geom<double> &geom_of<double>(const PhantomDev *p) { return p->g64; }
+
+int omc_seterr(int code, const char *fmt, ...);
+int omc_check_opts(const omc_run_opts *o);
+PhantomDev *omc_get_phantom(omc_ctx *c, int id);
+int omc_phantom_make(omc_ctx *c, const omc_phantom_desc *d, PhantomDev **out);
+void omc_phantom_destroy(PhantomDev *pd);
+size_t omc_phantom_bytes(const omc_phantom_desc *d);
+
+// event-queue form (omc_event.hip)
+template <class R>
+int omc_run_dosxyz_event(omc_ctx *c, PhantomDev *pd, const omc_dosxyz_source_desc *src, int64_t nper, int nbatch,
+                         const omc_run_opts *o, double *sum, double *sum2, omc_run_stats *st);

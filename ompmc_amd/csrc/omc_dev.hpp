@@ -1,0 +1,1463 @@
+// Device-side transport physics for gfx950, shared by every kernel form.
+//
+// Everything is templated on the transport precision R (float: production,
+// double: the FP64 GPU oracle run) and on the random-number source G, so the
+// plain history kernel, the event-queue kernels and the sampler-parity kernels
+// all execute the same physics.  The control flow of each function follows
+// the reference routine it names (src/ompmc.c, ucodes/omc_dosxyz/omc_dosxyz.c,
+// ucodes/omc_matrad/omc_matrad.c) so that, fed the same random numbers, the
+// FP64 instantiation consumes them in the same order and reproduces the
+// reference's histories.
+//
+// Differences in representation (not in results):
+//  * the voxel region is carried as (ix, iy, iz) instead of a linear region
+//    number, so howfar/hownear need no integer division; ix < 0 means
+//    "outside the phantom" (the reference's region 0);
+//  * the perpendicular boundary distance `dnear` is written but never read by
+//    the reference, so it is not carried;
+//  * photon() returns to the caller after each interaction instead of looping
+//    over the next photon on the stack (equivalent for nsplit = 1, the only
+//    splitting factor the GPU paths accept);
+//  * the particle being transported and the one secondary an interaction
+//    creates live in registers: every routine works on (p, q) and returns the
+//    change of stack depth, -1 (p finished), 0 (p continues) or +1 (q is the
+//    new top, p lies below it), which is exactly the reference's LIFO order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace omc {
+
+constexpr int MXGE = 2000;
+constexpr int MXEKE = 500;
+constexpr int MXRAYFF = 100;
+constexpr int MXE_SPIN = 15;
+constexpr int MXE_SPIN1 = 2 * MXE_SPIN + 1;
+constexpr int MXQ_SPIN = 15;
+constexpr int MXU_SPIN = 31;
+constexpr int NSPIN_MED = 2 * (MXE_SPIN1 + 1) * (MXQ_SPIN + 1) * (MXU_SPIN + 1);
+constexpr int MXL_MS = 63;
+constexpr int MXQ_MS = 7;
+constexpr int MXU_MS = 31;
+constexpr int MXMED = 9;
+constexpr uint8_t VACUUM = 255;
+
+// ------------------------------------------------------------------ math
+template <class R> struct M;
+template <> struct M<double> {
+    static __device__ __forceinline__ double log(double x) { return ::log(x); }
+    static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
+    static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+    static __device__ __forceinline__ double sin(double x) { return ::sin(x); }
+    static __device__ __forceinline__ double cos(double x) { return ::cos(x); }
+    static __device__ __forceinline__ double fmax(double a, double b) { return ::fmax(a, b); }
+    static __device__ __forceinline__ double fmin(double a, double b) { return ::fmin(a, b); }
+    static __device__ __forceinline__ double fabs(double a) { return ::fabs(a); }
+    // x^3 and x^4 through pow(): repeated products round differently in a
+    // large share of cases, and the FP64 path must round as the reference does
+    static __device__ __forceinline__ double pow3(double x) { return ::pow(x, 3.0); }
+    static __device__ __forceinline__ double pow4(double x) { return ::pow(x, 4.0); }
+};
+template <> struct M<float> {
+    static __device__ __forceinline__ float log(float x) { return ::logf(x); }
+    static __device__ __forceinline__ float exp(float x) { return ::expf(x); }
+    static __device__ __forceinline__ float sqrt(float x) { return ::sqrtf(x); }
+    static __device__ __forceinline__ float sin(float x) { return ::sinf(x); }
+    static __device__ __forceinline__ float cos(float x) { return ::cosf(x); }
+    static __device__ __forceinline__ float fmax(float a, float b) { return ::fmaxf(a, b); }
+    static __device__ __forceinline__ float fmin(float a, float b) { return ::fminf(a, b); }
+    static __device__ __forceinline__ float fabs(float a) { return ::fabsf(a); }
+    static __device__ __forceinline__ float pow3(float x) { return x * x * x; }
+    static __device__ __forceinline__ float pow4(float x) { float y = x * x; return y * y; }
+};
+
+// ------------------------------------------------------------------ tables
+// Piecewise-linear function of a log energy: value = lv * c1 + c0, one
+// interleaved pair per (medium, interval) so one 8/16-byte load serves a lookup.
+template <class R> struct PW { R c1, c0; };
+
+template <class R> struct Med {
+    R ge0, ge1, eke0, eke1;
+    R rho, te, thmoll, ap, delcm, zbrang, bpar0, bpar1;
+    R blcc, xcc, esig_e, psig_e;
+    R e_array0;                    // first energy of the electron grid
+    R dl1[8], dl2[8], dl3[8], dl4[8], dl5[8], dl6[8];
+    int sig_mono[2];
+    int meke;
+    int pad;
+};
+
+template <class R> struct Tables {
+    int nmed;
+    const Med<R> *med;
+    const PW<R> *gmfp, *gbr1, *gbr2, *cohe, *pmax;              // [nmed*MXGE]
+    const R *ray_xgrid, *ray_fcum, *ray_b, *ray_c;              // medium 0 only (reference quirk Q1)
+    const int *ray_iarr;
+    const PW<R> *esig, *psig, *ededx, *pdedx, *ebr1, *pbr1, *pbr2, *tmxs;
+    const PW<R> *blcce, *etae, *etap, *q1ce, *q1cp, *q2ce, *q2cp; // [nmed*MXEKE]
+    const R *range_ep;                                          // [2][nmed][MXEKE]
+    const R *e_array;                                           // [nmed][MXEKE]
+    const R *spin_rej;                                          // [nmed][NSPIN_MED]
+    const R *ums, *fms, *wms;
+    const int *ims;
+    R b2spin_min, dbeta2i, espml, dleneri, dqq1i, dllambi, dqmsi;
+};
+
+// Rectilinear voxel phantom resident in HBM: boundaries plus one byte of
+// medium and one word of density ratio per voxel (x fastest).
+template <class R> struct Geom {
+    int nx, ny, nz;
+    const R *xb, *yb, *zb;
+    const uint8_t *med;
+    const R *rhof;
+    R pcut[MXMED], ecut[MXMED];    // region cuts per medium: global cut, or the PEGS AP/AE when higher
+};
+
+// ------------------------------------------------------------------ particles
+template <class R> struct Part {
+    R e, x, y, z, u, v, w, wt;
+    int iq;
+    int ix, iy, iz;                // ix < 0: outside the phantom
+};
+
+
+
+// ------------------------------------------------------------------ random numbers
+// All sources return k * 2^-24 with k a 24-bit integer, the lattice of the
+// reference's RANMAR generator (0 possible, 1 excluded).
+struct Philox {
+    // Philox4x32-10 (Salmon et al., SC'11): counter-based, keyed per stream.
+    uint32_t k0, k1, c0, c1, c2, c3;
+    uint32_t buf[4];
+    int pos;
+    __device__ __forceinline__ void init(uint64_t seed, uint64_t stream) {
+        k0 = (uint32_t)seed; k1 = (uint32_t)(seed >> 32);
+        c0 = (uint32_t)stream; c1 = (uint32_t)(stream >> 32); c2 = 0; c3 = 0;
+        pos = 4;
+    }
+    __device__ __forceinline__ void round4() {
+        uint32_t x0 = c0, x1 = c1, x2 = c2, x3 = c3, a = k0, b = k1;
+#pragma unroll
+        for (int r = 0; r < 10; r++) {
+            uint32_t hi0 = __umulhi(0xD2511F53u, x0), lo0 = 0xD2511F53u * x0;
+            uint32_t hi1 = __umulhi(0xCD9E8D57u, x2), lo1 = 0xCD9E8D57u * x2;
+            uint32_t y0 = hi1 ^ x1 ^ a, y1 = lo1, y2 = hi0 ^ x3 ^ b, y3 = lo0;
+            x0 = y0; x1 = y1; x2 = y2; x3 = y3;
+            a += 0x9E3779B9u; b += 0xBB67AE85u;
+        }
+        buf[0] = x0; buf[1] = x1; buf[2] = x2; buf[3] = x3;
+        if (++c2 == 0) ++c3;
+        pos = 0;
+    }
+    template <class R> __device__ __forceinline__ R next() {
+        if (pos >= 4) round4();
+        return (R)(buf[pos++] >> 8) * (R)(1.0 / 16777216.0);
+    }
+};
+
+// Replays a precomputed stream of 24-bit integers (RANMAR output, or any
+// injected list).  Running past the end sets *err and returns 0.5.
+struct Replay {
+    const uint32_t *base;
+    long pos, end;
+    int *err;
+    template <class R> __device__ __forceinline__ R next() {
+        if (pos >= end) { if (err) *err = 1; return (R)0.5; }
+        return (R)base[pos++] * (R)(1.0 / 16777216.0);
+    }
+};
+
+// ------------------------------------------------------------------ tallies
+// Energy deposition: wt*edep is summed in a register while consecutive
+// deposits hit the same region, and flushed with one FP64 atomic on change.
+struct TallyNone {
+    __device__ __forceinline__ void add(int, double) {}
+    __device__ __forceinline__ void flush() {}
+};
+
+struct TallyAtomic {
+    double *endep;       // [nreg] with region 0 = outside
+    int cur;
+    double acc;
+    __device__ __forceinline__ void init(double *p) { endep = p; cur = -1; acc = 0.0; }
+    __device__ __forceinline__ void add(int region, double v) {
+        if (region != cur) {
+            if (cur >= 0 && acc != 0.0) atomicAdd(endep + cur, acc);
+            cur = region;
+            acc = v;
+        } else {
+            acc += v;
+        }
+    }
+    __device__ __forceinline__ void flush() {
+        if (cur >= 0 && acc != 0.0) atomicAdd(endep + cur, acc);
+        cur = -1;
+        acc = 0.0;
+    }
+};
+
+// Per-lane event counters feeding the algorithmic-bytes roofline.
+struct Counters {
+    unsigned long long nhowfar, nhownear, ncross, ndep, nrn, nphoton, nelectron;
+};
+
+// ------------------------------------------------------------------ transport engine
+template <class R, class G, class T> struct Engine {
+    const Tables<R> &t;
+    const Geom<R> &g;
+    G &rng;
+    T &tally;
+    Counters cnt;
+    bool count;
+
+    __device__ __forceinline__ R rnd() {
+        if (count) cnt.nrn++;
+        return rng.template next<R>();
+    }
+
+    static __device__ __forceinline__ int pwlf_i(R lv, R c1, R c0) { return (int)(lv * c1 + c0); }
+    static __device__ __forceinline__ R pwlf(const PW<R> &q, R lv) { return lv * q.c1 + q.c0; }
+
+    __device__ __forceinline__ int region(const Part<R> &p) const {
+        return p.ix < 0 ? 0 : 1 + p.ix + g.nx * (p.iy + g.ny * p.iz);
+    }
+    __device__ __forceinline__ int medium(const Part<R> &p) const {
+        if (p.ix < 0) return -1;
+        uint8_t m = g.med[p.ix + g.nx * (p.iy + g.ny * p.iz)];
+        return m == VACUUM ? -1 : (int)m;
+    }
+    __device__ __forceinline__ R rhof_of(const Part<R> &p) const {
+        return p.ix < 0 ? (R)0 : g.rhof[p.ix + g.nx * (p.iy + g.ny * p.iz)];
+    }
+    __device__ __forceinline__ R pcut_of(int imed) const { return imed < 0 ? (R)0 : g.pcut[imed]; }
+    __device__ __forceinline__ R ecut_of(int imed) const { return imed < 0 ? (R)0 : g.ecut[imed]; }
+
+    // ausgab (omc_dosxyz.c:683-694)
+    __device__ __forceinline__ void ausgab(const Part<R> &p, R edep) {
+        if (count) cnt.ndep++;
+        tally.add(region(p), (double)(p.wt * edep));
+    }
+
+    // howfar (omc_dosxyz.c:187-297): distance to the next voxel plane along z,
+    // then x, then y with strict '<' (ties go to the first axis tested).
+    // Returns the move: 0 none, +-1 x, +-2 y, +-3 z; *leaves when the new
+    // region is outside.  The caller has checked p.ix >= 0.
+    __device__ __forceinline__ int howfar(const Part<R> &p, R &ustep, bool &leaves) {
+        if (count) cnt.nhowfar++;
+        int mv = 0;
+        R dist;
+        leaves = false;
+        if (p.w > (R)0) {
+            dist = (g.zb[p.iz + 1] - p.z) / p.w;
+            if (dist < ustep) { ustep = dist; mv = 3; leaves = p.iz == g.nz - 1; }
+        } else if (p.w < (R)0) {
+            dist = -(p.z - g.zb[p.iz]) / p.w;
+            if (dist < ustep) { ustep = dist; mv = -3; leaves = p.iz == 0; }
+        }
+        if (p.u > (R)0) {
+            dist = (g.xb[p.ix + 1] - p.x) / p.u;
+            if (dist < ustep) { ustep = dist; mv = 1; leaves = p.ix == g.nx - 1; }
+        } else if (p.u < (R)0) {
+            dist = -(p.x - g.xb[p.ix]) / p.u;
+            if (dist < ustep) { ustep = dist; mv = -1; leaves = p.ix == 0; }
+        }
+        if (p.v > (R)0) {
+            dist = (g.yb[p.iy + 1] - p.y) / p.v;
+            if (dist < ustep) { ustep = dist; mv = 2; leaves = p.iy == g.ny - 1; }
+        } else if (p.v < (R)0) {
+            dist = -(p.y - g.yb[p.iy]) / p.v;
+            if (dist < ustep) { ustep = dist; mv = -2; leaves = p.iy == 0; }
+        }
+        return mv;
+    }
+
+    __device__ __forceinline__ void apply_move(Part<R> &p, int mv, bool leaves) {
+        if (count) cnt.ncross++;
+        if (leaves) { p.ix = -1; return; }
+        if (mv == 1) p.ix++; else if (mv == -1) p.ix--;
+        else if (mv == 2) p.iy++; else if (mv == -2) p.iy--;
+        else if (mv == 3) p.iz++; else if (mv == -3) p.iz--;
+    }
+
+    // hownear (omc_dosxyz.c:299-334)
+    __device__ __forceinline__ R hownear(const Part<R> &p) {
+        if (count) cnt.nhownear++;
+        if (p.ix < 0) return (R)0;
+        R tp = (R)1.0E10;
+        tp = M<R>::fmin(tp, g.xb[p.ix + 1] - p.x);
+        tp = M<R>::fmin(tp, p.x - g.xb[p.ix]);
+        tp = M<R>::fmin(tp, g.yb[p.iy + 1] - p.y);
+        tp = M<R>::fmin(tp, p.y - g.yb[p.iy]);
+        tp = M<R>::fmin(tp, g.zb[p.iz + 1] - p.z);
+        tp = M<R>::fmin(tp, p.z - g.zb[p.iz]);
+        return tp;
+    }
+
+    // ---------------------------------------------------------------- kinematics
+    struct Uphi { R A, B, C, cosphi, sinphi; };
+
+    // box-rejection azimuth (src/ompmc.c:102-122)
+    __device__ __forceinline__ void azimuth(R &c, R &sn) {
+        R xphi, xphi2, yphi, yphi2, r2;
+        do {
+            xphi = rnd();
+            xphi = (R)2.0 * xphi - (R)1.0;
+            xphi2 = xphi * xphi;
+            yphi = rnd();
+            yphi2 = yphi * yphi;
+            r2 = xphi2 + yphi2;
+        } while (r2 > (R)1.0);
+        r2 = (R)1 / r2;
+        c = (xphi2 - yphi2) * r2;
+        sn = (R)2.0 * xphi * yphi * r2;
+    }
+
+    __device__ __forceinline__ static void rotate(Part<R> &p, const Uphi &q, R costhe, R sinthe) {
+        R sinps2 = q.A * q.A + q.B * q.B;
+        if (sinps2 < (R)1.0E-20) {
+            p.u = sinthe * q.cosphi;
+            p.v = sinthe * q.sinphi;
+            p.w = q.C * costhe;
+        } else {
+            R sinpsi = M<R>::sqrt(sinps2);
+            R us = sinthe * q.cosphi, vs = sinthe * q.sinphi;
+            R sindel = q.B / sinpsi, cosdel = q.A / sinpsi;
+            p.u = q.C * cosdel * us - sindel * vs + q.A * costhe;
+            p.v = q.C * sindel * us + cosdel * vs + q.B * costhe;
+            p.w = -sinpsi * us + q.C * costhe;
+        }
+    }
+
+    // uphi21 (src/ompmc.c:128-165)
+    __device__ __forceinline__ void uphi21(Part<R> &p, Uphi &q, R costhe, R sinthe) {
+        azimuth(q.cosphi, q.sinphi);
+        q.A = p.u; q.B = p.v; q.C = p.w;
+        rotate(p, q, costhe, sinthe);
+    }
+
+    // uphi32 (src/ompmc.c:167-202): the new particle inherits position,
+    // region and weight of the one below it
+    __device__ __forceinline__ static void uphi32(Part<R> &p, const Part<R> &below, const Uphi &q, R costhe, R sinthe) {
+        p.x = below.x; p.y = below.y; p.z = below.z;
+        p.u = below.u; p.v = below.v; p.w = below.w;
+        p.ix = below.ix; p.iy = below.iy; p.iz = below.iz; p.wt = below.wt;
+        rotate(p, q, costhe, sinthe);
+    }
+
+    __device__ __forceinline__ static void transfer(Part<R> &p, const Part<R> &from) {
+        p.x = from.x; p.y = from.y; p.z = from.z;
+        p.u = from.u; p.v = from.v; p.w = from.w;
+        p.ix = from.ix; p.iy = from.iy; p.iz = from.iz; p.wt = from.wt;
+    }
+
+    // ---------------------------------------------------------------- photon interactions
+    // Rayleigh (src/ompmc.c:1120-1163): form-factor CDF of medium 0 with bin
+    // index 0, as the reference reads them (quirk Q1)
+    __device__ void rayleigh(Part<R> &p, int imed, R eig, R gle, int lgle) {
+        R pmax = pwlf(t.pmax[imed * MXGE + lgle], gle);
+        R xmax = (R)80.65506856998 * eig;
+        R xv, costhe, csqthe, r0, r1;
+        do {
+            r1 = rnd();
+            do {
+                r0 = rnd();
+                r0 *= pmax;
+                int ib = t.ray_iarr[0] - 1;
+                if ((t.ray_iarr[1] - 1) > ib)
+                    while (r0 >= t.ray_fcum[ib + 1]) ib++;
+                r0 = (r0 - t.ray_fcum[ib]) * t.ray_c[ib];
+                xv = t.ray_xgrid[ib] * M<R>::exp(M<R>::log((R)1.0 + r0) * t.ray_b[ib]);
+            } while (xv >= xmax);
+            xv /= eig;
+            costhe = (R)1.0 - (R)0.000307444456 * (xv * xv);
+            csqthe = costhe * costhe;
+        } while ((R)2.0 * r1 >= ((R)1.0 + csqthe));
+        R sinthe = M<R>::sqrt((R)1.0 - csqthe);
+        Uphi q;
+        uphi21(p, q, costhe, sinthe);
+    }
+
+    __device__ __forceinline__ R pair_rej(const Med<R> &m, R xi, R esedei, R eseder, R tteig) {
+        R a = (R)1.0 + eseder, b = (R)1.0 + esedei;
+        R c = a * b / ((R)2.0 * tteig);
+        R xm = xi - (R)0.5;
+        return (R)2.0 + (R)3.0 * (esedei + eseder) -
+               (R)4.0 * (esedei + eseder + (R)1.0 - (R)4.0 * (xm * xm)) *
+                   ((R)1.0 + (R)0.25 * M<R>::log(c * c + m.zbrang * (xi * xi)));
+    }
+
+    // pair production (src/ompmc.c:1447-1685)
+    __device__ void pair(int imed, Part<R> &p0, Part<R> &p1) {
+        const Med<R> &m = t.med[imed];
+        const R RM = (R)0.5109989461;
+        R eig = p0.e, ese1, ese2;
+        int iq1, iq2, l, l1;
+        if (eig <= (R)2.1) {
+            R r0 = rnd(), r1 = rnd();
+            ese2 = RM + (R)0.5 * r0 * (eig - (R)2.0 * RM);
+            ese1 = eig - ese2;
+            if (r1 < (R)0.5) { iq1 = -1; iq2 = 1; } else { iq1 = 1; iq2 = -1; }
+        } else {
+            R Amax, Bmax, delta, aux;
+            if (eig < (R)50.0) {
+                l = 4; l1 = 5;
+                delta = (R)4.0 * m.delcm / eig;
+                if (delta < (R)1.0) {
+                    Amax = m.dl1[l] + delta * (m.dl2[l] + delta * m.dl3[l]);
+                    Bmax = m.dl1[l1] + delta * (m.dl2[l1] + delta * m.dl3[l1]);
+                } else {
+                    aux = M<R>::log(delta + m.dl6[l]);
+                    Amax = m.dl4[l] + m.dl5[l] * aux;
+                    Bmax = m.dl4[l1] + m.dl5[l1] * aux;
+                }
+                aux = (R)1.0 - (R)2.0 * RM / eig;
+                aux = aux * aux;
+                aux *= Amax / (R)3.0;
+                aux /= (Bmax + aux);
+            } else {
+                l = 6;
+                Amax = m.dl1[l];
+                Bmax = m.dl1[l + 1];
+                aux = m.bpar1 * ((R)1.0 - m.bpar0 * RM / eig);
+            }
+            R eavail = eig - (R)2.0 * RM, rejf, rejmax, r4;
+            do {
+                R br, r0 = rnd(), r1 = rnd();
+                r4 = rnd();
+                if (r0 > aux) {
+                    br = (R)0.5 * r1;
+                    rejmax = Bmax;
+                    l1 = l + 1;
+                } else {
+                    R r2 = rnd(), r3 = rnd();
+                    br = (R)0.5 * ((R)1.0 - M<R>::fmax(M<R>::fmax(r1, r2), r3));
+                    rejmax = Amax;
+                    l1 = l;
+                }
+                ese2 = br * eavail + RM;
+                ese1 = eig - ese2;
+                delta = (eig * m.delcm) / (ese2 * ese1);
+                if (delta < (R)1.0) rejf = m.dl1[l1] + delta * (m.dl2[l1] + delta * m.dl3[l1]);
+                else rejf = m.dl4[l1] + m.dl5[l1] * M<R>::log(delta + m.dl6[l1]);
+            } while (r4 * rejmax > rejf);
+            ese1 = eig - ese2;
+            r4 = rnd();
+            if (r4 < (R)0.5) { iq1 = -1; iq2 = 1; } else { iq1 = 1; iq2 = -1; }
+        }
+        p0.e = ese1;
+        p1.e = ese2;
+        Uphi q;
+        for (int j = 0; j < 2; j++) {
+            R ese = j == 0 ? ese1 : ese2;
+            R tteig = eig / RM, ttese = ese / RM;
+            R esedei = ttese / (tteig - ttese), eseder = (R)1.0 / esedei;
+            R pt = (R)3.14159265358979323846 * ttese;
+            R ximin = (R)1.0 / ((R)1.0 + pt * pt);
+            R rejmin = pair_rej(m, ximin, esedei, eseder, tteig);
+            R ya = ((R)2.0 / tteig) * ((R)2.0 / tteig);
+            R xitry = M<R>::fmax((R)0.01, M<R>::fmax(ximin, M<R>::fmin((R)0.5, M<R>::sqrt(ya / m.zbrang))));
+            R galpha = (R)1.0 + (R)0.25 * M<R>::log(ya + m.zbrang * xitry * xitry);
+            R gbeta = (R)0.5 * m.zbrang * xitry / (ya + m.zbrang * xitry * xitry);
+            galpha -= gbeta * (xitry - (R)0.5);
+            R ximid = galpha / ((R)3.0 * gbeta);
+            if (galpha >= (R)0.0) ximid = (R)0.5 - ximid + M<R>::sqrt(ximid * ximid + (R)0.25);
+            else ximid = (R)0.5 - ximid - M<R>::sqrt(ximid * ximid + (R)0.25);
+            ximid = M<R>::fmax((R)0.01, M<R>::fmax(ximin, M<R>::fmin((R)0.5, ximid)));
+            R rejmid = pair_rej(m, ximid, esedei, eseder, tteig);
+            R rejtop = (R)1.0 * M<R>::fmax(rejmin, rejmid);
+            R theta, rtest, rejfactor;
+            // accepted when either test passes (reference quirk Q2)
+            do {
+                R xitst = rnd();
+                R rejtst = pair_rej(m, xitst, esedei, eseder, tteig);
+                rtest = rnd();
+                theta = M<R>::sqrt((R)1.0 / xitst - (R)1.0) / ttese;
+                rejfactor = rejtst / rejtop;
+            } while ((rtest > rejfactor) && (theta >= (R)3.14159265358979323846));
+            R sinthe = M<R>::sin(theta), costhe = M<R>::cos(theta);
+            if (j == 0) {
+                uphi21(p0, q, costhe, sinthe);
+            } else {
+                uphi32(p1, p0, q, costhe, -sinthe);
+            }
+        }
+        p1.iq = iq2;
+        p0.iq = iq1;
+    }
+
+    // Klein-Nishina Compton (src/ompmc.c:1688-1801)
+    __device__ void compton(Part<R> &p0, Part<R> &p1) {
+        const R RM = (R)0.5109989461;
+        R eig = p0.e, ko = p0.e / RM;
+        R broi = (R)1.0 + (R)2.0 * ko, bro = (R)1.0 / broi;
+        bool first = true;
+        R sinthe = (R)0, costhe, br, aux, rejf3, temp = (R)0;
+        R alph1 = (R)0, alph2 = (R)0, alpha = (R)0, rejmax = (R)0;
+        do {
+            if (ko > (R)2.0) {
+                if (first) {
+                    alph1 = M<R>::log(broi);
+                    alph2 = ko * (broi + (R)1.0) * (bro * bro);
+                    alpha = alph1 + alph2;
+                }
+                R r3;
+                do {
+                    R r1 = rnd(), r2 = rnd();
+                    if (r1 * alpha < alph1) br = M<R>::exp(alph1 * r2) * bro;
+                    else br = M<R>::sqrt(r2 * (broi * broi) + ((R)1.0 - r2)) * bro;
+                    temp = ((R)1.0 - br) / (ko * br);
+                    sinthe = M<R>::fmax((R)0.0, temp * ((R)2.0 - temp));
+                    aux = (R)1.0 + br * br;
+                    rejf3 = aux - br * sinthe;
+                    r3 = rnd();
+                } while (r3 * aux > rejf3);
+            } else {
+                if (first) rejmax = broi + bro;
+                R r2;
+                do {
+                    R r1 = rnd();
+                    r2 = rnd();
+                    br = bro + ((R)1.0 - bro) * r1;
+                    temp = ((R)1.0 - br) / (ko * br);
+                    sinthe = M<R>::fmax((R)0.0, temp * ((R)2.0 - temp));
+                    rejf3 = (R)1.0 + br * br - br * sinthe;
+                } while (r2 * br * rejmax > rejf3);
+            }
+            first = false;
+        } while ((br < bro) || (br > (R)1));
+        costhe = (R)1.0 - temp;
+        sinthe = M<R>::sqrt(sinthe);
+        R esg = br * eig, ese = eig - esg + RM;
+        p0.e = esg;
+        Uphi q;
+        uphi21(p0, q, costhe, sinthe);
+        aux = (R)1.0 + br * br - (R)2.0 * br * costhe;
+        if (aux > (R)1.0E-8) {
+            costhe = ((R)1.0 - br * costhe) / M<R>::sqrt(aux);
+            sinthe = ((R)1.0 - costhe) * ((R)1.0 + costhe);
+            sinthe = sinthe > (R)0.0 ? -M<R>::sqrt(sinthe) : (R)0.0;
+        } else {
+            costhe = (R)0.0;
+            sinthe = (R)-1.0;
+        }
+        uphi32(p1, p0, q, costhe, sinthe);
+        p1.e = ese;
+        p1.iq = -1;
+    }
+
+    // photo-electric (src/ompmc.c:1804-1864); no binding energy (quirk Q3)
+    __device__ void photo(Part<R> &p) {
+        const R RM = (R)0.5109989461;
+        p.e += RM;
+        p.iq = -1;
+        R eelec = p.e;
+        if (eelec > ecut_of(medium(p))) {
+            R beta = M<R>::sqrt((eelec - RM) * (eelec + RM)) / eelec;
+            R gamma = eelec / RM;
+            R alpha = (R)0.5 * gamma - (R)0.5 + (R)1.0 / gamma;
+            R ratio = beta / alpha, costhe, sinth2, xi, r2;
+            do {
+                R rn = rnd();
+                rn = (R)2.0 * rn - (R)1.0;
+                if (ratio <= (R)0.2) {
+                    R fk = rn + (R)0.5 * ratio * ((R)1.0 - rn) * ((R)1.0 + rn);
+                    if (gamma < (R)100.0) {
+                        costhe = (beta + fk) / ((R)1.0 + beta * fk);
+                    } else {
+                        if (fk > (R)0.0) {
+                            costhe = (R)1.0 - ((R)1.0 - fk) * (gamma - (R)3.0) / ((R)2.0 * ((R)1.0 + fk) * M<R>::pow3(gamma - (R)1.0));
+                        } else {
+                            costhe = (beta + fk) / ((R)1.0 + beta * fk);
+                        }
+                    }
+                    xi = ((R)1.0 + beta * fk) * (gamma * gamma);
+                } else {
+                    xi = (gamma * gamma) * ((R)1.0 + alpha * (M<R>::sqrt((R)1.0 + ratio * ((R)2.0 * rn + ratio)) - (R)1.0));
+                    costhe = ((R)1.0 - (R)1.0 / xi) / beta;
+                }
+                sinth2 = M<R>::fmax(((R)1.0 - costhe) * ((R)1.0 + costhe), (R)0.0);
+                r2 = rnd();
+            } while (r2 > (R)0.5 * ((R)1.0 + gamma) * sinth2 * xi / gamma);
+            Uphi q;
+            uphi21(p, q, costhe, M<R>::sqrt(sinth2));
+        }
+    }
+
+    // photon step (src/ompmc.c:1867-2139) for nsplit = 1: transport to the
+    // next interaction (or out of the phantom / below PCUT), interact, return.
+    __device__ int photon(Part<R> &p, Part<R> &q) {
+        const R RM = (R)0.5109989461;
+        if (count) cnt.nphoton++;
+        int imed = medium(p);
+        R eig = p.e;
+        if (eig <= pcut_of(imed) || p.wt == (R)0) {
+            ausgab(p, eig);
+            return -1;
+        }
+        R gle = M<R>::log(eig);
+        // splitting bookkeeping of the reference with nsplit = 1: two draws,
+        // eta' = (1 - r + 1) - 1, survivor index 0
+        R rn = rnd();
+        R eta = (R)1.0 - rn + (R)1.0;
+        (void)rnd();
+        eta -= (R)1.0;
+        R dpmfp = -M<R>::log(eta);
+        int lgle = 0;
+        R cohfac = (R)0, gmfp = (R)0;
+        for (;;) {
+            R tstep;
+            if (imed != -1) {
+                const Med<R> &m = t.med[imed];
+                lgle = pwlf_i(gle, m.ge1, m.ge0) - 1;
+                R gmfpr0 = pwlf(t.gmfp[imed * MXGE + lgle], gle);
+                gmfp = gmfpr0 / rhof_of(p);
+                cohfac = pwlf(t.cohe[imed * MXGE + lgle], gle);
+                gmfp *= cohfac;
+                tstep = gmfp * dpmfp;
+            } else {
+                tstep = (R)1.0E8;
+            }
+            R ustep = tstep;
+            if (p.ix < 0) {           // howfar: outside the geometry -> discard
+                if (count) cnt.nhowfar++;
+                p.x += ustep * p.u; p.y += ustep * p.v; p.z += ustep * p.w;
+                return -1;
+            }
+            bool leaves;
+            int mv = howfar(p, ustep, leaves);
+            p.x += ustep * p.u;
+            p.y += ustep * p.v;
+            p.z += ustep * p.w;
+            if (imed != -1) dpmfp = M<R>::fmax((R)0.0, dpmfp - ustep / gmfp);
+            if (mv != 0) {
+                apply_move(p, mv, leaves);
+                imed = medium(p);
+            }
+            if (imed != -1 && dpmfp <= (R)1.0E-05) break;
+        }
+        R r = rnd();
+        if (r <= (R)1.0 - cohfac) {
+            rayleigh(p, imed, eig, gle, lgle);
+            return 0;
+        }
+        r = rnd();
+        R gbr1 = pwlf(t.gbr1[imed * MXGE + lgle], gle);
+        if (r <= gbr1 && eig > (R)2.0 * RM) {
+            pair(imed, p, q);
+            return 1;
+        }
+        R gbr2 = pwlf(t.gbr2[imed * MXGE + lgle], gle);
+        if (r < gbr2) {
+            compton(p, q);
+            return 1;
+        }
+        photo(p);
+        return 0;
+    }
+
+    // ---------------------------------------------------------------- electron physics
+    struct Spinr { int i, j; };
+    struct Mscats { int i, j; R omega2; };
+
+    // spin rejection (src/ompmc.c:3115-3186)
+    __device__ R spin_rejection(int imed, int qel, R elke, R beta2, R q1, R cost, bool &spin_index, bool is_single,
+                                Spinr &sr) {
+        if (spin_index) {
+            spin_index = false;
+            R ai;
+            if (beta2 >= t.b2spin_min) {
+                ai = (beta2 - t.b2spin_min) * t.dbeta2i;
+                sr.i = (int)ai;
+                ai -= (R)sr.i;
+                sr.i += MXE_SPIN + 1;
+            } else if (elke > t.espml) {
+                ai = (elke - t.espml) * t.dleneri;
+                sr.i = (int)ai;
+                ai -= (R)sr.i;
+            } else {
+                sr.i = 0;
+                ai = (R)-1.0;
+            }
+            R rn = rnd();
+            if (rn < ai) sr.i += 1;
+            if (is_single) {
+                sr.j = 0;
+            } else {
+                R qq1 = (R)2.0 * q1;
+                qq1 = qq1 / ((R)1.0 + qq1);
+                R aj = qq1 * t.dqq1i;
+                sr.j = (int)aj;
+                if (sr.j >= MXQ_SPIN) {
+                    sr.j = MXQ_SPIN;
+                } else {
+                    aj -= (R)sr.j;
+                    rn = rnd();
+                    if (rn < aj) sr.j += 1;
+                }
+            }
+        }
+        R xi = M<R>::sqrt((R)0.5 * ((R)1.0 - cost));
+        R ak = xi * (R)MXU_SPIN;
+        int k = (int)ak;
+        ak -= (R)k;
+        const R *row = t.spin_rej + (size_t)imed * NSPIN_MED +
+                       ((qel * (MXE_SPIN1 + 1) + sr.i) * (MXQ_SPIN + 1) + sr.j) * (MXU_SPIN + 1);
+        return ((R)1.0 - ak) * row[k] + ak * row[k + 1];
+    }
+
+    // single elastic scattering (src/ompmc.c:3188-3217)
+    __device__ void sscat(int imed, int qel, R chia2, R elke, R beta2, R &cost, R &sint) {
+        bool spin_index = true;
+        Spinr sr = {0, 0};
+        R xi, rejf, rn;
+        do {
+            xi = rnd();
+            xi = (R)2.0 * chia2 * xi / ((R)1.0 - xi + chia2);
+            cost = (R)1.0 - xi;
+            rejf = spin_rejection(imed, qel, elke, beta2, (R)0, cost, spin_index, true, sr);
+            rn = rnd();
+        } while (rn > rejf);
+        sint = M<R>::sqrt(xi * ((R)2.0 - xi));
+    }
+
+    // multiple-scattering angle (src/ompmc.c:3624-3803).  Within a bin of the
+    // q^(2+) surface the reference's interpolation step is du = 0 (it
+    // subtracts the bin value from itself), so u is the tabulated value.
+    __device__ void mscat(int imed, int qel, bool &spin_index, bool &find_index, R elke, R beta2, R q1, R lambda,
+                          R chia2, R &cost, R &sint, Mscats &ms, Spinr &sr) {
+        R xi, rejf, rn;
+        R explambda = M<R>::exp(-lambda);
+        if (lambda <= (R)13.8) {
+            R sprob = rnd();
+            if (sprob < explambda) { cost = (R)1.0; sint = (R)0.0; return; }
+            R wsum = ((R)1.0 + lambda) * explambda;
+            if (sprob < wsum) {
+                do {
+                    xi = rnd();
+                    xi = (R)2.0 * chia2 * xi / ((R)1.0 - xi + chia2);
+                    cost = (R)1.0 - xi;
+                    rejf = spin_rejection(imed, qel, elke, beta2, q1, cost, spin_index, false, sr);
+                    rn = rnd();
+                } while (rn > rejf);
+                sint = M<R>::sqrt(xi * ((R)2.0 - xi));
+                return;
+            }
+            if (lambda <= (R)1) {
+                int icount = 0;
+                R wprob = explambda, sinz, cosz;
+                wsum = explambda;
+                cost = (R)1.0;
+                sint = (R)0.0;
+                do {
+                    icount += 1;
+                    if (icount > 20) break;
+                    wprob = wprob * lambda / (R)icount;
+                    wsum = wsum + wprob;
+                    do {
+                        xi = rnd();
+                        xi = (R)2.0 * chia2 * xi / ((R)1.0 - xi + chia2);
+                        cosz = (R)1.0 - xi;
+                        rejf = spin_rejection(imed, qel, elke, beta2, q1, cosz, spin_index, false, sr);
+                        rn = rnd();
+                    } while (rn > rejf);
+                    sinz = xi * ((R)2.0 - xi);
+                    if (sinz > (R)1.0E-20) {
+                        sinz = M<R>::sqrt(sinz);
+                        xi = rnd();
+                        R phi = xi * (R)6.2831853;
+                        cost = cost * cosz - sint * sinz * M<R>::cos(phi);
+                        sint = M<R>::sqrt(M<R>::fmax((R)0.0, ((R)1.0 - cost) * ((R)1.0 + cost)));
+                    }
+                } while (wsum <= sprob);
+                return;
+            }
+        }
+        if (lambda <= (R)1.0E5) {
+            R llmbda = M<R>::log(lambda);
+            if (find_index) {
+                R ai = llmbda * t.dllambi;
+                ms.i = (int)ai;
+                ai -= (R)ms.i;
+                xi = rnd();
+                if (xi < ai) ms.i += 1;
+                if (q1 < (R)1.0E-3) {
+                    ms.j = 0;
+                } else if (q1 < (R)0.5) {
+                    R aj = q1 * t.dqmsi;
+                    ms.j = (int)aj;
+                    aj -= (R)ms.j;
+                    xi = rnd();
+                    if (xi < aj) ms.j += 1;
+                } else {
+                    ms.j = MXQ_MS;
+                }
+                if (llmbda < (R)2.2299)
+                    ms.omega2 = chia2 * (lambda + (R)4.0) *
+                                ((R)1.347006 + llmbda * ((R)0.209364 - llmbda * ((R)0.45525 - llmbda * ((R)0.50142 - (R)0.081234 * llmbda))));
+                else
+                    ms.omega2 = chia2 * (lambda + (R)4.0) *
+                                ((R)-2.77164 + llmbda * ((R)2.94874 - llmbda * ((R)0.1535754 - llmbda * (R)0.00552888)));
+                find_index = false;
+            }
+            int base = (ms.i * (MXQ_MS + 1) + ms.j) * (MXU_MS + 1);
+            do {
+                xi = rnd();
+                R ak = xi * (R)MXU_MS;
+                int k = (int)ak;
+                ak -= (R)k;
+                if (ak > t.wms[base + k]) k = t.ims[base + k];
+                R a = t.fms[base + k];
+                R u = t.ums[base + k];
+                R du = (R)0;
+                xi = rnd();
+                if (M<R>::fabs(a) < (R)0.2) {
+                    R x1 = (R)0.5 * ((R)1.0 - xi) * a;
+                    u += xi * du * ((R)1.0 + x1 * ((R)1.0 - xi * a));
+                } else {
+                    u -= du / a * ((R)1.0 - M<R>::sqrt((R)1.0 + xi * a * ((R)2.0 + a)));
+                }
+                xi = ms.omega2 * u / ((R)1.0 + (R)0.5 * ms.omega2 - u);
+                if (xi > (R)1.99999) xi = (R)1.99999;
+                cost = (R)1.0 - xi;
+                rejf = spin_rejection(imed, qel, elke, beta2, q1, cost, spin_index, false, sr);
+                rn = rnd();
+            } while (rn > rejf);
+            sint = M<R>::sqrt(xi * ((R)2.0 - xi));
+        }
+    }
+
+    // two-substep multiple scattering, PRESTA-II style (src/ompmc.c:3805-3994)
+    __device__ R msdist(const Part<R> &p, int imed, int iq, R rhof, R de, R tustep, R eke, Part<R> &out) {
+        const R RM = (R)0.5109989461;
+        const Med<R> &m = t.med[imed];
+        int qel = (1 + iq) / 2;
+        Mscats ms = {0, 0, (R)0};
+        Spinr sr = {0, 0};
+        R blccc = m.blcc, xcccc = m.xcc;
+        R e = eke - (R)0.5 * de;
+        R tau = e / RM, tau2 = tau * tau;
+        R epsilon = de / eke, epsilonp = de / e;
+        e *= ((R)1.0 - (epsilonp * epsilonp) * ((R)6.0 + (R)10.0 * tau + (R)5.0 * tau2) / ((R)24.0 * tau2 + (R)72.0 * tau + (R)48.0));
+        R p2 = e * (e + (R)2.0 * RM);
+        R beta2 = p2 / (p2 + RM * RM);
+        R chia2 = xcccc / ((R)4.0 * p2 * blccc);
+        R lambda = (R)0.5 * tustep * rhof * blccc / beta2;
+        R tq = epsilonp / ((tau + (R)1.0) * (tau + (R)2.0));
+        R temp2 = (R)0.166666 * ((R)4.0 + tau * ((R)6.0 + tau * ((R)7.0 + tau * ((R)4.0 + tau)))) * tq * tq;
+        lambda *= ((R)1.0 - temp2);
+        R elke = M<R>::log(e);
+        int lelke = pwlf_i(elke, m.eke1, m.eke0) - 1;
+        if (lelke < 0) {
+            lelke = 0;
+            elke = ((R)1.0 - m.eke0) / m.eke1;
+        }
+        int ix = MXEKE * imed + lelke;
+        R etap, xi_corr, gamma;
+        if (qel == 0) {
+            etap = pwlf(t.etae[ix], elke);
+            xi_corr = pwlf(t.q1ce[ix], elke);
+            gamma = pwlf(t.q2ce[ix], elke);
+        } else {
+            etap = pwlf(t.etap[ix], elke);
+            xi_corr = pwlf(t.q1cp[ix], elke);
+            gamma = pwlf(t.q2cp[ix], elke);
+        }
+        R ms_corr = pwlf(t.blcce[ix], elke);
+        chia2 *= etap;
+        lambda /= (etap * ((R)1.0 + chia2));
+        lambda *= ms_corr;
+        R chilog = M<R>::log((R)1.0 + (R)1.0 / chia2);
+        R q1 = (R)2.0 * chia2 * (chilog * ((R)1.0 + chia2) - (R)1.0);
+        gamma = (R)6.0 * chia2 * ((R)1.0 + chia2) * (chilog * ((R)1.0 + (R)2.0 * chia2) - (R)2.0) / q1 * gamma;
+        R xi = q1 * lambda;
+        bool find_index = true, spin_index = true;
+        R w1 = (R)0, sint1 = (R)0, w2 = (R)0, sint2 = (R)0, cphi1, sphi1, cphi2, sphi2;
+        mscat(imed, qel, spin_index, find_index, elke, beta2, xi, lambda, chia2, w1, sint1, ms, sr);
+        azimuth(cphi1, sphi1);
+        mscat(imed, qel, spin_index, find_index, elke, beta2, xi, lambda, chia2, w2, sint2, ms, sr);
+        azimuth(cphi2, sphi2);
+        R u2 = sint2 * cphi2, v2 = sint2 * sphi2;
+        R u2p = w1 * u2 + sint1 * w2;
+        R us = u2p * cphi1 - v2 * sphi1;
+        R vs = u2p * sphi1 + v2 * cphi1;
+        R ws = w1 * w2 - sint1 * u2;
+        xi *= (R)2 * xi_corr;
+        R eta = rnd();
+        R eta1 = (R)0.5 * ((R)1.0 - eta);
+        R delta = (R)0.9082483 - ((R)0.1020621 - (R)0.0263747 * gamma) * xi;
+        R temp1 = (R)2.0 + tau;
+        R temp = ((R)2.0 + tau * temp1) / ((tau + (R)1.0) * temp1);
+        temp -= (tau + (R)1.0) / ((tau + (R)2.0) * (chilog * ((R)1.0 + chia2) - (R)1.0));
+        temp *= epsilonp;
+        temp1 = (R)1.0 - temp;
+        delta += (R)0.40824829 * (epsilon * (tau + (R)1.0) /
+                                      ((tau + (R)2.0) * (chilog * ((R)1.0 + chia2) - (R)1.0) * (chilog * ((R)1.0 + (R)2.0 * chia2) - (R)2.0)) -
+                                  (R)0.25 * (temp * temp));
+        R b = eta * delta, c = eta * ((R)1.0 - delta);
+        R w1v2 = w1 * v2;
+        R ut = b * sint1 * cphi1 + c * (cphi1 * u2 - sphi1 * w1v2) + eta1 * us * temp1;
+        R vt = b * sint1 * sphi1 + c * (sphi1 * u2 + cphi1 * w1v2) + eta1 * vs * temp1;
+        R wt = eta1 * ((R)1.0 + temp) + b * w1 + c * w2 + eta1 * ws * temp1;
+        R ustep = tustep * M<R>::sqrt(ut * ut + vt * vt + wt * wt);
+        R u0 = p.u, v0 = p.v, w0 = p.w;
+        R sint02 = u0 * u0 + v0 * v0;
+        if (sint02 > (R)1.0E-20) {
+            R sint0 = M<R>::sqrt(sint02), sint0i = (R)1.0 / sint0;
+            R cphi0 = sint0i * u0, sphi0 = sint0i * v0;
+            u2p = w0 * us + sint0 * ws;
+            ws = w0 * ws - sint0 * us;
+            us = u2p * cphi0 - vs * sphi0;
+            vs = u2p * sphi0 + vs * cphi0;
+            u2p = w0 * ut + sint0 * wt;
+            wt = w0 * wt - sint0 * ut;
+            ut = u2p * cphi0 - vt * sphi0;
+            vt = u2p * sphi0 + vt * cphi0;
+        } else {
+            wt = w0 * wt;
+            ws = w0 * ws;
+        }
+        out.x = p.x + tustep * ut;
+        out.y = p.y + tustep * vt;
+        out.z = p.z + tustep * wt;
+        out.u = us;
+        out.v = vs;
+        out.w = ws;
+        return ustep;
+    }
+
+    // CSDA path length inside one PWLF interval (src/ompmc.c:3997-4032)
+    __device__ R drange(int imed, int iq, int lelke, R ekei, R ekef, R elkei, R elkef) {
+        R fedep = (R)1.0 - ekef / ekei;
+        R elktmp = (R)0.5 * (elkei + elkef + (R)0.25 * fedep * fedep * ((R)1.0 + fedep * ((R)1.0 + (R)0.875 * fedep)));
+        const PW<R> &q = iq < 0 ? t.ededx[MXEKE * imed + lelke] : t.pdedx[MXEKE * imed + lelke];
+        R dedxmid = (R)1.0 / pwlf(q, elktmp);
+        R aux = q.c1 * dedxmid;
+        aux = aux * ((R)1.0 + (R)2.0 * aux) * fedep * fedep / ((R)6.0 * ((R)2.0 - fedep) * ((R)2.0 - fedep));
+        return fedep * ekei * dedxmid * ((R)1.0 + aux);
+    }
+
+    // sub-threshold energy loss over a path tustep (src/ompmc.c:4034-4126)
+    __device__ R eloss(int imed, int iq, R rhof, R tustep, R range, R eke, R elke, int lelke) {
+        const Med<R> &m = t.med[imed];
+        int qel = (1 + iq) / 2;
+        const R *rep = t.range_ep + ((size_t)qel * t.nmed + imed) * MXEKE;
+        const PW<R> *dx = iq < 0 ? t.ededx : t.pdedx;
+        R de, fedep;
+        R tuss = range - rep[lelke] / rhof;
+        if (tuss >= tustep) {
+            const PW<R> &q = dx[imed * MXEKE + lelke];
+            R dedxmid = pwlf(q, elke);
+            R aux = q.c1 / dedxmid;
+            de = dedxmid * tustep * rhof;
+            fedep = de / eke;
+            de *= ((R)1.0 - (R)0.5 * fedep * aux * ((R)1.0 - (R)0.333333 * fedep * (aux - (R)1.0 - (R)0.25 * fedep * ((R)2.0 - aux * ((R)4.0 - aux)))));
+        } else {
+            int l = lelke;
+            tuss = (range - tustep) * rhof;
+            if (tuss <= (R)0) {
+                de = eke - m.te * (R)0.99;
+            } else {
+                while (l > 0 && tuss < rep[l]) l -= 1;
+                R elktmp = ((R)(l + 2) - m.eke0) / m.eke1;
+                R eketmp = t.e_array[imed * MXEKE + l + 1];
+                tuss = (rep[l + 1] - tuss) / rhof;
+                const PW<R> &q = dx[MXEKE * imed + l];
+                R dedxmid = pwlf(q, elktmp);
+                R aux = q.c1 / dedxmid;
+                de = dedxmid * tuss * rhof;
+                fedep = de / eketmp;
+                de *= ((R)1.0 - (R)0.5 * fedep * aux * ((R)1.0 - (R)0.333333 * fedep * (aux - (R)1.0 - (R)0.25 * fedep * ((R)2.0 - aux * ((R)4.0 - aux)))));
+                de += eke - eketmp;
+            }
+        }
+        return de;
+    }
+
+    // two-photon annihilation at rest (src/ompmc.c:4129-4185)
+    __device__ void rannih(Part<R> &a, Part<R> &b) {
+        const R RM = (R)0.5109989461;
+        R rn = rnd();
+        R costhe = (R)2.0 * rn - (R)1;
+        R sinthe = M<R>::sqrt(M<R>::fmax((R)0.0, ((R)1.0 - costhe) * ((R)1.0 + costhe)));
+        (void)rnd();                    // drawn and unused, as in the reference
+        R cphi, sphi;
+        azimuth(cphi, sphi);
+        a.e = RM; a.iq = 0;
+        a.u = sinthe * cphi; a.v = sinthe * sphi; a.w = costhe;
+        b.e = RM; b.iq = 0;
+        transfer(b, a);
+        b.u = (R)-1.0 * a.u; b.v = (R)-1.0 * a.v; b.w = (R)-1.0 * a.w;
+    }
+
+    // bremsstrahlung (src/ompmc.c:4188-4374)
+    __device__ void brems(Part<R> &pe, Part<R> &pg) {
+        const R RM = (R)0.5109989461;
+        int imed = medium(pe);
+        const Med<R> &m = t.med[imed];
+        R eie = pe.e;
+        int l = eie < (R)50.0 ? 1 : 3, l1 = l + 1;
+        R ekin = eie - RM, brmin = m.ap / ekin, waux = -M<R>::log(brmin);
+        R a = pe.u, b = pe.v, c = pe.w;
+        R sinpsi = a * a + b * b, sindel = (R)0, cosdel = (R)0;
+        if (sinpsi > (R)1.0E-20) {
+            sinpsi = M<R>::sqrt(sinpsi);
+            sindel = b / sinpsi;
+            cosdel = a / sinpsi;
+        }
+        R ztarg = m.zbrang, tteie = eie / RM;
+        R beta = M<R>::sqrt((tteie - (R)1.0) * (tteie + (R)1.0)) / tteie;
+        R y2max = (R)2.0 * beta * ((R)1.0 + beta) * tteie * tteie;
+        R y2maxi = (R)1.0 / y2max;
+        R z2max = y2max + (R)1.0;
+        R z2maxi = M<R>::sqrt(z2max);
+        R aux, br, delta, r7, rejf, ese, esg, phi1, phi2;
+        do {
+            R r6 = rnd();
+            r7 = rnd();
+            br = brmin * M<R>::exp(r6 * waux);
+            esg = ekin * br;
+            ese = eie - esg;
+            delta = esg / eie / ese * m.delcm;
+            aux = ese / eie;
+            if (delta < (R)1.0) {
+                phi1 = m.dl1[l - 1] + delta * (m.dl2[l - 1] + delta * m.dl3[l - 1]);
+                phi2 = m.dl1[l1 - 1] + delta * (m.dl2[l1 - 1] + delta * m.dl3[l1 - 1]);
+            } else {
+                phi1 = m.dl4[l - 1] + m.dl5[l - 1] * M<R>::log(delta + m.dl6[l - 1]);
+                phi2 = phi1;
+            }
+            rejf = ((R)1.0 + aux * aux) * phi1 - (R)2.0 * aux * phi2 / (R)3.0;
+        } while (r7 >= rejf);
+        pg.e = esg;
+        pg.iq = 0;
+        transfer(pg, pe);
+        R y2tst = (R)0, ttese = ese / RM, esedei = ttese / tteie;
+        R rjarg1 = (R)1.0 + esedei * esedei, rjarg2 = rjarg1 + (R)2.0 * esedei, rjarg3;
+        R rtest = (R)1.0, rejtst = (R)0.0;
+        aux = (R)2.0 * ese * tteie / esg;
+        aux = aux * aux;
+        R aux1 = aux * ztarg;
+        if (aux1 > (R)10.0) rjarg3 = -M<R>::log(m.zbrang) + ((R)1.0 - aux1) / (aux1 * aux1);
+        else rjarg3 = M<R>::log(aux / ((R)1.0 + aux1));
+        R rejmax = rjarg1 * rjarg3 - rjarg2;
+        while (rtest >= rejtst) {
+            y2tst = rnd();
+            rtest = rnd();
+            R aux3 = z2maxi / (y2tst + ((R)1.0 - y2tst) * z2maxi);
+            rtest = rtest * aux3 * rejmax;
+            y2tst = aux3 * aux3 - (R)1.0;
+            R a4 = M<R>::pow4(aux3);
+            R y2tst1 = esedei * y2tst / a4;
+            R aux4 = (R)16.0 * y2tst1 - rjarg2;
+            R aux5 = rjarg1 - (R)4.0 * y2tst1;
+            if (rtest < aux4 + aux5 * rjarg3) break;
+            R aux2 = M<R>::log(aux / ((R)1.0 + aux1 / a4));
+            rejtst = aux4 + aux5 * aux2;
+        }
+        R costhe = (R)1.0 - (R)2.0 * y2tst * y2maxi;
+        R sinthe = M<R>::sqrt(M<R>::fmax((R)0.0, ((R)1.0 - costhe * costhe)));
+        R cphi, sphi;
+        azimuth(cphi, sphi);
+        if (sinpsi >= (R)1.0E-10) {
+            R us = sinthe * cphi, vs = sinthe * sphi;
+            pg.u = c * cosdel * us - sindel * vs + a * costhe;
+            pg.v = c * sindel * us + cosdel * vs + b * costhe;
+            pg.w = c * costhe - sinpsi * us;
+        } else {
+            pg.u = sinthe * cphi;
+            pg.v = sinthe * sphi;
+            pg.w = c * costhe;
+        }
+        pe.e = ese;
+    }
+
+    // Moller (src/ompmc.c:4377-4453); nothing happens at or below 2*TE (quirk Q4)
+    __device__ int moller(Part<R> &p0, Part<R> &p1) {
+        const R RM = (R)0.5109989461;
+        const Med<R> &m = t.med[medium(p0)];
+        R eie = p0.e, ekin = eie - RM;
+        if (ekin <= (R)2.0 * m.te) return 0;
+        R t0 = ekin / RM, e0 = t0 + (R)1.0, extrae = eie - m.thmoll;
+        R g2 = (t0 * t0) / (e0 * e0), g3 = ((R)2.0 * t0 + (R)1.0) / (e0 * e0);
+        R br, gmax = ((R)1.0 + (R)1.25 * g2), rejf4, r28;
+        do {
+            R r27 = rnd();
+            br = m.te / (ekin - extrae * r27);
+            R r = br / ((R)1.0 - br);
+            r28 = rnd();
+            rejf4 = ((R)1.0 + g2 * (br * br) + r * (r - g3));
+            r28 *= gmax;
+        } while (r28 > rejf4);
+        R ekse2 = br * ekin, ese1 = eie - ekse2, ese2 = ekse2 + RM;
+        p0.e = ese1;
+        p1.e = ese2;
+        R h1 = (eie + RM) / ekin;
+        R costh = h1 * (ese1 - RM) / (ese1 + RM);
+        Uphi q;
+        uphi21(p0, q, M<R>::sqrt(costh), M<R>::sqrt((R)1.0 - costh));
+        p1.iq = -1;
+        costh = h1 * (ese2 - RM) / (ese2 + RM);
+        uphi32(p1, p0, q, M<R>::sqrt(costh), -M<R>::sqrt((R)1.0 - costh));
+        return 1;
+    }
+
+    // Bhabha (src/ompmc.c:4456-4543)
+    __device__ void bhabha(Part<R> &p0, Part<R> &p1) {
+        const R RM = (R)0.5109989461;
+        const Med<R> &m = t.med[medium(p0)];
+        R eip = p0.e, ekin = eip - RM, t0 = ekin / RM, e0 = t0 + (R)1.0;
+        R yy = (R)1.0 / (t0 + (R)2.0);
+        R beta2 = (e0 * e0 - (R)1.0) / (e0 * e0);
+        R ep0 = m.te / ekin, ep0c = (R)1.0 - ep0, yp = (R)1.0 - (R)2.0 * yy;
+        R b4 = M<R>::pow3(yp);
+        R b3 = b4 + yp * yp;
+        R b2 = yp * ((R)3.0 + yy * yy);
+        R b1 = (R)2.0 - yy * yy;
+        R br, rejf2, r4;
+        do {
+            R r3 = rnd();
+            br = ep0 / ((R)1.0 - ep0c * r3);
+            r4 = rnd();
+            rejf2 = ((R)1.0 - beta2 * br * (b1 - br * (b2 - br * (b3 - br * b4))));
+        } while (r4 > rejf2);
+        p1.iq = 1;
+        if (br < (R)0.5) {
+            p1.iq = -1;
+        } else {
+            p0.iq = -1;
+            p1.iq = 1;
+            br = (R)1.0 - br;
+        }
+        br = M<R>::fmax(br, (R)0.0);
+        R ekse2 = br * ekin, ese1 = eip - ekse2, ese2 = ekse2 + RM;
+        p0.e = ese1;
+        p1.e = ese2;
+        R h1 = (eip + RM) / ekin;
+        R costh = M<R>::fmin((R)1.0, h1 * (ese1 - RM) / (ese1 + RM));
+        Uphi q;
+        uphi21(p0, q, M<R>::sqrt(costh), M<R>::sqrt((R)1.0 - costh));
+        costh = h1 * (ese2 - RM) / (ese2 + RM);
+        uphi32(p1, p0, q, M<R>::sqrt(costh), -M<R>::sqrt((R)1.0 - costh));
+    }
+
+    // two-photon annihilation in flight (src/ompmc.c:4546-4663)
+    __device__ void annih(Part<R> &p0, Part<R> &p1) {
+        const R RM = (R)0.5109989461;
+        R avip = p0.e + RM, a = avip / RM;
+        R g = a - (R)1.0, tt = g - (R)1.0, pp = M<R>::sqrt(a * tt);
+        R pot = pp / tt, ep0 = (R)1.0 / (a + pp), wsamp = M<R>::log(((R)1.0 - ep0) / ep0);
+        R aa = p0.u, bb = p0.v, cc = p0.w;
+        R sinpsi = aa * aa + bb * bb, sindel = (R)0, cosdel = (R)0;
+        if (sinpsi > (R)1.0E-20) {
+            sinpsi = M<R>::sqrt(sinpsi);
+            sindel = bb / sinpsi;
+            cosdel = aa / sinpsi;
+        }
+        R ep, rejf, r2;
+        do {
+            R r1 = rnd();
+            ep = ep0 * M<R>::exp(r1 * wsamp);
+            r2 = rnd();
+            R d = ep * a - (R)1.0;
+            rejf = (R)1.0 - d * d / (ep * (a * a - (R)2.0));
+        } while (r2 > rejf);
+        R esg1 = avip * ep;
+        p0.e = esg1;
+        p0.iq = 0;
+        R costhe = M<R>::fmin((R)1.0, (esg1 - RM) * pot / esg1);
+        R sinthe = M<R>::sqrt((R)1.0 - costhe * costhe);
+        R sphi, cphi;
+        azimuth(cphi, sphi);
+        R us, vs;
+        if (sinpsi >= (R)1.0E-10) {
+            us = sinthe * cphi;
+            vs = sinthe * sphi;
+            p0.u = cc * cosdel * us - sindel * vs + aa * costhe;
+            p0.v = cc * sindel * us + cosdel * vs + bb * costhe;
+            p0.w = cc * costhe - sinpsi * us;
+        } else {
+            p0.u = sinthe * cphi;
+            p0.v = sinthe * sphi;
+            p0.w = cc * costhe;
+        }
+        R esg2 = avip - esg1;
+        p1.e = esg2;
+        p1.iq = 0;
+        transfer(p1, p0);
+        costhe = M<R>::fmin((R)1.0, (esg2 - RM) * pot / esg2);
+        sinthe = -M<R>::sqrt((R)1.0 - costhe * costhe);
+        if (sinpsi >= (R)1.0E-10) {
+            us = sinthe * cphi;
+            vs = sinthe * sphi;
+            p1.u = cc * cosdel * us - sindel * vs + aa * costhe;
+            p1.v = cc * sindel * us + cosdel * vs + bb * costhe;
+            p1.w = cc * costhe - sinpsi * us;
+        } else {
+            p1.u = sinthe * cphi;
+            p1.v = sinthe * sphi;
+            p1.w = cc * costhe;
+        }
+    }
+
+    // electron below cut-off or discarded: deposit, annihilate a positron
+    // (src/ompmc.c:4683-4705, 5079-5106)
+    __device__ int electron_stop(Part<R> &p, Part<R> &q, int iq, R eie, bool discard) {
+        const R RM = (R)0.5109989461;
+        R edep = (discard && iq > 0) ? p.e + RM : p.e - RM;
+        ausgab(p, edep);
+        if (iq > 0 && edep < eie) {
+            rannih(p, q);
+            return 1;
+        }
+        return -1;
+    }
+
+    // condensed-history electron step (src/ompmc.c:4666-5451).  The variable
+    // scopes of the reference are kept: the accumulated path to the next
+    // interaction (total_tstep) restarts at zero in every ustep iteration, so
+    // a step that does not reach the interaction site is followed by a
+    // zero-length step that resamples the distance.
+    __device__ int electron(Part<R> &p, Part<R> &q) {
+        const R RM = (R)0.5109989461;
+        if (count) cnt.nelectron++;
+        int imed = medium(p);
+        R rhof = rhof_of(p);
+        R eie = p.e;
+        int iq = p.iq, qel = (1 + iq) / 2, medold = imed;
+        if (eie <= ecut_of(imed)) return electron_stop(p, q, iq, eie, false);
+        R elke = (R)0;
+        int lelke = 0;
+        R sigratio = (R)0, rfict = (R)0;
+        do {
+            bool compute_tstep = true;
+            R eke = eie - RM, demfp = (R)0, sig0 = (R)0, ustep = (R)0;
+            if (imed != -1) {
+                const Med<R> &m = t.med[imed];
+                R rn = rnd();
+                if (rn == (R)0.0) rn = (R)1.0E-30;
+                demfp = M<R>::fmax(-M<R>::log(rn), (R)1.0E-5);
+                elke = M<R>::log(eke);
+                lelke = pwlf_i(elke, m.eke1, m.eke0) - 1;
+                int ix = imed * MXEKE + lelke;
+                if (m.sig_mono[qel]) {
+                    if (iq < 0) sig0 = pwlf(t.esig[ix], elke) / pwlf(t.ededx[ix], elke);
+                    else sig0 = pwlf(t.psig[ix], elke) / pwlf(t.pdedx[ix], elke);
+                } else {
+                    sig0 = iq < 0 ? m.esig_e : m.psig_e;
+                }
+            }
+            do {
+                bool call_howfar = false, do_single = false, called_msdist = false;
+                R tstep, tustep, total_tstep = (R)0, range = (R)0, tvstep = (R)0, de = (R)0, tperp = (R)0;
+                Part<R> fin;
+                if (imed == -1) {
+                    tstep = (R)10.0E8;
+                    ustep = tstep;
+                    tustep = ustep;
+                    call_howfar = true;
+                } else {
+                    const Med<R> &m = t.med[imed];
+                    const R *rep = t.range_ep + ((size_t)qel * t.nmed + imed) * MXEKE;
+                    rhof = rhof_of(p);
+                    if (sig0 <= (R)0.0) {
+                        tstep = (R)10.0E8;
+                        sig0 = (R)1.0E-15;
+                    } else {
+                        if (compute_tstep) {
+                            R total_de = demfp / sig0;
+                            R ekef = eke - total_de;
+                            if (ekef <= m.e_array0) {
+                                tstep = (R)10.0E8;
+                            } else {
+                                R elkef = M<R>::log(ekef);
+                                int lelkef = pwlf_i(elkef, m.eke1, m.eke0) - 1;
+                                if (lelkef == lelke) {
+                                    tstep = drange(imed, iq, lelke, eke, ekef, elke, elkef);
+                                } else {
+                                    R ekei = t.e_array[imed * MXEKE + lelke];
+                                    R elkei = ((R)(lelke + 1) - m.eke0) / m.eke1;
+                                    R tuss = drange(imed, iq, lelke, eke, ekei, elke, elkei);
+                                    ekei = t.e_array[imed * MXEKE + lelkef + 1];
+                                    elkei = ((R)(lelkef + 2) - m.eke0) / m.eke1;
+                                    tstep = drange(imed, iq, lelkef, ekei, ekef, elkei, elkef);
+                                    tstep += tuss + rep[lelke] - rep[lelkef + 1];
+                                }
+                            }
+                            total_tstep = tstep;
+                            compute_tstep = false;
+                        }
+                        tstep = total_tstep / rhof;
+                    }
+                    int ix = imed * MXEKE + lelke;
+                    R dedx0 = iq < 0 ? pwlf(t.ededx[ix], elke) : pwlf(t.pdedx[ix], elke);
+                    R dedx = rhof * dedx0;
+                    R tmxs = pwlf(t.tmxs[ix], elke);
+                    tmxs /= rhof;
+                    R ekei = t.e_array[imed * MXEKE + lelke];
+                    R elkei = ((R)(lelke + 1) - m.eke0) / m.eke1;
+                    range = drange(imed, iq, lelke, eke, ekei, elke, elkei);
+                    range += rep[lelke];
+                    range /= rhof;
+                    tustep = M<R>::fmin(M<R>::fmin(tstep, tmxs), range);
+                    tperp = hownear(p);
+                    R blccl = rhof * m.blcc, xccl = rhof * m.xcc;
+                    R p2 = eke * (eke + (R)2.0 * RM);
+                    R beta2 = p2 / (p2 + RM * RM);
+                    R etap = iq < 0 ? pwlf(t.etae[ix], elke) : pwlf(t.etap[ix], elke);
+                    R ms_corr = pwlf(t.blcce[ix], elke);
+                    blccl = blccl / etap / ((R)1.0 + (R)0.25 * etap * xccl / blccl / p2) * ms_corr;
+                    R ssmfp = beta2 / blccl;
+                    R skindepth = (R)3 * ssmfp;
+                    tustep = M<R>::fmin(tustep, M<R>::fmax(tperp, skindepth));
+                    if ((tustep <= tperp) && (tustep > skindepth)) {
+                        called_msdist = true;
+                        de = eloss(imed, iq, rhof, tustep, range, eke, elke, lelke);
+                        tvstep = tustep;
+                        ustep = msdist(p, imed, iq, rhof, de, tustep, eke, fin);
+                    } else {
+                        R rn = rnd();
+                        if (rn < (R)1.0E-30) rn = (R)1.0E-30;
+                        R lambda = (R)(-1.0) * M<R>::log((R)1.0 - rn);
+                        R lambda_max = (R)0.5 * blccl * RM / dedx;
+                        R ek1 = eke / RM + (R)1.0;
+                        lambda_max *= ek1 * ek1 * ek1;
+                        if (lambda >= (R)0.0 && lambda_max > (R)0.0) {
+                            R tuss;
+                            if (lambda < lambda_max) tuss = lambda * ssmfp * ((R)1.0 - (R)0.5 * lambda / lambda_max);
+                            else tuss = (R)0.5 * lambda * ssmfp;
+                            if (tuss < tustep) {
+                                tustep = tuss;
+                                do_single = true;
+                            }
+                        } else {
+                            return -1;       // reference warns and drops (quirk Q6)
+                        }
+                        ustep = tustep;
+                        call_howfar = !(ustep < tperp);
+                    }
+                }
+                int mv = 0;
+                bool leaves = false;
+                if (call_howfar) {
+                    if (p.ix < 0) {
+                        if (count) cnt.nhowfar++;
+                        return electron_stop(p, q, iq, eie, true);
+                    }
+                    mv = howfar(p, ustep, leaves);
+                }
+                if (ustep < (R)0) ustep = (R)0.0;
+                R vstep;
+                if (ustep == (R)0.0 || imed == -1) {
+                    if (ustep != (R)0.0) {
+                        vstep = ustep;
+                        p.x += p.u * vstep;
+                        p.y += p.v * vstep;
+                        p.z += p.w * vstep;
+                    }
+                    if (mv != 0) {
+                        apply_move(p, mv, leaves);
+                        imed = medium(p);
+                    }
+                    if (eie <= ecut_of(imed)) return electron_stop(p, q, iq, eie, false);
+                    break;
+                }
+                vstep = ustep;
+                if (call_howfar) {
+                    tvstep = vstep;
+                    if (tvstep != tustep) do_single = false;
+                    de = eloss(imed, iq, rhof, tvstep, range, eke, elke, lelke);
+                } else {
+                    tvstep = tustep;
+                    if (!called_msdist) de = eloss(imed, iq, rhof, tvstep, range, eke, elke, lelke);
+                }
+                R edep = de;
+                R ekef = eke - de;
+                if (!called_msdist) {
+                    R costhe = (R)1.0, sinthe = (R)0.0;
+                    if (do_single) {
+                        const Med<R> &m = t.med[imed];
+                        R ekems = M<R>::fmax(ekef, g.ecut[imed] - RM);
+                        R p2 = ekems * (ekems + (R)2.0 * RM);
+                        R beta2 = p2 / (p2 + RM * RM);
+                        R chia2 = m.xcc / ((R)4.0 * m.blcc * p2);
+                        R elkems = M<R>::log(ekems);
+                        int lelkems = pwlf_i(elkems, m.eke1, m.eke0) - 1;
+                        R etap = iq < 0 ? pwlf(t.etae[MXEKE * imed + lelkems], elkems)
+                                        : pwlf(t.etap[MXEKE * imed + lelkems], elkems);
+                        chia2 *= etap;
+                        sscat(imed, qel, chia2, elkems, beta2, costhe, sinthe);
+                    }
+                    fin.x = p.x + p.u * vstep;
+                    fin.y = p.y + p.v * vstep;
+                    fin.z = p.z + p.w * vstep;
+                    if (do_single) {
+                        Uphi qf;
+                        uphi21(p, qf, costhe, sinthe);
+                    }
+                    fin.u = p.u; fin.v = p.v; fin.w = p.w;
+                }
+                ausgab(p, edep);
+                p.x = fin.x; p.y = fin.y; p.z = fin.z;
+                p.u = fin.u; p.v = fin.v; p.w = fin.w;
+                eie -= edep;
+                p.e = eie;
+                if (mv == 0 && eie <= ecut_of(imed)) return electron_stop(p, q, iq, eie, false);
+                medold = imed;
+                if (imed != -1) {
+                    const Med<R> &m = t.med[imed];
+                    eke = eie - RM;
+                    elke = M<R>::log(eke);
+                    lelke = pwlf_i(elke, m.eke1, m.eke0) - 1;
+                }
+                if (mv != 0) {
+                    apply_move(p, mv, leaves);
+                    imed = medium(p);
+                }
+                if (eie <= ecut_of(imed)) return electron_stop(p, q, iq, eie, false);
+                if (imed != medold) break;
+                demfp -= de * sig0;
+                total_tstep -= tvstep * rhof;
+                if (total_tstep < (R)1.0E-9) demfp = (R)0.0;
+            } while (demfp >= (R)1.0E-5);
+            if ((imed != medold) || (ustep == (R)0.0) || (imed == -1)) continue;
+            int ix = imed * MXEKE + lelke;
+            R sigf = iq < 0 ? pwlf(t.esig[ix], elke) / pwlf(t.ededx[ix], elke)
+                            : pwlf(t.psig[ix], elke) / pwlf(t.pdedx[ix], elke);
+            sigratio = sigf / sig0;
+            rfict = rnd();
+        } while (rfict >= sigratio);
+        int ix = imed * MXEKE + lelke;
+        if (iq < 0) {
+            R ebr1 = pwlf(t.ebr1[ix], elke);
+            R rn = rnd();
+            if (rn <= ebr1) {
+                brems(p, q);
+                return 1;
+            }
+            if (p.e <= t.med[imed].thmoll) {
+                if (ebr1 <= (R)0) return 0;
+                brems(p, q);
+                return 1;
+            }
+            return moller(p, q);
+        }
+        R pbr1 = pwlf(t.pbr1[ix], elke);
+        R rn = rnd();
+        if (rn < pbr1) {
+            brems(p, q);
+            return 1;
+        }
+        R pbr2 = pwlf(t.pbr2[ix], elke);
+        if (rn < pbr2) bhabha(p, q);
+        else annih(p, q);
+        return 1;
+    }
+
+    // one pass of shower() (src/ompmc.c:5454-5465) on the particle on top:
+    // returns the change of stack depth (-1, 0, +1); q receives the new top
+    __device__ __forceinline__ int step(Part<R> &p, Part<R> &q) {
+        return p.iq == 0 ? photon(p, q) : electron(p, q);
+    }
+};
+
+}  // namespace omc

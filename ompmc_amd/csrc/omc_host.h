@@ -51,20 +51,21 @@ extern "C" {
 #define OMC_INVDIM 1000
 #define OMC_PI 3.14159265358979323846
 
-/* Error codes returned by every entry point (0 = success). */
-enum {
-    OMC_OK = 0,
-    OMC_ERR_IO = 1,        /* file missing or unreadable */
-    OMC_ERR_FORMAT = 2,    /* malformed file content */
-    OMC_ERR_MEDIUM = 3,    /* medium not found in the PEGS4 file */
-    OMC_ERR_LIMIT = 4,     /* a compile-time limit of the reference exceeded */
-    OMC_ERR_ARG = 5,       /* invalid argument */
-    OMC_ERR_DEVICE = 6,    /* HIP runtime failure */
-    OMC_ERR_NOMEM = 7,     /* host or device allocation failed */
-    OMC_ERR_BUDGET = 8,    /* request exceeds the HBM admission budget */
-    OMC_ERR_STACK = 9,     /* particle stack overflow */
-    OMC_ERR_CANCELLED = 10
-};
+/* Error codes returned by every entry point (0 = success); the same values
+ * as include/omc_gpu.h. */
+#ifndef OMC_OK
+#define OMC_OK 0
+#define OMC_ERR_IO 1        /* file missing or unreadable */
+#define OMC_ERR_FORMAT 2    /* malformed file content */
+#define OMC_ERR_MEDIUM 3    /* medium not found in the PEGS4 file */
+#define OMC_ERR_LIMIT 4     /* a compile-time limit of the reference exceeded */
+#define OMC_ERR_ARG 5       /* invalid argument */
+#define OMC_ERR_DEVICE 6    /* HIP runtime failure */
+#define OMC_ERR_NOMEM 7     /* host or device allocation failed */
+#define OMC_ERR_BUDGET 8    /* request exceeds the HBM admission budget */
+#define OMC_ERR_STACK 9     /* particle stack overflow */
+#define OMC_ERR_CANCELLED 10
+#endif
 
 /* Last error message of the calling thread. */
 const char *omc_host_last_error(void);
@@ -158,7 +159,7 @@ typedef struct omc_paths {
 } omc_paths;
 
 /* Builds every table for the named media (reference initMediaData,
- * src/ompmc.c:5468-5505).  `quiet` suppresses the informational prints. */
+ * src/ompmc.c:5468-5505). */
 int omc_tables_build(omc_tables *t, int nmed, const char names[][60], const omc_paths *paths);
 void omc_tables_free(omc_tables *t);
 

@@ -4,6 +4,7 @@ File = b"OMCG" followed by records:
   b"OMCA" name[32] int32 dtype(0 f64, 1 i32) int64 n  data      -> named array
   b"OMCC" int32 kind, n_in, n_rn, n_out  f64 in[] rn[] out[]     -> sampler case
 """
+import gzip
 import struct
 
 import numpy as np
@@ -16,7 +17,8 @@ KIND_ID = {v: k for k, v in KINDS.items()}
 
 def read(path):
     """Returns (arrays: dict name -> ndarray, cases: list of (kind, in, rn, out))."""
-    with open(path, "rb") as f:
+    opener = gzip.open if path.endswith(".gz") else open
+    with opener(path, "rb") as f:
         buf = f.read()
     if buf[:4] != b"OMCG":
         raise ValueError(f"{path}: not a golden file")

@@ -1,0 +1,180 @@
+"""The oracle (oracle/omc_oracle.c, FP64 CPU restatement) against the
+reference's own golden vectors, plus known answers and kinematic identities
+that pin it independently of the reference.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+from tests.conftest import DOSE_CASES, golden, write_case_inp
+from tests.golden import golden_io
+
+B = pytest.importorskip("oracle.bindings")
+
+RM = 0.5109989461
+
+
+@pytest.fixture(scope="module")
+def small_session(tmp_path_factory):
+    stem = write_case_inp(tmp_path_factory.mktemp("small"), "small")
+    return B.Session(stem)
+
+
+def test_ranmar_marsaglia_zaman_kat():
+    # Marsaglia & Zaman's published test: seeds 1802/9373, after 20000 draws
+    # the next six values times 2^24
+    got = B.ranmar_stream(1802, 9373, 20000, 6)
+    assert list(got) == [6533892, 14220222, 7275067, 6172232, 8354498, 10633180]
+
+
+# arrays whose last entry the reference never initialises (or reads from
+# uninitialised memory): compared without those entries
+_SKIP_TAIL = {"wms": 32, "ims": 32}
+_SKIP_LAST = {"ray_b": 100, "ray_c": 100}
+
+
+def test_tables_bitwise_equal_to_reference(small_session):
+    arrays, _ = golden_io.read(golden("tables_small.bin.gz"))
+    checked = 0
+    for name, ref in arrays.items():
+        if name in ("spin_scalars", "mscat_scalars", "pegs_f", "pegs_i", "elements"):
+            continue
+        mine = small_session.table(name)
+        assert mine.shape == ref.shape, name
+        if name in _SKIP_TAIL:
+            w = _SKIP_TAIL[name]
+            ref, mine = ref.reshape(-1, w)[:, :w - 1], mine.reshape(-1, w)[:, :w - 1]
+        if name in _SKIP_LAST:
+            w = _SKIP_LAST[name]
+            ref, mine = ref.reshape(-1, w)[:, :w - 1], mine.reshape(-1, w)[:, :w - 1]
+        if name in ("etap_ms0", "etap_ms1"):
+            # top three bins of the positron screening table: the reference
+            # interpolates with uninitialised memory there (documented)
+            ref, mine = ref.reshape(-1, 500)[:, :147], mine.reshape(-1, 500)[:, :147]
+        assert np.array_equal(ref.view(np.uint8), np.ascontiguousarray(mine).view(np.uint8)), name
+        checked += 1
+    assert checked >= 70
+    sc = small_session.scalars()
+    assert np.array_equal(sc[:5], arrays["spin_scalars"])
+    assert np.array_equal(sc[5:], arrays["mscat_scalars"])
+    f, i = small_session.pegs(4)
+    assert np.array_equal(f, arrays["pegs_f"]) and np.array_equal(i, arrays["pegs_i"])
+
+
+def _replay_all(session, path):
+    _, cases = golden_io.read(path)
+    bad = {}
+    n = {}
+    for kind, inp, rn, out in cases:
+        name = golden_io.KINDS[kind]
+        n[name] = n.get(name, 0) + 1
+        mine, used = session.case(kind, inp, rn)
+        ok = mine is not None and used == len(rn) and len(mine) == len(out) and \
+            np.array_equal(mine.view(np.uint64), out.view(np.uint64))
+        if not ok:
+            bad[name] = bad.get(name, 0) + 1
+    return n, bad
+
+
+def test_samplers_bitwise_neutral_spin(small_session):
+    n, bad = _replay_all(small_session, golden("samplers_neutral.bin.gz"))
+    assert len(n) == 16 and min(n.values()) >= 100
+    assert bad == {}
+
+
+def test_samplers_bitwise_textured_spin(tmp_path, textured_data):
+    s = B.Session(write_case_inp(tmp_path, "small", data_folder=textured_data))
+    n, bad = _replay_all(s, golden("samplers_textured.bin.gz"))
+    assert bad == {}
+    # the textured surfaces make the spin rejection loops reject: more draws
+    # than the neutral file for the same scattering routines
+    _, cases_t = golden_io.read(golden("samplers_textured.bin.gz"))
+    _, cases_n = golden_io.read(golden("samplers_neutral.bin.gz"))
+    def draws(cases, kind):
+        return sum(len(rn) for k, _, rn, _ in cases if k == kind)
+    kinds = [golden_io.KIND_ID[k] for k in ("mscat", "msdist", "sscat")]
+    assert sum(draws(cases_t, k) for k in kinds) > sum(draws(cases_n, k) for k in kinds)
+
+
+@pytest.mark.parametrize("name", sorted(DOSE_CASES))
+def test_whole_run_bitwise(tmp_path, name):
+    """Serial runs on one RANMAR stream: per-history random-number counts,
+    batch sums, dose and uncertainty equal the reference's bit for bit."""
+    arrays, _ = golden_io.read(golden(f"dose_{name}.bin.gz"))
+    _, nh, nb = DOSE_CASES[name]
+    s = B.Session(write_case_inp(tmp_path, name))
+    s1, s2, cnt = s.dose(nh // nb, nb, want_counts=True)
+    assert np.array_equal(cnt, arrays["rncount"])
+    assert np.array_equal(s1.view(np.uint64), arrays["accum_endep"].view(np.uint64))
+    assert np.array_equal(s2.view(np.uint64), arrays["accum_endep2"].view(np.uint64))
+    d, u = s.finish(s1, s2, nh // nb, nb)
+    assert np.array_equal(d.view(np.uint64), arrays["dose"].view(np.uint64))
+    assert np.array_equal(u.view(np.uint64), arrays["dose_unc"].view(np.uint64))
+
+
+# ---------------------------------------------------------------- identities
+def _cases(kind):
+    _, cases = golden_io.read(golden("samplers_neutral.bin.gz"))
+    return [(i, r, o) for k, i, r, o in cases if k == golden_io.KIND_ID[kind]]
+
+
+def test_compton_kinematics():
+    """E' = E / (1 + k (1 - cos theta)) and the electron takes the rest."""
+    for inp, _, out in _cases("compton"):
+        e0, u0 = inp[2], inp[6:9]
+        ph, el = out[1:12], out[12:23]
+        cos = float(np.dot(u0, ph[6:9]))
+        k = e0 / RM
+        assert ph[2] == pytest.approx(e0 / (1.0 + k * (1.0 - cos)), rel=1e-9, abs=1e-12)
+        assert el[2] == pytest.approx(e0 - ph[2] + RM, rel=1e-12)
+        assert el[0] == -1 and ph[0] == 0
+        for p in (ph, el):
+            assert np.linalg.norm(p[6:9]) == pytest.approx(1.0, abs=1e-9)
+
+
+def test_moller_bhabha_energy_and_angles():
+    """Two-body kinematics: energies add up and cos^2 theta follows
+    h1 (E_i - m) / (E_i + m) for each outgoing lepton."""
+    for kind in ("moller", "bhabha"):
+        for inp, _, out in _cases(kind):
+            if out[0] < 1:          # Moller below threshold: no interaction
+                continue
+            e0, u0 = inp[2], inp[6:9]
+            a, b = out[1:12], out[12:23]
+            assert a[2] + b[2] == pytest.approx(e0 + RM, rel=1e-12)
+            h1 = (e0 + RM) / (e0 - RM)
+            for p in (a, b):
+                cos = float(np.dot(u0, p[6:9]))
+                c2 = min(1.0, h1 * (p[2] - RM) / (p[2] + RM))
+                assert cos * abs(cos) == pytest.approx(c2, abs=1e-8)
+
+
+def test_annihilation_at_rest_back_to_back():
+    for inp, _, out in _cases("rannih"):
+        a, b = out[1:12], out[12:23]
+        assert a[2] == RM and b[2] == RM and a[0] == 0 and b[0] == 0
+        assert np.allclose(a[6:9], -b[6:9], atol=0)
+        assert np.linalg.norm(a[6:9]) == pytest.approx(1.0, abs=1e-12)
+
+
+def test_pair_and_annihilation_in_flight_conserve_energy():
+    for inp, _, out in _cases("pair"):
+        assert out[1 + 2] + out[12 + 2] == pytest.approx(inp[2], rel=1e-12)
+        assert sorted([out[1], out[12]]) == [-1, 1]
+    for inp, _, out in _cases("annih"):
+        assert out[1 + 2] + out[12 + 2] == pytest.approx(inp[2] + RM, rel=1e-12)
+
+
+def test_drange_matches_quadrature(small_session):
+    """computeDrange against numerical integration of 1/(dE/dx) with the
+    log-linear stopping power of the same PWLF interval."""
+    ed0, ed1 = small_session.table("ededx0"), small_session.table("ededx1")
+    pd0, pd1 = small_session.table("pdedx0"), small_session.table("pdedx1")
+    for inp, _, out in _cases("drange"):
+        imed, iq, lel, ekei, ekef = int(inp[0]), int(inp[1]), int(inp[2]), inp[3], inp[4]
+        c0, c1 = (ed0, ed1) if iq < 0 else (pd0, pd1)
+        i = imed * 500 + lel
+        e = np.linspace(ekef, ekei, 2001)
+        f = 1.0 / (c1[i] * np.log(e) + c0[i])
+        quad = float(np.sum(0.5 * (f[1:] + f[:-1]) * np.diff(e)))
+        assert out[0] == pytest.approx(quad, rel=2e-6, abs=1e-12)
