@@ -111,15 +111,37 @@ typedef struct omc_run_opts {
     const int64_t *replay_offsets;/* first draw of each history, nhist+1 entries */
     int64_t replay_len;
     int grid, block;        /* 0 = automatic */
-    int count_events;       /* fill omc_run_stats counters */
+    int count_events;       /* fill the event counters and per-kernel times of omc_run_stats */
+    int64_t event_target;   /* event form: particles in flight per round (0 = automatic) */
+    int electron_budget;    /* event form: condensed-history steps per charged-particle event (0 = automatic) */
+    int group;              /* Dij: beamlets transported together (0 = as many as hbm_budget allows,
+                               1 = one beamlet at a time, the reference's loop order) */
+    int64_t hbm_budget;     /* device bytes a Dij run may use for its tallies (0 = automatic) */
 } omc_run_opts;
+
+/* Kernel classes of omc_run_stats.kernel_seconds / kernel_launches */
+#define OMC_K_HISTORY 0     /* plain history kernel */
+#define OMC_K_SOURCE 1      /* event form: primaries */
+#define OMC_K_PHOTON 2      /* event form: photon events */
+#define OMC_K_CHARGED 3     /* event form: electron/positron events */
+#define OMC_K_OTHER 4       /* batch statistics, Dij finish and compaction */
 
 typedef struct omc_run_stats {
     double gpu_seconds;             /* device time of the transport kernels */
     double wall_seconds;            /* whole call */
     double histories;
     uint64_t n_howfar, n_hownear, n_cross, n_dep, n_rn, n_photon, n_electron;
-    int errors;                     /* bit 0 stack overflow, 1 replay underrun, 2 step guard */
+    int errors;                     /* bit 0 stack or pool overflow, 1 replay underrun, 2 step guard */
+    /* count_events runs: device time and launches per kernel class, and the
+     * howfar/hownear/crossing/deposit counts of the photon and charged-particle
+     * event kernels ([0] photon, [1] charged) */
+    double kernel_seconds[5];
+    uint64_t kernel_launches[5];
+    uint64_t ev_howfar[2], ev_hownear[2], ev_cross[2], ev_dep[2];
+    /* event form: events processed, rounds, largest number of particles in flight */
+    uint64_t events_photon, events_charged, rounds, peak_inflight;
+    /* Dij: beamlet groups, beamlets per group, non-zeros */
+    int64_t groups, group_size, nnz;
 } omc_run_stats;
 
 /* Defaults: FP32, event form, Philox, seed 1802, nsplit 1, automatic launch. */
@@ -169,6 +191,13 @@ void omc_csc_free(omc_csc *m);
 int omc_run_dij(omc_ctx *ctx, int phantom_id, const omc_spectrum_desc *spec, const omc_beamlets_desc *bix,
                 int nbatch, double rel_threshold, const omc_run_opts *opts, omc_csc *dij, omc_csc *var,
                 omc_run_stats *stats);
+/* omc_run_dij with two optional outputs of nvox doubles each: the dose of
+ * all beamlets summed with unit weights and the variance of that sum (sum of
+ * the per-beamlet batch variances without the carry-over); no counterpart in
+ * the reference, for plan-level statistics. */
+int omc_run_dij_ex(omc_ctx *ctx, int phantom_id, const omc_spectrum_desc *spec, const omc_beamlets_desc *bix,
+                   int nbatch, double rel_threshold, const omc_run_opts *opts, omc_csc *dij, omc_csc *var,
+                   double *plan_dose, double *plan_var, omc_run_stats *stats);
 
 /* Injected-random-number sampler evaluation for verification: ncase cases of
  * the golden-vector layout (kinds 1..16: compton, pair, photo, rayleigh,

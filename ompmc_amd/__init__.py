@@ -57,17 +57,27 @@ class DosxyzSourceDesc(C.Structure):
 class RunOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("form", C.c_int), ("rng", C.c_int), ("seed", C.c_uint64),
                 ("nsplit", C.c_int), ("replay", _u32), ("replay_offsets", _i64), ("replay_len", C.c_int64),
-                ("grid", C.c_int), ("block", C.c_int), ("count_events", C.c_int)]
+                ("grid", C.c_int), ("block", C.c_int), ("count_events", C.c_int), ("event_target", C.c_int64),
+                ("electron_budget", C.c_int), ("group", C.c_int), ("hbm_budget", C.c_int64)]
 
 
 class RunStats(C.Structure):
     _fields_ = [("gpu_seconds", C.c_double), ("wall_seconds", C.c_double), ("histories", C.c_double),
                 ("n_howfar", C.c_uint64), ("n_hownear", C.c_uint64), ("n_cross", C.c_uint64),
                 ("n_dep", C.c_uint64), ("n_rn", C.c_uint64), ("n_photon", C.c_uint64),
-                ("n_electron", C.c_uint64), ("errors", C.c_int)]
+                ("n_electron", C.c_uint64), ("errors", C.c_int),
+                ("kernel_seconds", C.c_double * 5), ("kernel_launches", C.c_uint64 * 5),
+                ("ev_howfar", C.c_uint64 * 2), ("ev_hownear", C.c_uint64 * 2), ("ev_cross", C.c_uint64 * 2),
+                ("ev_dep", C.c_uint64 * 2), ("events_photon", C.c_uint64), ("events_charged", C.c_uint64),
+                ("rounds", C.c_uint64), ("peak_inflight", C.c_uint64), ("groups", C.c_int64),
+                ("group_size", C.c_int64), ("nnz", C.c_int64)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            out[k] = list(v) if hasattr(v, "_length_") else v
+        return out
 
 
 class BeamletsDesc(C.Structure):
@@ -79,6 +89,26 @@ class BeamletsDesc(C.Structure):
 class CSC(C.Structure):
     _fields_ = [("nrows", C.c_int64), ("ncols", C.c_int64), ("nnz", C.c_int64), ("jc", _i64), ("ir", _i64),
                 ("val", _d)]
+
+    def to_numpy(self):
+        """(jc, ir, val) copies; the library arrays stay owned by the struct."""
+        jc = np.ctypeslib.as_array(self.jc, shape=(self.ncols + 1,)).copy()
+        n = int(self.nnz)
+        ir = np.ctypeslib.as_array(self.ir, shape=(n,)).copy() if n else np.zeros(0, np.int64)
+        val = np.ctypeslib.as_array(self.val, shape=(n,)).copy() if n else np.zeros(0)
+        return jc, ir, val
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("phantom", PhantomDesc), ("dose_src", DosxyzSourceDesc),
+                ("nperbatch", C.c_int64), ("spec", SpectrumDesc), ("bix", BeamletsDesc),
+                ("rel_threshold", C.c_double), ("nbatch", C.c_int), ("seed", C.c_uint64), ("precision", C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [("state", C.c_int), ("error", C.c_int), ("sum", _d), ("sum2", _d), ("nreg", C.c_int64),
+                ("dij", CSC), ("submit_t", C.c_double), ("start_t", C.c_double), ("end_t", C.c_double),
+                ("histories", C.c_double)]
 
 
 _lib = None
@@ -115,7 +145,16 @@ def lib():
     L.omc_sampler_eval.argtypes = [vp, C.c_int, C.c_int, C.c_int, _i, _d, C.c_int, _u32, _i64, _d, C.c_int, _i64]
     L.omc_run_dij.argtypes = [vp, C.c_int, C.POINTER(SpectrumDesc), C.POINTER(BeamletsDesc), C.c_int, C.c_double,
                               C.POINTER(RunOpts), C.POINTER(CSC), C.POINTER(CSC), C.POINTER(RunStats)]
+    L.omc_run_dij_ex.argtypes = [vp, C.c_int, C.POINTER(SpectrumDesc), C.POINTER(BeamletsDesc), C.c_int,
+                                 C.c_double, C.POINTER(RunOpts), C.POINTER(CSC), C.POINTER(CSC), _d, _d,
+                                 C.POINTER(RunStats)]
     L.omc_csc_free.argtypes = [C.POINTER(CSC)]
+    L.omc_server_create.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(vp)]
+    L.omc_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_int64)]
+    L.omc_server_poll.argtypes = [vp, C.c_int64, _i]
+    L.omc_server_wait.argtypes = [vp, C.c_int64, C.POINTER(Result)]
+    L.omc_result_free.argtypes = [C.POINTER(Result)]
+    L.omc_server_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -139,11 +178,13 @@ def default_paths():
 
 
 def run_opts(precision=FP32, form=FORM_EVENT, seed=1802, rng=RNG_PHILOX, replay=None, replay_offsets=None,
-             grid=0, block=0, count_events=False, nsplit=1):
+             grid=0, block=0, count_events=False, nsplit=1, event_target=0, electron_budget=0, group=0,
+             hbm_budget=0):
     o = RunOpts()
     lib().omc_run_opts_default(C.byref(o))
     o.precision, o.form, o.seed, o.rng, o.nsplit = precision, form, seed, rng, nsplit
     o.grid, o.block, o.count_events = grid, block, int(bool(count_events))
+    o.event_target, o.electron_budget, o.group, o.hbm_budget = event_target, electron_budget, group, hbm_budget
     keep = []
     if replay is not None:
         rp = np.ascontiguousarray(replay, np.uint32)
@@ -290,9 +331,55 @@ class Context:
                                      nperbatch, nbatch, _dp(dose), _dp(unc)))
         return dose, unc
 
-    def sampler_eval(self, phantom: Phantom, precision, kinds, inputs, rn_lists, in_stride=24, out_stride=128):
+    def run_dij(self, phantom: Phantom, spectrum: Spectrum, bix: "phantoms.Beamlets", nbatch=10, rel_threshold=1e-4,
+                nhist_each=0, want_var=False, want_plan=False, gaussian=False, sigma=0.0, **opts):
+        """omc_matrad dose-influence matrix: returns (dij, var, plan, stats)
+        with dij/var as (jc, ir, val) (var None unless asked; 0-based voxel
+        rows) and plan = (beamlet-summed dose, its variance) when asked."""
+        keep = []
+        b = BeamletsDesc()
+        b.nbixels, b.nbeams = bix.n, len(bix.src)
+        arrs = dict(ibeam=np.ascontiguousarray(bix.ibeam, np.int32),
+                    src=np.ascontiguousarray(bix.src, float).reshape(-1),
+                    corner=np.ascontiguousarray(bix.corner, float).reshape(-1),
+                    side1=np.ascontiguousarray(bix.side1, float).reshape(-1),
+                    side2=np.ascontiguousarray(bix.side2, float).reshape(-1))
+        keep += list(arrs.values())
+        b.ibeam = _ip(arrs["ibeam"])
+        b.src, b.corner, b.side1, b.side2 = (_dp(arrs[k]) for k in ("src", "corner", "side1", "side2"))
+        b.gaussian, b.sigma = int(bool(gaussian)), float(sigma)
+        if nhist_each:
+            b.nhist_each = int(nhist_each)
+        else:
+            nh = np.ascontiguousarray(bix.nhist, np.int64)
+            keep.append(nh)
+            b.nhist = nh.ctypes.data_as(_i64)
+        o, k2 = run_opts(**opts)
+        keep += k2
+        sp = spectrum.desc()
+        dij, var = CSC(), CSC()
+        nv = phantom.nreg - 1
+        pd = np.zeros(nv) if want_plan else None
+        pv = np.zeros(nv) if want_plan else None
+        st = RunStats()
+        _check(lib().omc_run_dij_ex(self.h, phantom.id, C.byref(sp), C.byref(b), nbatch, rel_threshold, C.byref(o),
+                                    C.byref(dij), C.byref(var) if want_var else None,
+                                    _dp(pd) if want_plan else None, _dp(pv) if want_plan else None, C.byref(st)))
+        out = dij.to_numpy()
+        lib().omc_csc_free(C.byref(dij))
+        vout = None
+        if want_var:
+            vout = var.to_numpy()
+            lib().omc_csc_free(C.byref(var))
+        del keep
+        return out, vout, ((pd, pv) if want_plan else None), st
+
+    def sampler_eval(self, phantom: Phantom, precision, kinds, inputs, rn_lists, in_stride=24, out_stride=64 + 2 * 512):
         """Evaluates golden-vector cases with injected random numbers on the
-        GPU.  rn_lists: per case, the draws as floats k*2^-24."""
+        GPU.  rn_lists: per case, the draws as floats k*2^-24.  Output row:
+        [0] stack depth change (-1, 0, 1; 2 for scalar routines), [1:12] and
+        [12:23] the particles (iq ir e x y z u v w wt -), [23] tally calls,
+        [24] howfar and [25] hownear calls, [64:] (region, wt*edep) per tally."""
         n = len(kinds)
         kind = np.ascontiguousarray(kinds, np.int32)
         inp = np.zeros((n, in_stride))
@@ -308,3 +395,101 @@ class Context:
                                       rn.ctypes.data_as(_u32), off.ctypes.data_as(_i64), _dp(out), out_stride,
                                       used.ctypes.data_as(_i64)))
         return out, used
+
+
+class Server:
+    """Resident request loop on a context's device (omc_server_*): requests are
+    admitted in arrival order while their device memory fits the budget and
+    fewer than ``max_inflight`` run; admitted requests run concurrently."""
+
+    def __init__(self, ctx: Context, hbm_budget=0, max_inflight=4):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(lib().omc_server_create(ctx.h, int(hbm_budget), int(max_inflight), C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def _phantom_desc(self, ph, ecut, pcut, keep):
+        xb, yb, zb = (np.ascontiguousarray(a, float) for a in (ph.xb, ph.yb, ph.zb))
+        med = np.ascontiguousarray(ph.med.reshape(-1), np.int32)
+        rho = np.ascontiguousarray(ph.rho.reshape(-1), float)
+        keep += [xb, yb, zb, med, rho]
+        return PhantomDesc(len(xb) - 1, len(yb) - 1, len(zb) - 1, _dp(xb), _dp(yb), _dp(zb), _ip(med), _dp(rho),
+                           ecut, pcut)
+
+    def submit_dose(self, ph, spectrum, nperbatch, nbatch=10, collimator=(-5, 5, -5, 5), ssd=1.0e10, charge=0,
+                    seed=1802, precision=FP32, ecut=0.7, pcut=0.01):
+        keep = []
+        r = Request()
+        r.kind = 0
+        r.phantom = self._phantom_desc(ph, ecut, pcut, keep)
+        r.dose_src.spec = spectrum.desc()
+        keep.append(spectrum)
+        r.dose_src.charge, r.dose_src.ssd = charge, ssd
+        for k in range(4):
+            r.dose_src.collimator[k] = collimator[k]
+        r.nperbatch, r.nbatch, r.seed, r.precision = int(nperbatch), int(nbatch), int(seed), precision
+        rid = C.c_int64(0)
+        _check(lib().omc_server_submit(self.h, C.byref(r), C.byref(rid)))
+        return rid.value
+
+    def submit_dij(self, ph, spectrum, bix, nbatch=10, rel_threshold=1e-4, seed=1802, precision=FP32, ecut=0.7,
+                   pcut=0.01, nhist_each=0):
+        keep = []
+        r = Request()
+        r.kind = 1
+        r.phantom = self._phantom_desc(ph, ecut, pcut, keep)
+        r.spec = spectrum.desc()
+        keep.append(spectrum)
+        arrs = [np.ascontiguousarray(bix.ibeam, np.int32), np.ascontiguousarray(bix.src, float).reshape(-1),
+                np.ascontiguousarray(bix.corner, float).reshape(-1), np.ascontiguousarray(bix.side1, float).reshape(-1),
+                np.ascontiguousarray(bix.side2, float).reshape(-1)]
+        keep += arrs
+        r.bix.nbixels, r.bix.nbeams = bix.n, len(bix.src)
+        r.bix.ibeam = _ip(arrs[0])
+        r.bix.src, r.bix.corner, r.bix.side1, r.bix.side2 = (_dp(a) for a in arrs[1:])
+        if nhist_each:
+            r.bix.nhist_each = int(nhist_each)
+        else:
+            nh = np.ascontiguousarray(bix.nhist, np.int64)
+            keep.append(nh)
+            r.bix.nhist = nh.ctypes.data_as(_i64)
+        r.rel_threshold, r.nbatch, r.seed, r.precision = rel_threshold, int(nbatch), int(seed), precision
+        rid = C.c_int64(0)
+        _check(lib().omc_server_submit(self.h, C.byref(r), C.byref(rid)))
+        return rid.value
+
+    def poll(self, rid):
+        st = C.c_int(0)
+        _check(lib().omc_server_poll(self.h, rid, C.byref(st)))
+        return st.value
+
+    def wait(self, rid):
+        """Returns a dict: state, error, times (submit/start/end on the server
+        clock), histories, and sum/sum2 (dose requests) or dij=(jc, ir, val)."""
+        res = Result()
+        rc = lib().omc_server_wait(self.h, rid, C.byref(res))
+        out = dict(state=res.state, error=res.error, submit_t=res.submit_t, start_t=res.start_t, end_t=res.end_t,
+                   histories=res.histories)
+        if rc == 0:
+            if res.sum:
+                n = int(res.nreg)
+                out["sum"] = np.ctypeslib.as_array(res.sum, shape=(n,)).copy()
+                out["sum2"] = np.ctypeslib.as_array(res.sum2, shape=(n,)).copy()
+            if res.dij.jc:
+                out["dij"] = res.dij.to_numpy()
+        else:
+            out["message"] = lib().omc_last_error().decode()
+        lib().omc_result_free(C.byref(res))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().omc_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

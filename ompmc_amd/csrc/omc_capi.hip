@@ -234,7 +234,7 @@ template <class R> static void fill_geom(omc_ctx *c, PhantomDev *pd, Geom<R> &G)
 
 size_t omc_phantom_bytes(const omc_phantom_desc *d) {
     size_t nvox = (size_t)d->nx * d->ny * d->nz;
-    return nvox * (1 + 4 + 8) + 16 * (size_t)(d->nx + d->ny + d->nz + 3);
+    return nvox * (1 + 4 + 8 + 16) + 16 * (size_t)(d->nx + d->ny + d->nz + 3);
 }
 
 int omc_phantom_make(omc_ctx *c, const omc_phantom_desc *d, PhantomDev **out) {
@@ -267,6 +267,20 @@ int omc_phantom_make(omc_ctx *c, const omc_phantom_desc *d, PhantomDev **out) {
     pd->dmed = pd->mem.upload(m8);
     fill_geom<float>(c, pd, pd->g32);
     fill_geom<double>(c, pd, pd->g64);
+    {
+        // voxel masses as the reference's accumulateResults forms them (omc_matrad.c:944-950)
+        std::vector<double> mass(nvox), rho(ph.rho, ph.rho + nvox);
+        for (int iz = 0; iz < ph.nz; iz++)
+            for (int iy = 0; iy < ph.ny; iy++)
+                for (int ix = 0; ix < ph.nx; ix++) {
+                    long v = ix + (long)iy * ph.nx + (long)iz * ph.nx * ph.ny;
+                    double m = (ph.xb[ix + 1] - ph.xb[ix]) * (ph.yb[iy + 1] - ph.yb[iy]) * (ph.zb[iz + 1] - ph.zb[iz]);
+                    m *= ph.rho[v];
+                    mass[v] = m;
+                }
+        pd->dmass = pd->mem.upload(mass);
+        pd->drho = pd->mem.upload(rho);
+    }
     if (!pd->mem.ok) {
         pd->mem.release();
         omc_phantom_free(&pd->host);
@@ -326,52 +340,17 @@ int omc_check_opts(const omc_run_opts *o) {
     if (o->form != OMC_FORM_HISTORY && o->form != OMC_FORM_EVENT) return omc_seterr(OMC_ERR_ARG, "form %d", o->form);
     if (o->rng == OMC_RNG_REPLAY && (!o->replay || !o->replay_offsets))
         return omc_seterr(OMC_ERR_ARG, "replay random numbers requested without a stream");
+    if (o->rng == OMC_RNG_REPLAY && o->form != OMC_FORM_HISTORY)
+        return omc_seterr(OMC_ERR_ARG, "replayed random numbers need the history form (one stream per history)");
+    if (o->event_target < 0 || o->electron_budget < 0 || o->group < 0 || o->hbm_budget < 0 || o->grid < 0 || o->block < 0)
+        return omc_seterr(OMC_ERR_ARG, "negative launch option");
     return OMC_OK;
 }
-
-template <class R> static void fill_spectrum(DevArena &A, const omc_spectrum_desc &sp, int &spectrum, R &energy,
-                                             R &deltak, const R *&c1, const R *&c2) {
-    spectrum = sp.spectrum;
-    energy = (R)sp.energy;
-    deltak = (R)OMC_INVDIM;
-    c1 = c2 = nullptr;
-    if (sp.spectrum) {
-        std::vector<R> a(OMC_INVDIM), b(OMC_INVDIM);
-        for (int k = 0; k < OMC_INVDIM; k++) { a[k] = (R)sp.cdfinv1[k]; b[k] = (R)sp.cdfinv2[k]; }
-        c1 = A.upload(a);
-        c2 = A.upload(b);
-    }
-}
-
-template <class R> static int make_dosxyz_src(DevArena &A, const PhantomDev *pd, const omc_dosxyz_source_desc *src,
-                                              SrcDosxyz<R> &S) {
-    omc_dosxyz_source hs;
-    memset(&hs, 0, sizeof hs);
-    int rc = omc_dosxyz_source_init(&hs, &pd->host, src->collimator, src->ssd, src->charge);
-    if (rc) return rc;
-    if (src->spec.spectrum && (!src->spec.cdfinv1 || !src->spec.cdfinv2))
-        return omc_seterr(OMC_ERR_ARG, "spectrum source without inverse CDF");
-    fill_spectrum<R>(A, src->spec, S.spectrum, S.energy, S.deltak, S.cdfinv1, S.cdfinv2);
-    S.charge = hs.charge;
-    S.ssd = (R)hs.ssd;
-    S.xinl = (R)hs.xinl;
-    S.yinl = (R)hs.yinl;
-    S.xsize = (R)hs.xsize;
-    S.ysize = (R)hs.ysize;
-    S.ixinl = hs.ixinl;
-    S.iyinl = hs.iyinl;
-    return OMC_OK;
-}
-
-#define HIPCHK(x)                                                                                         \
-    do {                                                                                                  \
-        hipError_t e_ = (x);                                                                              \
-        if (e_ != hipSuccess) { rc = omc_seterr(OMC_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); goto done; } \
-    } while (0)
 
 template <class R>
-static int run_dosxyz_history(omc_ctx *c, PhantomDev *pd, const omc_dosxyz_source_desc *src, int64_t nper, int nbatch,
-                              const omc_run_opts *o, double *sum, double *sum2, omc_run_stats *st) {
+static int run_dosxyz_history(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_dosxyz_source_desc *src,
+                              int64_t nper, int nbatch, const omc_run_opts *o, double *sum, double *sum2,
+                              omc_run_stats *st) {
     int rc = OMC_OK;
     DevArena A;
     HistArgs<R> a;
@@ -398,11 +377,11 @@ static int run_dosxyz_history(omc_ctx *c, PhantomDev *pd, const omc_dosxyz_sourc
         HIPCHK(hipMemcpy(d_rn, o->replay, sizeof(uint32_t) * o->replay_len, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d_off, o->replay_offsets, sizeof(long) * (nh + 1), hipMemcpyHostToDevice));
     }
-    HIPCHK(hipMemsetAsync(d_e, 0, sizeof(double) * nreg, c->stream));
-    HIPCHK(hipMemsetAsync(d_s, 0, sizeof(double) * nreg, c->stream));
-    HIPCHK(hipMemsetAsync(d_s2, 0, sizeof(double) * nreg, c->stream));
-    HIPCHK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long) * 8, c->stream));
-    HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int) * 4, c->stream));
+    HIPCHK(hipMemsetAsync(d_e, 0, sizeof(double) * nreg, stream));
+    HIPCHK(hipMemsetAsync(d_s, 0, sizeof(double) * nreg, stream));
+    HIPCHK(hipMemsetAsync(d_s2, 0, sizeof(double) * nreg, stream));
+    HIPCHK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long) * 8, stream));
+    HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int) * 4, stream));
     a.endep = d_e;
     a.counters = d_cnt;
     a.err = d_err;
@@ -412,21 +391,21 @@ static int run_dosxyz_history(omc_ctx *c, PhantomDev *pd, const omc_dosxyz_sourc
     a.chunk = 1;
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
-    HIPCHK(hipEventRecord(ev0, c->stream));
+    HIPCHK(hipEventRecord(ev0, stream));
     for (int b = 0; b < nbatch; b++) {
         a.nhist = nper;
         a.hist_base = (long)b * nper;
         a.replay_off = d_off ? d_off + (long)b * nper : nullptr;
-        HIPCHK(hipMemsetAsync(d_next, 0, sizeof(unsigned long long), c->stream));
+        HIPCHK(hipMemsetAsync(d_next, 0, sizeof(unsigned long long), stream));
         int block = o->block > 0 ? o->block : 256;
         long want = (nper + block - 1) / block;
         int grid = o->grid > 0 ? o->grid : (int)(want < 2048 ? want : 2048);
         HIPCHK(launch_history<R>(a, o->rng == OMC_RNG_REPLAY ? RNG_REPLAY : RNG_PHILOX, o->count_events != 0, grid,
-                                 block, c->stream));
-        HIPCHK(launch_accumulate(d_e, d_s, d_s2, nreg, 1.0, c->stream));
+                                 block, stream));
+        HIPCHK(launch_accumulate(d_e, d_s, d_s2, nreg, 1.0, stream));
     }
-    HIPCHK(hipEventRecord(ev1, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipEventRecord(ev1, stream));
+    HIPCHK(hipStreamSynchronize(stream));
     HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
     HIPCHK(hipMemcpy(sum, d_s, sizeof(double) * nreg, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(sum2, d_s2, sizeof(double) * nreg, hipMemcpyDeviceToHost));
@@ -440,6 +419,8 @@ static int run_dosxyz_history(omc_ctx *c, PhantomDev *pd, const omc_dosxyz_sourc
         st->n_howfar = cnt[0]; st->n_hownear = cnt[1]; st->n_cross = cnt[2]; st->n_dep = cnt[3];
         st->n_rn = cnt[4]; st->n_photon = cnt[5]; st->n_electron = cnt[6];
         st->errors = (err[0] ? 1 : 0) | (err[1] ? 2 : 0) | (err[2] ? 4 : 0);
+        st->kernel_seconds[OMC_K_HISTORY] = ms * 1e-3;
+        st->kernel_launches[OMC_K_HISTORY] = (uint64_t)nbatch;
         if (err[0]) rc = omc_seterr(OMC_ERR_STACK, "particle stack overflow on the device");
     }
 done:
@@ -450,20 +431,28 @@ done:
     return rc;
 }
 
-extern "C" int omc_run_dosxyz(omc_ctx *c, int phantom_id, const omc_dosxyz_source_desc *src, int64_t nperbatch,
-                              int nbatch, const omc_run_opts *o, double *sum, double *sum2, omc_run_stats *st) {
-    if (!c || !src || !sum || !sum2 || nperbatch < 1 || nbatch < 1) return omc_seterr(OMC_ERR_ARG, "omc_run_dosxyz: bad arguments");
+int omc_dosxyz_run(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_dosxyz_source_desc *src,
+                   int64_t nperbatch, int nbatch, const omc_run_opts *o, double *sum, double *sum2, omc_run_stats *st) {
+    if (!c || !pd || !src || !sum || !sum2 || nperbatch < 1 || nbatch < 1)
+        return omc_seterr(OMC_ERR_ARG, "omc_run_dosxyz: bad arguments");
     int rc = omc_check_opts(o);
     if (rc) return rc;
-    PhantomDev *pd = omc_get_phantom(c, phantom_id);
-    if (!pd) return omc_seterr(OMC_ERR_ARG, "no phantom %d", phantom_id);
     (void)hipSetDevice(c->device);
     if (st) memset(st, 0, sizeof *st);
     if (o->form == OMC_FORM_EVENT)
-        return o->precision == OMC_FP32 ? omc_run_dosxyz_event<float>(c, pd, src, nperbatch, nbatch, o, sum, sum2, st)
-                                        : omc_run_dosxyz_event<double>(c, pd, src, nperbatch, nbatch, o, sum, sum2, st);
-    return o->precision == OMC_FP32 ? run_dosxyz_history<float>(c, pd, src, nperbatch, nbatch, o, sum, sum2, st)
-                                    : run_dosxyz_history<double>(c, pd, src, nperbatch, nbatch, o, sum, sum2, st);
+        return o->precision == OMC_FP32
+                   ? omc_run_dosxyz_event<float>(c, stream, pd, src, nperbatch, nbatch, o, sum, sum2, st)
+                   : omc_run_dosxyz_event<double>(c, stream, pd, src, nperbatch, nbatch, o, sum, sum2, st);
+    return o->precision == OMC_FP32 ? run_dosxyz_history<float>(c, stream, pd, src, nperbatch, nbatch, o, sum, sum2, st)
+                                    : run_dosxyz_history<double>(c, stream, pd, src, nperbatch, nbatch, o, sum, sum2, st);
+}
+
+extern "C" int omc_run_dosxyz(omc_ctx *c, int phantom_id, const omc_dosxyz_source_desc *src, int64_t nperbatch,
+                              int nbatch, const omc_run_opts *o, double *sum, double *sum2, omc_run_stats *st) {
+    if (!c) return omc_seterr(OMC_ERR_ARG, "omc_run_dosxyz: null context");
+    PhantomDev *pd = omc_get_phantom(c, phantom_id);
+    if (!pd) return omc_seterr(OMC_ERR_ARG, "no phantom %d", phantom_id);
+    return omc_dosxyz_run(c, c->stream, pd, src, nperbatch, nbatch, o, sum, sum2, st);
 }
 
 extern "C" int omc_dosxyz_dose(omc_ctx *c, int phantom_id, const double *sum, const double *sum2, int64_t nperbatch,

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
+#include <cstdlib>
+#include <cstring>
 #include <map>
 #include <vector>
 
@@ -36,6 +38,8 @@ struct PhantomDev {
     long nreg = 0;               // voxels + 1
     DevArena mem;
     const uint8_t *dmed = nullptr;
+    const double *dmass = nullptr;   // voxel mass (g): the reference's dx*dy*dz*rho, for the Dij finish
+    const double *drho = nullptr;
     omc::Geom<float> g32;
     omc::Geom<double> g64;
 };
@@ -68,5 +72,53 @@ size_t omc_phantom_bytes(const omc_phantom_desc *d);
 
 // event-queue form (omc_event.hip)
 template <class R>
-int omc_run_dosxyz_event(omc_ctx *c, PhantomDev *pd, const omc_dosxyz_source_desc *src, int64_t nper, int nbatch,
-                         const omc_run_opts *o, double *sum, double *sum2, omc_run_stats *st);
+int omc_run_dosxyz_event(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_dosxyz_source_desc *src,
+                         int64_t nper, int nbatch, const omc_run_opts *o, double *sum, double *sum2, omc_run_stats *st);
+// Dij run on a given stream (omc_event.hip)
+int omc_dij_run(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_desc *spec,
+                const omc_beamlets_desc *bix, int nbatch, double rel, const omc_run_opts *o, omc_csc *dij,
+                omc_csc *var, double *plan_dose, double *plan_var, omc_run_stats *st);
+// dose run on a given stream (omc_capi.hip); the server runs requests through it
+int omc_dosxyz_run(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_dosxyz_source_desc *src,
+                   int64_t nperbatch, int nbatch, const omc_run_opts *o, double *sum, double *sum2, omc_run_stats *st);
+
+template <class R> inline void fill_spectrum(DevArena &A, const omc_spectrum_desc &sp, int &spectrum, R &energy,
+                                             R &deltak, const R *&c1, const R *&c2) {
+    spectrum = sp.spectrum;
+    energy = (R)sp.energy;
+    deltak = (R)OMC_INVDIM;
+    c1 = c2 = nullptr;
+    if (sp.spectrum) {
+        std::vector<R> a(OMC_INVDIM), b(OMC_INVDIM);
+        for (int k = 0; k < OMC_INVDIM; k++) { a[k] = (R)sp.cdfinv1[k]; b[k] = (R)sp.cdfinv2[k]; }
+        c1 = A.upload(a);
+        c2 = A.upload(b);
+    }
+}
+
+template <class R> inline int make_dosxyz_src(DevArena &A, const PhantomDev *pd, const omc_dosxyz_source_desc *src,
+                                              omc::SrcDosxyz<R> &S) {
+    omc_dosxyz_source hs;
+    memset(&hs, 0, sizeof hs);
+    int rc = omc_dosxyz_source_init(&hs, &pd->host, src->collimator, src->ssd, src->charge);
+    if (rc) return rc;
+    if (src->spec.spectrum && (!src->spec.cdfinv1 || !src->spec.cdfinv2))
+        return omc_seterr(OMC_ERR_ARG, "spectrum source without inverse CDF");
+    fill_spectrum<R>(A, src->spec, S.spectrum, S.energy, S.deltak, S.cdfinv1, S.cdfinv2);
+    S.charge = hs.charge;
+    S.ssd = (R)hs.ssd;
+    S.xinl = (R)hs.xinl;
+    S.yinl = (R)hs.yinl;
+    S.xsize = (R)hs.xsize;
+    S.ysize = (R)hs.ysize;
+    S.ixinl = hs.ixinl;
+    S.iyinl = hs.iyinl;
+    return OMC_OK;
+}
+
+#define HIPCHK(x)                                                                                         \
+    do {                                                                                                  \
+        hipError_t e_ = (x);                                                                              \
+        if (e_ != hipSuccess) { rc = omc_seterr(OMC_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); goto done; } \
+    } while (0)
+

@@ -1212,26 +1212,53 @@ template <class R, class G, class T> struct Engine {
         return -1;
     }
 
+    // Resumable state of an electron inside the reference's ustep loop: the
+    // remaining number of mean free paths and the cross section of the
+    // fictitious method.  phase 0: enter electron() afresh; 1: resume;
+    // 2: resume with the distance to the interaction still to be computed
+    // (after a step taken with a zero cross section).
+    struct EState { R demfp, sig0; int phase; };
+
     // condensed-history electron step (src/ompmc.c:4666-5451).  The variable
     // scopes of the reference are kept: the accumulated path to the next
     // interaction (total_tstep) restarts at zero in every ustep iteration, so
     // a step that does not reach the interaction site is followed by a
     // zero-length step that resamples the distance.
     __device__ int electron(Part<R> &p, Part<R> &q) {
+        EState es = {(R)0, (R)0, 0};
+        return electron_run(p, q, es, 0x7fffffff);
+    }
+
+    // The same routine, yielding after `budget` ustep iterations: returns 0
+    // with es.phase != 0 and the loop state saved; called again with that state
+    // it continues exactly where the uninterrupted routine would.  Every other
+    // exit leaves es.phase = 0.
+    __device__ int electron_run(Part<R> &p, Part<R> &q, EState &es, int budget) {
         const R RM = (R)0.5109989461;
         if (count) cnt.nelectron++;
         int imed = medium(p);
         R rhof = rhof_of(p);
         R eie = p.e;
         int iq = p.iq, qel = (1 + iq) / 2, medold = imed;
-        if (eie <= ecut_of(imed)) return electron_stop(p, q, iq, eie, false);
+        bool resume = es.phase != 0;
+        bool resume_compute = es.phase == 2;
+        es.phase = 0;
+        if (!resume && eie <= ecut_of(imed)) return electron_stop(p, q, iq, eie, false);
         R elke = (R)0;
         int lelke = 0;
+        if (resume) {
+            elke = M<R>::log(eie - RM);
+            lelke = pwlf_i(elke, t.med[imed].eke1, t.med[imed].eke0) - 1;
+        }
         R sigratio = (R)0, rfict = (R)0;
         do {
-            bool compute_tstep = true;
+            bool compute_tstep = resume ? resume_compute : true;
             R eke = eie - RM, demfp = (R)0, sig0 = (R)0, ustep = (R)0;
-            if (imed != -1) {
+            if (resume) {
+                demfp = es.demfp;
+                sig0 = es.sig0;
+                resume = false;
+            } else if (imed != -1) {
                 const Med<R> &m = t.med[imed];
                 R rn = rnd();
                 if (rn == (R)0.0) rn = (R)1.0E-30;
@@ -1418,6 +1445,12 @@ template <class R, class G, class T> struct Engine {
                 demfp -= de * sig0;
                 total_tstep -= tvstep * rhof;
                 if (total_tstep < (R)1.0E-9) demfp = (R)0.0;
+                if (demfp >= (R)1.0E-5 && --budget <= 0) {
+                    es.demfp = demfp;
+                    es.sig0 = sig0;
+                    es.phase = compute_tstep ? 2 : 1;
+                    return 0;
+                }
             } while (demfp >= (R)1.0E-5);
             if ((imed != medold) || (ustep == (R)0.0) || (imed == -1)) continue;
             int ix = imed * MXEKE + lelke;

@@ -1,32 +1,936 @@
-// Event-queue (batched) transport form.
+// Host side of the event-queue form, the dose-influence (Dij) runs and the
+// request server.
 #include "omc_capi_internal.hpp"
 
-template <class R>
-int omc_run_dosxyz_event(omc_ctx *c, PhantomDev *pd, const omc_dosxyz_source_desc *src, int64_t nper, int nbatch,
-                         const omc_run_opts *o, double *sum, double *sum2, omc_run_stats *st) {
-    (void)c; (void)pd; (void)src; (void)nper; (void)nbatch; (void)o; (void)sum; (void)sum2; (void)st;
-    return omc_seterr(OMC_ERR_ARG, "event form not available in this build");
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+
+using namespace omc;
+
+namespace {
+
+#define CK(x)                                                                                    \
+    do {                                                                                         \
+        hipError_t e_ = (x);                                                                     \
+        if (e_ != hipSuccess) return omc_seterr(OMC_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+struct ArenaGuard {
+    DevArena &a;
+    ~ArenaGuard() { a.release(); }
+};
+
+struct Events {
+    std::vector<hipEvent_t> ev;
+    explicit Events(int n) : ev(n, nullptr) {
+        for (auto &e : ev) (void)hipEventCreate(&e);
+    }
+    ~Events() {
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    float ms(int a, int b) const {
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, ev[a], ev[b]);
+        return t;
+    }
+};
+
+struct Pinned {
+    void *p = nullptr;
+    explicit Pinned(size_t n) {
+        if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) p = nullptr;
+    }
+    ~Pinned() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+int device_cus(int dev) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    return n;
 }
 
-template int omc_run_dosxyz_event<float>(omc_ctx *, PhantomDev *, const omc_dosxyz_source_desc *, int64_t, int,
-                                         const omc_run_opts *, double *, double *, omc_run_stats *);
-template int omc_run_dosxyz_event<double>(omc_ctx *, PhantomDev *, const omc_dosxyz_source_desc *, int64_t, int,
-                                          const omc_run_opts *, double *, double *, omc_run_stats *);
+// ------------------------------------------------------------------ event-form state
+constexpr int DEFAULT_BUDGET = 16;           // condensed-history steps per charged event
+constexpr long MAX_ROUNDS = 2000000;         // a job that has not drained by then is reported as a step-guard error
+
+template <class R> size_t pool_entry_bytes() { return sizeof(R) * 10 + sizeof(int) * 6 + 4 + 8; }
+
+template <class R> size_t event_bytes(long long target) {
+    return (size_t)4 * (size_t)(4 * target) * pool_entry_bytes<R>() + sizeof(EvCtl);
+}
+
+long long default_target(int dev) { return (long long)device_cus(dev) * 4096; }
+
+template <class R> struct EvState {
+    DevArena A;
+    Pool<R> pool[2][2];
+    EvCtl *ctl = nullptr;
+    long long target = 0, cap = 0;
+    ~EvState() { A.release(); }
+    int alloc(long long tgt) {
+        target = tgt < 64 ? 64 : tgt;
+        cap = 4 * target;
+        for (int p = 0; p < 2; p++)
+            for (int k = 0; k < 2; k++) {
+                Pool<R> &P = pool[p][k];
+                R **fr[10] = {&P.e, &P.x, &P.y, &P.z, &P.u, &P.v, &P.w, &P.wt, &P.demfp, &P.sig0};
+                for (R **f : fr) *f = A.alloc_n<R>(cap);
+                int **fi[6] = {&P.ix, &P.iy, &P.iz, &P.iq, &P.phase, &P.slot};
+                for (int **f : fi) *f = A.alloc_n<int>(cap);
+                P.ctr = A.alloc_n<uint32_t>(cap);
+                P.stream = A.alloc_n<uint64_t>(cap);
+            }
+        ctl = A.alloc_n<EvCtl>(1);
+        if (!A.ok) return omc_seterr(OMC_ERR_NOMEM, "event pools for %lld particles do not fit", cap);
+        return OMC_OK;
+    }
+};
+
+struct JobStats {
+    double t_source = 0, t_photon = 0, t_charged = 0, t_other = 0;
+    uint64_t n_source = 0, n_photon = 0, n_charged = 0, n_other = 0;
+    uint64_t ev_photon = 0, ev_charged = 0, rounds = 0, peak = 0;
+};
+
+// Runs one job (nhist primaries) to completion.  Rounds are enqueued in
+// chunks; the host reads a snapshot of the control block after each chunk
+// while the next chunk runs.  With `timing`, every round is bracketed by
+// events per kernel class instead (slower; for profiling runs).
+template <class R>
+int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int dest, bool count, bool timing,
+            JobStats &js) {
+    CK(launch_job_start(S.ctl, nhist, S.target, S.cap, dest, st));
+    Pinned snapbuf(2 * sizeof(EvCtl));
+    if (!snapbuf.p) return omc_seterr(OMC_ERR_NOMEM, "pinned host buffer");
+    EvCtl *snap = (EvCtl *)snapbuf.p;
+    const EvCtl *done = nullptr;
+    if (timing) {
+        Events ev(5);
+        for (long r = 0;; r++) {
+            CK(hipEventRecord(ev.ev[0], st));
+            for (int part = 0; part < 4; part++) {
+                CK(launch_round_part(L, r, part, count, st));
+                CK(hipEventRecord(ev.ev[part + 1], st));
+            }
+            CK(hipMemcpyAsync(&snap[0], S.ctl, sizeof(EvCtl), hipMemcpyDeviceToHost, st));
+            CK(hipStreamSynchronize(st));
+            js.t_source += ev.ms(0, 1) * 1e-3;
+            js.t_photon += ev.ms(1, 2) * 1e-3;
+            js.t_charged += ev.ms(2, 3) * 1e-3;
+            js.t_other += ev.ms(3, 4) * 1e-3;
+            js.n_source++; js.n_photon++; js.n_charged++; js.n_other++;
+            if (snap[0].err || snap[0].done_round >= 0 || r > MAX_ROUNDS) { done = &snap[0]; break; }
+        }
+    } else {
+        Events ev(2);
+        const int K = 4;
+        long r = 0;
+        int cur = 0;
+        CK(launch_rounds(L, r, K, count, st));
+        CK(hipMemcpyAsync(&snap[0], S.ctl, sizeof(EvCtl), hipMemcpyDeviceToHost, st));
+        CK(hipEventRecord(ev.ev[0], st));
+        r += K;
+        for (;;) {
+            CK(launch_rounds(L, r, K, count, st));
+            CK(hipMemcpyAsync(&snap[1 - cur], S.ctl, sizeof(EvCtl), hipMemcpyDeviceToHost, st));
+            CK(hipEventRecord(ev.ev[1 - cur], st));
+            r += K;
+            CK(hipEventSynchronize(ev.ev[cur]));
+            if (snap[cur].err || snap[cur].done_round >= 0 || r > MAX_ROUNDS) break;
+            cur = 1 - cur;
+        }
+        CK(hipStreamSynchronize(st));
+        done = &snap[1 - cur];   // the last snapshot, taken after every enqueued round
+    }
+    js.ev_photon += done->events[0];
+    js.ev_charged += done->events[1];
+    js.rounds += (uint64_t)(done->done_round >= 0 ? done->done_round : done->rounds);
+    js.peak = std::max<uint64_t>(js.peak, done->peak);
+    if (done->err & 1) return omc_seterr(OMC_ERR_STACK, "event pool overflow (target %lld, capacity %lld)", S.target, S.cap);
+    if (done->done_round < 0) return omc_seterr(OMC_ERR_LIMIT, "event job did not drain within %ld rounds", MAX_ROUNDS);
+    return OMC_OK;
+}
+
+template <class R> void fill_launch(omc_ctx *c, const PhantomDev *pd, EvState<R> &S, const omc_run_opts *o,
+                                    EvLaunch<R> &L) {
+    memset(&L, 0, sizeof L);
+    L.t = tables_of<R>(c);
+    L.g = geom_of<R>(pd);
+    L.seed = o->seed;
+    for (int p = 0; p < 2; p++)
+        for (int k = 0; k < 2; k++) L.pool[p][k] = S.pool[p][k];
+    L.ctl = S.ctl;
+    L.cap = S.cap;
+    L.budget = o->electron_budget > 0 ? o->electron_budget : DEFAULT_BUDGET;
+    int cus = device_cus(c->device);
+    L.grid_event = o->grid > 0 ? o->grid : cus * 8;
+    L.grid_source = cus * 4;
+    L.src.t = L.t;
+    L.src.g = L.g;
+    L.src.seed = o->seed;
+}
+
+void stats_from_job(omc_run_stats *st, const JobStats &js) {
+    if (!st) return;
+    st->kernel_seconds[OMC_K_SOURCE] += js.t_source;
+    st->kernel_seconds[OMC_K_PHOTON] += js.t_photon;
+    st->kernel_seconds[OMC_K_CHARGED] += js.t_charged;
+    st->kernel_seconds[OMC_K_OTHER] += js.t_other;
+    st->kernel_launches[OMC_K_SOURCE] += js.n_source;
+    st->kernel_launches[OMC_K_PHOTON] += js.n_photon;
+    st->kernel_launches[OMC_K_CHARGED] += js.n_charged;
+    st->kernel_launches[OMC_K_OTHER] += js.n_other;
+    st->events_photon += js.ev_photon;
+    st->events_charged += js.ev_charged;
+    st->rounds += js.rounds;
+    st->peak_inflight = std::max<uint64_t>(st->peak_inflight, js.peak);
+}
+
+void stats_counters(omc_run_stats *st, const unsigned long long *cnt) {
+    // cnt[0..7] photon kernel (or history kernel), cnt[8..15] charged kernel
+    st->n_howfar = cnt[0] + cnt[8];
+    st->n_hownear = cnt[1] + cnt[9];
+    st->n_cross = cnt[2] + cnt[10];
+    st->n_dep = cnt[3] + cnt[11];
+    st->n_rn = cnt[4] + cnt[12];
+    st->n_photon = cnt[5] + cnt[13];
+    st->n_electron = cnt[6] + cnt[14];
+    for (int k = 0; k < 2; k++) {
+        st->ev_howfar[k] = cnt[8 * k + 0];
+        st->ev_hownear[k] = cnt[8 * k + 1];
+        st->ev_cross[k] = cnt[8 * k + 2];
+        st->ev_dep[k] = cnt[8 * k + 3];
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ dose runs, event form
+template <class R>
+int omc_run_dosxyz_event(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_dosxyz_source_desc *src,
+                         int64_t nper, int nbatch, const omc_run_opts *o, double *sum, double *sum2,
+                         omc_run_stats *st) {
+    auto wall0 = std::chrono::steady_clock::now();
+    DevArena A;
+    ArenaGuard ag{A};
+    EvState<R> S;
+    long long tgt = o->event_target > 0 ? o->event_target : default_target(c->device);
+    if (tgt > 4 * (long long)nper) tgt = std::max<long long>(4 * (long long)nper, 64);
+    int rc = S.alloc(tgt);
+    if (rc) return rc;
+    long nreg = pd->nreg;
+    double *d_e = A.alloc_n<double>(nreg), *d_s = A.alloc_n<double>(nreg), *d_s2 = A.alloc_n<double>(nreg);
+    unsigned long long *d_cnt = A.alloc_n<unsigned long long>(16);
+    if (!A.ok) return omc_seterr(OMC_ERR_NOMEM, "device allocation failed");
+    EvLaunch<R> L;
+    fill_launch(c, pd, S, o, L);
+    rc = make_dosxyz_src<R>(A, pd, src, L.src.dos);
+    if (rc) return rc;
+    L.src.use_beamlets = 0;
+    L.endep = d_e;
+    L.slot_stride = 0;
+    L.counters = o->count_events ? d_cnt : nullptr;
+    CK(hipMemsetAsync(d_e, 0, sizeof(double) * nreg, stream));
+    CK(hipMemsetAsync(d_s, 0, sizeof(double) * nreg, stream));
+    CK(hipMemsetAsync(d_s2, 0, sizeof(double) * nreg, stream));
+    CK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long) * 16, stream));
+    int dest = L.src.dos.charge != 0 ? 1 : 0;
+    JobStats js;
+    Events ev(2);
+    CK(hipEventRecord(ev.ev[0], stream));
+    for (int b = 0; b < nbatch; b++) {
+        L.src.hist_base = (long)b * nper;
+        rc = run_job(stream, S, L, nper, dest, o->count_events != 0, o->count_events != 0, js);
+        if (rc) return rc;
+        CK(launch_accumulate(d_e, d_s, d_s2, nreg, 1.0, stream));
+    }
+    CK(hipEventRecord(ev.ev[1], stream));
+    CK(hipStreamSynchronize(stream));
+    CK(hipMemcpy(sum, d_s, sizeof(double) * nreg, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(sum2, d_s2, sizeof(double) * nreg, hipMemcpyDeviceToHost));
+    if (st) {
+        unsigned long long cnt[16];
+        CK(hipMemcpy(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost));
+        st->gpu_seconds = ev.ms(0, 1) * 1e-3;
+        st->histories = (double)nper * nbatch;
+        stats_counters(st, cnt);
+        stats_from_job(st, js);
+        st->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
+    }
+    return OMC_OK;
+}
+
+template int omc_run_dosxyz_event<float>(omc_ctx *, hipStream_t, PhantomDev *, const omc_dosxyz_source_desc *, int64_t,
+                                         int, const omc_run_opts *, double *, double *, omc_run_stats *);
+template int omc_run_dosxyz_event<double>(omc_ctx *, hipStream_t, PhantomDev *, const omc_dosxyz_source_desc *,
+                                          int64_t, int, const omc_run_opts *, double *, double *, omc_run_stats *);
+
+// ------------------------------------------------------------------ Dij runs
+namespace {
+
+// Histories per batch of each beamlet as the reference derives them
+// (omc_matrad.c:1490-1498): nperbatch = nhist / nbatch, at least 1.
+std::vector<long> beamlet_nper(const omc_beamlets_desc *bix, int nbatch) {
+    std::vector<long> v(bix->nbixels);
+    for (int b = 0; b < bix->nbixels; b++) {
+        long nh = bix->nhist ? (long)bix->nhist[b] : (long)bix->nhist_each;
+        if (nh / nbatch == 0) nh = nbatch;
+        v[b] = nh / nbatch;
+    }
+    return v;
+}
+
+size_t dij_slot_bytes(long nreg) { return (size_t)3 * nreg * sizeof(double); }
+
+size_t dij_fixed_bytes(long nreg, int precision, int form, long long target) {
+    size_t b = (size_t)3 * nreg * sizeof(double) + ((size_t)64 << 20);
+    if (form == OMC_FORM_EVENT)
+        b += precision == OMC_FP32 ? event_bytes<float>(target) : event_bytes<double>(target);
+    return b;
+}
+
+int pick_group(const omc_run_opts *o, long nreg, int nbix, int dev, long long target) {
+    if (o->group > 0) return std::min(o->group, nbix);
+    size_t budget = (size_t)o->hbm_budget;
+    if (!budget) {
+        size_t freeb = 0, totalb = 0;
+        if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) freeb = (size_t)8 << 30;
+        budget = std::min(freeb / 2, (size_t)48 << 30);
+    }
+    size_t fixed = dij_fixed_bytes(nreg, o->precision, o->form, target);
+    if (budget <= fixed + dij_slot_bytes(nreg)) return 1;
+    long g = (long)((budget - fixed) / (dij_slot_bytes(nreg) + 64));
+    (void)dev;
+    return (int)std::max(1L, std::min<long>(g, std::min(nbix, 4096)));
+}
+
+template <class R> struct DijDev {
+    SrcBeamlets<R> src;
+};
+
+template <class R>
+int upload_beamlets(DevArena &A, const omc_spectrum_desc *spec, const omc_beamlets_desc *b, SrcBeamlets<R> &S) {
+    fill_spectrum<R>(A, *spec, S.spectrum, S.energy, S.deltak, S.cdfinv1, S.cdfinv2);
+    std::vector<int> ib(b->ibeam, b->ibeam + b->nbixels);
+    for (int v : ib)
+        if (v < 0 || v >= b->nbeams) return omc_seterr(OMC_ERR_ARG, "beamlet beam index %d of %d beams", v, b->nbeams);
+    auto vec = [&](const double *p, size_t n) {
+        std::vector<R> v(n);
+        for (size_t k = 0; k < n; k++) v[k] = (R)p[k];
+        return A.upload(v);
+    };
+    S.ibeam = A.upload(ib);
+    S.src = vec(b->src, 3 * (size_t)b->nbeams);
+    S.corner = vec(b->corner, 3 * (size_t)b->nbixels);
+    S.side1 = vec(b->side1, 3 * (size_t)b->nbixels);
+    S.side2 = vec(b->side2, 3 * (size_t)b->nbixels);
+    S.gaussian = b->gaussian;
+    S.sigma = (R)b->sigma;
+    if (!A.ok) return omc_seterr(OMC_ERR_NOMEM, "beamlet arrays");
+    return OMC_OK;
+}
+
+struct HostCSC {
+    std::vector<int64_t> jc, ir;
+    std::vector<double> val, var;
+};
+
+template <class R>
+int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_desc *spec, const omc_beamlets_desc *bix,
+            int nbatch, double rel, const omc_run_opts *o, omc_csc *dij, omc_csc *var, double *plan_dose,
+            double *plan_var, omc_run_stats *st) {
+    auto wall0 = std::chrono::steady_clock::now();
+    const int nbix = bix->nbixels;
+    const long nreg = pd->nreg, nvox = nreg - 1;
+    const bool event = o->form == OMC_FORM_EVENT;
+    const bool count = o->count_events != 0;
+    std::vector<long> nper = beamlet_nper(bix, nbatch);
+    for (long n : nper)
+        if (n >= (1L << 32)) return omc_seterr(OMC_ERR_LIMIT, "more than 2^32 histories per batch of a beamlet");
+    long long tgt = o->event_target > 0 ? o->event_target : default_target(c->device);
+    const int G = pick_group(o, nreg, nbix, c->device, tgt);
+    DevArena A;
+    ArenaGuard ag{A};
+    SrcBeamlets<R> bsrc;
+    int rc = upload_beamlets<R>(A, spec, bix, bsrc);
+    if (rc) return rc;
+    const int nblk = (int)((nvox + DIJ_BLOCK - 1) / DIJ_BLOCK);
+    const int nchunk = (int)((nvox + DIJ_CHUNK - 1) / DIJ_CHUNK);
+    double *d_e = A.alloc_n<double>((size_t)G * nreg), *d_s = A.alloc_n<double>((size_t)G * nreg),
+           *d_s2 = A.alloc_n<double>((size_t)G * nreg);
+    double *d_carry = A.alloc_n<double>(nreg), *d_tdose = A.alloc_n<double>(nreg), *d_tvar = A.alloc_n<double>(nreg);
+    double *d_bmax = A.alloc_n<double>((size_t)G * nblk), *d_thresh = A.alloc_n<double>(G);
+    double *d_nper = A.alloc_n<double>(G);
+    int *d_counts = A.alloc_n<int>((size_t)G * nchunk);
+    int64_t *d_off = A.alloc_n<int64_t>((size_t)G * nchunk);
+    long *d_slot_off = A.alloc_n<long>(G + 1);
+    int *d_slot_bix = A.alloc_n<int>(G);
+    unsigned long long *d_cnt = A.alloc_n<unsigned long long>(16), *d_next = A.alloc_n<unsigned long long>(1);
+    int *d_err = A.alloc_n<int>(4);
+    if (!A.ok) return omc_seterr(OMC_ERR_NOMEM, "Dij tallies for a group of %d beamlets do not fit", G);
+    // Replayed draws (verification): histories are numbered beamlet by beamlet,
+    // batch by batch, as the reference's serial loop consumes its stream.
+    const bool replay = o->rng == OMC_RNG_REPLAY;
+    std::vector<long> gfirst(nbix + 1, 0);
+    for (int b = 0; b < nbix; b++) gfirst[b + 1] = gfirst[b] + (long)nbatch * nper[b];
+    uint32_t *d_rn = nullptr;
+    long *d_rstart = nullptr, *d_rend = nullptr;
+    if (replay) {
+        if (o->replay_len < o->replay_offsets[gfirst[nbix]])
+            return omc_seterr(OMC_ERR_ARG, "replay stream shorter than its offsets");
+        d_rn = A.alloc_n<uint32_t>(o->replay_len > 0 ? o->replay_len : 1);
+        long maxjob = 0;
+        for (int g0 = 0; g0 < nbix; g0 += G) {
+            long t = 0;
+            for (int b = g0; b < std::min(nbix, g0 + G); b++) t += nper[b];
+            maxjob = std::max(maxjob, t);
+        }
+        d_rstart = A.alloc_n<long>(maxjob);
+        d_rend = A.alloc_n<long>(maxjob);
+        if (!A.ok) return omc_seterr(OMC_ERR_NOMEM, "replay stream");
+        if (o->replay_len > 0) CK(hipMemcpy(d_rn, o->replay, sizeof(uint32_t) * o->replay_len, hipMemcpyHostToDevice));
+    }
+    EvState<R> S;
+    if (event) {
+        long long maxjob = 0;
+        for (int g0 = 0; g0 < nbix; g0 += G) {
+            long long t = 0;
+            for (int b = g0; b < std::min(nbix, g0 + G); b++) t += nper[b];
+            maxjob = std::max(maxjob, t);
+        }
+        if (tgt > 4 * maxjob) tgt = std::max<long long>(4 * maxjob, 64);
+        rc = S.alloc(tgt);
+        if (rc) return rc;
+    }
+    CK(hipMemsetAsync(d_e, 0, sizeof(double) * G * nreg, stream));
+    CK(hipMemsetAsync(d_carry, 0, sizeof(double) * nreg, stream));
+    CK(hipMemsetAsync(d_tdose, 0, sizeof(double) * nreg, stream));
+    CK(hipMemsetAsync(d_tvar, 0, sizeof(double) * nreg, stream));
+    CK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long) * 16, stream));
+    CK(hipMemsetAsync(d_err, 0, sizeof(int) * 4, stream));
+    EvLaunch<R> L;
+    HistArgs<R> H;
+    memset(&H, 0, sizeof H);
+    if (event) {
+        fill_launch(c, pd, S, o, L);
+        L.src.bix = bsrc;
+        L.src.use_beamlets = 1;
+        L.src.slot_off = d_slot_off;
+        L.src.slot_beamlet = d_slot_bix;
+        L.endep = d_e;
+        L.slot_stride = nreg;
+        L.counters = count ? d_cnt : nullptr;
+    } else {
+        H.t = tables_of<R>(c);
+        H.g = geom_of<R>(pd);
+        H.bix = bsrc;
+        H.use_beamlets = 1;
+        H.slot_off = d_slot_off;
+        H.slot_beamlet = d_slot_bix;
+        H.slot_stride = nreg;
+        H.seed = o->seed;
+        H.endep = d_e;
+        H.counters = count ? d_cnt : nullptr;
+        H.err = d_err;
+        H.next = d_next;
+        H.chunk = 1;
+        H.replay = d_rn;
+        H.replay_off = d_rstart;
+        H.replay_end = d_rend;
+    }
+    HostCSC out;
+    out.jc.assign(nbix + 1, 0);
+    std::vector<int> counts;
+    std::vector<int64_t> off;
+    DevArena O;             // compaction output, grown on demand
+    ArenaGuard og{O};
+    size_t ocap = 0;
+    int64_t *d_ir = nullptr;
+    double *d_val = nullptr, *d_vval = nullptr;
+    JobStats js;
+    double t_hist = 0, t_other = 0;
+    uint64_t n_hist = 0, n_other = 0;
+    double histories = 0;
+    Events ev(4);
+    CK(hipEventRecord(ev.ev[0], stream));
+    for (int g0 = 0; g0 < nbix; g0 += G) {
+        const int ng = std::min(G, nbix - g0);
+        std::vector<long> soff(ng + 1, 0);
+        std::vector<int> sbix(ng);
+        std::vector<double> snper(ng);
+        for (int s = 0; s < ng; s++) {
+            sbix[s] = g0 + s;
+            snper[s] = (double)nper[g0 + s];
+            soff[s + 1] = soff[s] + nper[g0 + s];
+        }
+        const long njob = soff[ng];
+        CK(hipMemcpyAsync(d_slot_off, soff.data(), sizeof(long) * (ng + 1), hipMemcpyHostToDevice, stream));
+        CK(hipMemcpyAsync(d_slot_bix, sbix.data(), sizeof(int) * ng, hipMemcpyHostToDevice, stream));
+        CK(hipMemcpyAsync(d_nper, snper.data(), sizeof(double) * ng, hipMemcpyHostToDevice, stream));
+        CK(hipMemsetAsync(d_s, 0, sizeof(double) * ng * nreg, stream));
+        CK(hipMemsetAsync(d_s2, 0, sizeof(double) * ng * nreg, stream));
+        for (int bt = 0; bt < nbatch; bt++) {
+            if (event) {
+                L.src.nslot = ng;
+                L.src.batch = bt;
+                rc = run_job(stream, S, L, njob, 0, count, count, js);
+                if (rc) return rc;
+            } else {
+                H.nslot = ng;
+                H.batch = bt;
+                H.nhist = njob;
+                H.hist_base = 0;
+                if (replay) {
+                    std::vector<long> rs(njob), re(njob);
+                    for (int sl = 0; sl < ng; sl++)
+                        for (long i = 0; i < nper[g0 + sl]; i++) {
+                            long gh = gfirst[g0 + sl] + (long)bt * nper[g0 + sl] + i;
+                            rs[soff[sl] + i] = (long)o->replay_offsets[gh];
+                            re[soff[sl] + i] = (long)o->replay_offsets[gh + 1];
+                        }
+                    CK(hipMemcpyAsync(d_rstart, rs.data(), sizeof(long) * njob, hipMemcpyHostToDevice, stream));
+                    CK(hipMemcpyAsync(d_rend, re.data(), sizeof(long) * njob, hipMemcpyHostToDevice, stream));
+                    CK(hipStreamSynchronize(stream));   // rs/re go out of scope
+                }
+                CK(hipMemsetAsync(d_next, 0, sizeof(unsigned long long), stream));
+                int block = o->block > 0 ? o->block : 256;
+                long want = (njob + block - 1) / block;
+                int grid = o->grid > 0 ? o->grid : (int)std::min<long>(want, 2048);
+                if (count) CK(hipEventRecord(ev.ev[2], stream));
+                CK(launch_history<R>(H, replay ? RNG_REPLAY : RNG_PHILOX, count, grid, block, stream));
+                if (count) {
+                    CK(hipEventRecord(ev.ev[3], stream));
+                    CK(hipEventSynchronize(ev.ev[3]));
+                    t_hist += ev.ms(2, 3) * 1e-3;
+                }
+                n_hist++;
+            }
+            CK(launch_accum_slots(d_e, d_s, d_s2, nreg, nreg, ng, d_nper, stream));
+            histories += (double)njob;
+        }
+        if (count) CK(hipEventRecord(ev.ev[2], stream));
+        int nb = 0;
+        CK(launch_dij_finish(d_s, d_s2, d_carry, ng, nreg, nvox, pd->dmass, pd->drho, nbatch, d_bmax, d_tdose, d_tvar,
+                             &nb, stream));
+        CK(launch_dij_select(d_s, nreg, nvox, ng, d_bmax, nb, rel, d_thresh, d_counts, stream));
+        counts.resize((size_t)ng * nchunk);
+        off.resize((size_t)ng * nchunk);
+        CK(hipMemcpyAsync(counts.data(), d_counts, sizeof(int) * ng * nchunk, hipMemcpyDeviceToHost, stream));
+        CK(hipStreamSynchronize(stream));
+        int64_t tot = 0;
+        for (size_t k = 0; k < counts.size(); k++) {
+            off[k] = tot;
+            tot += counts[k];
+        }
+        for (int s = 0; s < ng; s++) out.jc[g0 + s + 1] = (int64_t)out.ir.size() + off[(size_t)(s + 1) * nchunk - 1] +
+                                                           counts[(size_t)(s + 1) * nchunk - 1];
+        if ((size_t)tot > ocap) {
+            O.release();
+            ocap = std::max<size_t>((size_t)tot, ocap * 2);
+            d_ir = O.alloc_n<int64_t>(ocap);
+            d_val = O.alloc_n<double>(ocap);
+            d_vval = var ? O.alloc_n<double>(ocap) : nullptr;
+            if (!O.ok) return omc_seterr(OMC_ERR_NOMEM, "Dij output buffers (%lld entries)", (long long)tot);
+        }
+        if (tot > 0) {
+            CK(hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * ng * nchunk, hipMemcpyHostToDevice, stream));
+            CK(launch_dij_compact(d_s, d_s2, nreg, nvox, ng, d_thresh, d_off, d_ir, d_val, d_vval, stream));
+            size_t base = out.ir.size();
+            out.ir.resize(base + tot);
+            out.val.resize(base + tot);
+            if (var) out.var.resize(base + tot);
+            CK(hipMemcpyAsync(out.ir.data() + base, d_ir, sizeof(int64_t) * tot, hipMemcpyDeviceToHost, stream));
+            CK(hipMemcpyAsync(out.val.data() + base, d_val, sizeof(double) * tot, hipMemcpyDeviceToHost, stream));
+            if (var)
+                CK(hipMemcpyAsync(out.var.data() + base, d_vval, sizeof(double) * tot, hipMemcpyDeviceToHost, stream));
+        }
+        if (count) {
+            CK(hipEventRecord(ev.ev[3], stream));
+            CK(hipEventSynchronize(ev.ev[3]));
+            t_other += ev.ms(2, 3) * 1e-3;
+            n_other += 4;
+        }
+        CK(hipStreamSynchronize(stream));
+    }
+    CK(hipEventRecord(ev.ev[1], stream));
+    CK(hipStreamSynchronize(stream));
+    int err[4];
+    CK(hipMemcpy(err, d_err, sizeof err, hipMemcpyDeviceToHost));
+    if (err[0]) return omc_seterr(OMC_ERR_STACK, "particle stack overflow on the device");
+    if (err[1]) return omc_seterr(OMC_ERR_ARG, "replay stream underrun");
+    if (err[2]) return omc_seterr(OMC_ERR_LIMIT, "history step guard reached");
+    if (plan_dose) CK(hipMemcpy(plan_dose, d_tdose + 1, sizeof(double) * nvox, hipMemcpyDeviceToHost));
+    if (plan_var) CK(hipMemcpy(plan_var, d_tvar + 1, sizeof(double) * nvox, hipMemcpyDeviceToHost));
+    const int64_t nnz = (int64_t)out.ir.size();
+    auto fill_csc = [&](omc_csc *m, const std::vector<double> &v) -> int {
+        m->nrows = nvox;
+        m->ncols = nbix;
+        m->nnz = nnz;
+        m->jc = (int64_t *)malloc(sizeof(int64_t) * (nbix + 1));
+        m->ir = (int64_t *)malloc(sizeof(int64_t) * (nnz ? nnz : 1));
+        m->val = (double *)malloc(sizeof(double) * (nnz ? nnz : 1));
+        if (!m->jc || !m->ir || !m->val) {
+            omc_csc_free(m);
+            return omc_seterr(OMC_ERR_NOMEM, "host CSC of %lld entries", (long long)nnz);
+        }
+        memcpy(m->jc, out.jc.data(), sizeof(int64_t) * (nbix + 1));
+        if (nnz) {
+            memcpy(m->ir, out.ir.data(), sizeof(int64_t) * nnz);
+            memcpy(m->val, v.data(), sizeof(double) * nnz);
+        }
+        return OMC_OK;
+    };
+    if (dij && (rc = fill_csc(dij, out.val))) return rc;
+    if (var && (rc = fill_csc(var, out.var))) {
+        if (dij) omc_csc_free(dij);
+        return rc;
+    }
+    if (st) {
+        unsigned long long cnt[16];
+        CK(hipMemcpy(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost));
+        st->gpu_seconds = ev.ms(0, 1) * 1e-3;
+        st->histories = histories;
+        stats_counters(st, cnt);
+        if (event) {
+            stats_from_job(st, js);
+        } else {
+            st->kernel_seconds[OMC_K_HISTORY] = t_hist;
+            st->kernel_launches[OMC_K_HISTORY] = n_hist;
+        }
+        st->kernel_seconds[OMC_K_OTHER] += t_other;
+        st->kernel_launches[OMC_K_OTHER] += n_other;
+        st->groups = (nbix + G - 1) / G;
+        st->group_size = G;
+        st->nnz = nnz;
+        st->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
+    }
+    return OMC_OK;
+}
+
+}  // namespace
+
+int omc_dij_run(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_desc *spec,
+                const omc_beamlets_desc *bix, int nbatch, double rel, const omc_run_opts *o, omc_csc *dij,
+                omc_csc *var, double *plan_dose, double *plan_var, omc_run_stats *st) {
+    if (!c || !pd || !spec || !bix || bix->nbixels < 1 || bix->nbeams < 1 || !bix->ibeam || !bix->src ||
+        !bix->corner || !bix->side1 || !bix->side2 || nbatch < 1)
+        return omc_seterr(OMC_ERR_ARG, "omc_run_dij: bad arguments");
+    if (nbatch > 255) return omc_seterr(OMC_ERR_LIMIT, "omc_run_dij: at most 255 batches");
+    if (!bix->nhist && bix->nhist_each < 1) return omc_seterr(OMC_ERR_ARG, "omc_run_dij: no histories per beamlet");
+    if (spec->spectrum && (!spec->cdfinv1 || !spec->cdfinv2))
+        return omc_seterr(OMC_ERR_ARG, "spectrum source without inverse CDF");
+    int rc = omc_check_opts(o);
+    if (rc) return rc;
+    if (o->rng == OMC_RNG_REPLAY && o->form != OMC_FORM_HISTORY)
+        return omc_seterr(OMC_ERR_ARG, "omc_run_dij: replayed random numbers need the history form");
+    (void)hipSetDevice(c->device);
+    if (st) memset(st, 0, sizeof *st);
+    if (dij) memset(dij, 0, sizeof *dij);
+    if (var) memset(var, 0, sizeof *var);
+    return o->precision == OMC_FP32
+               ? run_dij<float>(c, stream, pd, spec, bix, nbatch, rel, o, dij, var, plan_dose, plan_var, st)
+               : run_dij<double>(c, stream, pd, spec, bix, nbatch, rel, o, dij, var, plan_dose, plan_var, st);
+}
+
+extern "C" int omc_run_dij_ex(omc_ctx *c, int phantom_id, const omc_spectrum_desc *spec, const omc_beamlets_desc *bix,
+                              int nbatch, double rel_threshold, const omc_run_opts *opts, omc_csc *dij, omc_csc *var,
+                              double *plan_dose, double *plan_var, omc_run_stats *stats) {
+    PhantomDev *pd = c ? omc_get_phantom(c, phantom_id) : nullptr;
+    if (!pd) return omc_seterr(OMC_ERR_ARG, "no phantom %d", phantom_id);
+    return omc_dij_run(c, c->stream, pd, spec, bix, nbatch, rel_threshold, opts, dij, var, plan_dose, plan_var, stats);
+}
 
 extern "C" int omc_run_dij(omc_ctx *c, int phantom_id, const omc_spectrum_desc *spec, const omc_beamlets_desc *bix,
                            int nbatch, double rel_threshold, const omc_run_opts *opts, omc_csc *dij, omc_csc *var,
                            omc_run_stats *stats) {
-    (void)c; (void)phantom_id; (void)spec; (void)bix; (void)nbatch; (void)rel_threshold; (void)opts; (void)dij; (void)var; (void)stats;
-    return omc_seterr(OMC_ERR_ARG, "Dij not available in this build");
+    return omc_run_dij_ex(c, phantom_id, spec, bix, nbatch, rel_threshold, opts, dij, var, nullptr, nullptr, stats);
 }
+
 extern "C" void omc_csc_free(omc_csc *m) {
     if (!m) return;
-    free(m->jc); free(m->ir); free(m->val);
-    m->jc = m->ir = nullptr; m->val = nullptr; m->nnz = 0;
+    free(m->jc);
+    free(m->ir);
+    free(m->val);
+    m->jc = m->ir = nullptr;
+    m->val = nullptr;
+    m->nnz = 0;
 }
-extern "C" int omc_server_create(omc_ctx *, int64_t, int, omc_server **) { return omc_seterr(OMC_ERR_ARG, "serving not available in this build"); }
-extern "C" int omc_server_submit(omc_server *, const omc_request *, int64_t *) { return OMC_ERR_ARG; }
-extern "C" int omc_server_poll(omc_server *, int64_t, int *) { return OMC_ERR_ARG; }
-extern "C" int omc_server_wait(omc_server *, int64_t, omc_result *) { return OMC_ERR_ARG; }
-extern "C" void omc_result_free(omc_result *) {}
-extern "C" void omc_server_destroy(omc_server *) {}
+
+// ------------------------------------------------------------------ request server
+// Continuous batching over HIP streams: a dispatcher admits queued requests
+// in arrival order while their device memory fits the budget and a worker
+// slot is free; each admitted request runs on its own stream (default
+// priority), so kernels of concurrent requests overlap on the device and a
+// request that arrives while others run starts as soon as memory and a slot
+// free up.  Results do not depend on the mix: every history's random stream
+// is keyed by (seed, beamlet, batch, history).
+struct omc_server {
+    omc_ctx *ctx = nullptr;
+    int64_t budget = 0;
+    int max_inflight = 1;
+    std::mutex mu;
+    std::condition_variable cv;
+    struct Req {
+        int64_t id;
+        omc_request r;
+        // owned copies of the caller's arrays
+        std::vector<double> xb, yb, zb, rho, c1, c2, bsrc, corner, side1, side2;
+        std::vector<int> med, ibeam;
+        std::vector<int64_t> nhist;
+        size_t bytes = 0;
+        omc_result res;
+    };
+    std::deque<Req *> queue;
+    std::map<int64_t, Req *> all;
+    int64_t next_id = 1;
+    int64_t in_use = 0;
+    int running = 0;
+    bool stop = false;
+    std::vector<std::thread> workers;
+    std::thread dispatcher;
+    std::deque<Req *> ready;            // admitted, waiting for a worker
+    std::chrono::steady_clock::time_point t0;
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+size_t request_bytes(omc_server *s, const omc_request &r) {
+    const omc_phantom_desc &p = r.phantom;
+    long nreg = (long)p.nx * p.ny * p.nz + 1;
+    size_t b = omc_phantom_bytes(&p);
+    long long tgt = default_target(s->ctx->device);
+    size_t ev = r.precision == OMC_FP64 ? event_bytes<double>(tgt) : event_bytes<float>(tgt);
+    if (r.kind == OMC_REQ_DOSE) return b + (size_t)3 * nreg * 8 + ev;
+    // Dij: the group is sized to what is left of the request's share
+    return b + dij_fixed_bytes(nreg, r.precision, OMC_FORM_EVENT, tgt) + (size_t)std::min(r.bix.nbixels, 8) *
+                                                                             dij_slot_bytes(nreg);
+}
+
+void run_request(omc_server *s, omc_server::Req *q) {
+    omc_ctx *c = s->ctx;
+    (void)hipSetDevice(c->device);
+    hipStream_t st = nullptr;
+    omc_result &res = q->res;
+    int rc = OMC_OK;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        rc = omc_seterr(OMC_ERR_DEVICE, "stream creation failed");
+    }
+    PhantomDev *pd = nullptr;
+    omc_phantom_desc pdsc = q->r.phantom;
+    if (!rc) rc = omc_phantom_make(c, &pdsc, &pd);
+    omc_run_opts o;
+    omc_run_opts_default(&o);
+    o.precision = q->r.precision == OMC_FP64 ? OMC_FP64 : OMC_FP32;
+    o.form = OMC_FORM_EVENT;
+    o.seed = q->r.seed;
+    omc_run_stats stats;
+    memset(&stats, 0, sizeof stats);
+    if (!rc && q->r.kind == OMC_REQ_DOSE) {
+        res.nreg = pd->nreg;
+        res.sum = (double *)calloc(pd->nreg, sizeof(double));
+        res.sum2 = (double *)calloc(pd->nreg, sizeof(double));
+        if (!res.sum || !res.sum2) rc = omc_seterr(OMC_ERR_NOMEM, "result arrays");
+        if (!rc)
+            rc = omc_dosxyz_run(c, st, pd, &q->r.dose_src, q->r.nperbatch, q->r.nbatch, &o, res.sum, res.sum2, &stats);
+    } else if (!rc) {
+        size_t phb = omc_phantom_bytes(&pdsc);
+        o.hbm_budget = (int64_t)(q->bytes > phb ? q->bytes - phb : 0);
+        rc = omc_dij_run(c, st, pd, &q->r.spec, &q->r.bix, q->r.nbatch, q->r.rel_threshold, &o, &res.dij, nullptr,
+                         nullptr, nullptr, &stats);
+    }
+    if (pd) omc_phantom_destroy(pd);
+    if (st) (void)hipStreamDestroy(st);
+    std::lock_guard<std::mutex> lk(s->mu);
+    res.error = rc;
+    res.state = rc ? OMC_REQ_FAILED : OMC_REQ_DONE;
+    res.histories = stats.histories;
+    res.end_t = s->now();
+    s->in_use -= (int64_t)q->bytes;
+    s->running--;
+    s->cv.notify_all();
+}
+
+void worker_loop(omc_server *s) {
+    for (;;) {
+        omc_server::Req *q = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [&] { return s->stop || !s->ready.empty(); });
+            if (s->ready.empty()) return;
+            q = s->ready.front();
+            s->ready.pop_front();
+            q->res.state = OMC_REQ_RUNNING;
+            q->res.start_t = s->now();
+        }
+        run_request(s, q);
+    }
+}
+
+void dispatch_loop(omc_server *s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv.wait(lk, [&] {
+            if (s->stop) return true;
+            if (s->queue.empty() || s->running >= s->max_inflight) return false;
+            omc_server::Req *q = s->queue.front();
+            return s->in_use == 0 || s->in_use + (int64_t)q->bytes <= s->budget;
+        });
+        if (s->stop) return;
+        omc_server::Req *q = s->queue.front();
+        s->queue.pop_front();
+        s->in_use += (int64_t)q->bytes;
+        s->running++;
+        s->ready.push_back(q);
+        s->cv.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" int omc_server_create(omc_ctx *ctx, int64_t hbm_budget_bytes, int max_inflight, omc_server **srv) {
+    if (!ctx || !srv || max_inflight < 1 || max_inflight > 16) return omc_seterr(OMC_ERR_ARG, "omc_server_create: bad arguments");
+    omc_server *s = new omc_server();
+    s->ctx = ctx;
+    if (hbm_budget_bytes <= 0) {
+        size_t freeb = 0, totalb = 0;
+        (void)hipSetDevice(ctx->device);
+        if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) freeb = (size_t)8 << 30;
+        hbm_budget_bytes = (int64_t)(freeb * 3 / 4);
+    }
+    s->budget = hbm_budget_bytes;
+    s->max_inflight = max_inflight;
+    s->t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < max_inflight; k++) s->workers.emplace_back(worker_loop, s);
+    s->dispatcher = std::thread(dispatch_loop, s);
+    *srv = s;
+    return OMC_OK;
+}
+
+extern "C" int omc_server_submit(omc_server *s, const omc_request *r, int64_t *req_id) {
+    if (!s || !r || !req_id) return omc_seterr(OMC_ERR_ARG, "omc_server_submit: bad arguments");
+    if (r->kind != OMC_REQ_DOSE && r->kind != OMC_REQ_DIJ) return omc_seterr(OMC_ERR_ARG, "request kind %d", r->kind);
+    const omc_phantom_desc &p = r->phantom;
+    if (p.nx < 1 || p.ny < 1 || p.nz < 1 || !p.xb || !p.yb || !p.zb || !p.med || !p.rho || r->nbatch < 1)
+        return omc_seterr(OMC_ERR_ARG, "omc_server_submit: bad phantom or batches");
+    auto *q = new omc_server::Req();
+    q->r = *r;
+    long nvox = (long)p.nx * p.ny * p.nz;
+    q->xb.assign(p.xb, p.xb + p.nx + 1);
+    q->yb.assign(p.yb, p.yb + p.ny + 1);
+    q->zb.assign(p.zb, p.zb + p.nz + 1);
+    q->med.assign(p.med, p.med + nvox);
+    q->rho.assign(p.rho, p.rho + nvox);
+    q->r.phantom.xb = q->xb.data();
+    q->r.phantom.yb = q->yb.data();
+    q->r.phantom.zb = q->zb.data();
+    q->r.phantom.med = q->med.data();
+    q->r.phantom.rho = q->rho.data();
+    const omc_spectrum_desc &sp = r->kind == OMC_REQ_DOSE ? r->dose_src.spec : r->spec;
+    if (sp.spectrum) {
+        if (!sp.cdfinv1 || !sp.cdfinv2) { delete q; return omc_seterr(OMC_ERR_ARG, "spectrum without inverse CDF"); }
+        q->c1.assign(sp.cdfinv1, sp.cdfinv1 + OMC_INVDIM);
+        q->c2.assign(sp.cdfinv2, sp.cdfinv2 + OMC_INVDIM);
+    }
+    omc_spectrum_desc &dsp = r->kind == OMC_REQ_DOSE ? q->r.dose_src.spec : q->r.spec;
+    dsp.cdfinv1 = sp.spectrum ? q->c1.data() : nullptr;
+    dsp.cdfinv2 = sp.spectrum ? q->c2.data() : nullptr;
+    if (r->kind == OMC_REQ_DIJ) {
+        const omc_beamlets_desc &b = r->bix;
+        if (b.nbixels < 1 || b.nbeams < 1 || !b.ibeam || !b.src || !b.corner || !b.side1 || !b.side2) {
+            delete q;
+            return omc_seterr(OMC_ERR_ARG, "omc_server_submit: bad beamlets");
+        }
+        q->ibeam.assign(b.ibeam, b.ibeam + b.nbixels);
+        q->bsrc.assign(b.src, b.src + 3 * b.nbeams);
+        q->corner.assign(b.corner, b.corner + 3 * b.nbixels);
+        q->side1.assign(b.side1, b.side1 + 3 * b.nbixels);
+        q->side2.assign(b.side2, b.side2 + 3 * b.nbixels);
+        if (b.nhist) q->nhist.assign(b.nhist, b.nhist + b.nbixels);
+        omc_beamlets_desc &d = q->r.bix;
+        d.ibeam = q->ibeam.data();
+        d.src = q->bsrc.data();
+        d.corner = q->corner.data();
+        d.side1 = q->side1.data();
+        d.side2 = q->side2.data();
+        d.nhist = b.nhist ? q->nhist.data() : nullptr;
+    } else if (r->nperbatch < 1) {
+        delete q;
+        return omc_seterr(OMC_ERR_ARG, "omc_server_submit: no histories");
+    }
+    memset(&q->res, 0, sizeof q->res);
+    q->bytes = request_bytes(s, q->r);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if ((int64_t)q->bytes > s->budget) {
+        delete q;
+        return omc_seterr(OMC_ERR_BUDGET, "request needs %zu bytes, the server budget is %lld", request_bytes(s, *r),
+                          (long long)s->budget);
+    }
+    q->id = s->next_id++;
+    q->res.state = OMC_REQ_QUEUED;
+    q->res.submit_t = s->now();
+    s->all[q->id] = q;
+    s->queue.push_back(q);
+    *req_id = q->id;
+    s->cv.notify_all();
+    return OMC_OK;
+}
+
+extern "C" int omc_server_poll(omc_server *s, int64_t id, int *state) {
+    if (!s || !state) return omc_seterr(OMC_ERR_ARG, "omc_server_poll: bad arguments");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return omc_seterr(OMC_ERR_ARG, "no request %lld", (long long)id);
+    *state = it->second->res.state;
+    return OMC_OK;
+}
+
+extern "C" int omc_server_wait(omc_server *s, int64_t id, omc_result *res) {
+    if (!s || !res) return omc_seterr(OMC_ERR_ARG, "omc_server_wait: bad arguments");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return omc_seterr(OMC_ERR_ARG, "no request %lld", (long long)id);
+    omc_server::Req *q = it->second;
+    s->cv.wait(lk, [&] { return q->res.state == OMC_REQ_DONE || q->res.state == OMC_REQ_FAILED; });
+    *res = q->res;
+    s->all.erase(it);
+    delete q;
+    return res->error;
+}
+
+extern "C" void omc_result_free(omc_result *res) {
+    if (!res) return;
+    free(res->sum);
+    free(res->sum2);
+    res->sum = res->sum2 = nullptr;
+    omc_csc_free(&res->dij);
+}
+
+extern "C" void omc_server_destroy(omc_server *s) {
+    if (!s) return;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        // finish what is admitted or queued
+        s->cv.wait(lk, [&] { return s->queue.empty() && s->ready.empty() && s->running == 0; });
+        s->stop = true;
+        s->cv.notify_all();
+    }
+    s->dispatcher.join();
+    for (auto &t : s->workers) t.join();
+    for (auto &kv : s->all) {
+        omc_result_free(&kv.second->res);
+        delete kv.second;
+    }
+    delete s;
+}

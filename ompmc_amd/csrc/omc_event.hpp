@@ -1,0 +1,299 @@
+// Event-queue ("batched") transport form.
+//
+// Particles live in structure-of-arrays pools in HBM, one pool per type
+// (photons, charged particles), double-buffered by round parity.  A round is
+// four launches on one stream:
+//   k_source     samples this round's injected primaries into the input pool,
+//   k_event<1>   one photon() call per queued photon,
+//   k_event<0>   up to `budget` condensed-history steps (electron_run) per
+//                queued electron/positron,
+//   k_round_end  retires the consumed input pools and plans the next
+//                injection so that the number of particles in flight stays
+//                near a target.
+// Queue lengths live in device memory (EvCtl), so rounds are enqueued
+// without host round trips; the host reads EvCtl now and then to see whether
+// the job has drained.  Surviving particles and the secondaries they produced
+// are appended to the output pools with one atomic per wavefront (ballot +
+// popcount for the lane offsets).  Every particle carries its own Philox
+// stream: primaries are keyed like the plain history form, secondaries by a
+// hash of the parent's stream and draw position, so a result does not
+// depend on scheduling, grid size or the target.
+#pragma once
+#include "omc_kernels.hpp"
+
+namespace omc {
+
+template <class R> struct Pool {
+    R *e, *x, *y, *z, *u, *v, *w, *wt;
+    R *demfp, *sig0;             // resumable electron state (charged pool)
+    int *ix, *iy, *iz, *iq;
+    int *phase;
+    int *slot;                   // tally slot (beamlet of a Dij group); 0 for dose runs
+    uint32_t *ctr;               // next draw of the particle's stream
+    uint64_t *stream;
+};
+
+// Philox positioned at draw `pos` of stream `stream`
+struct PhiloxAt {
+    Philox g;
+    uint32_t n;
+    __device__ __forceinline__ void at(uint64_t seed, uint64_t stream, uint32_t pos) {
+        g.init(seed, stream);
+        g.c2 = pos >> 2;
+        n = pos;
+        if (pos & 3u) {
+            g.round4();
+            g.pos = (int)(pos & 3u);
+        }
+    }
+    template <class R> __device__ __forceinline__ R next() {
+        n++;
+        return g.template next<R>();
+    }
+};
+
+__device__ __forceinline__ uint64_t child_stream(uint64_t s, uint32_t n) {
+    // splitmix64 finaliser of (parent stream, draw position)
+    uint64_t z = s ^ (0x9E3779B97F4A7C15ull * ((uint64_t)n + 1ull)) ^ 0xD1B54A32D192ED03ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Device-resident control block of one job (one batch of histories).
+struct EvCtl {
+    unsigned long long cnt[2][2];     // [round parity][photon, charged] queued particles
+    long long nhist, next_hist;       // histories of the job / injected so far
+    long long inj_h0, inj_n, inj_at;  // injection of the coming round
+    long long target, cap;
+    long long rounds, done_round;     // rounds retired; round at which the job drained (-1 while running)
+    unsigned long long events[2];     // photon / charged events processed
+    unsigned long long peak;          // largest number of particles in flight
+    int inj_dest;                     // pool of the primaries: 0 photons, 1 charged
+    int err;                          // bit 0: pool overflow
+};
+
+template <class R> struct EvArgs {
+    Tables<R> t;
+    Geom<R> g;
+    uint64_t seed;
+    Pool<R> in;                       // input pool of this kernel's type
+    const unsigned long long *n_in;   // its length
+    Pool<R> outp, oute;               // photon and charged output pools
+    unsigned long long *n_out;        // [2] their lengths
+    long long cap;                    // capacity of each pool
+    double *endep;                    // tally base
+    long slot_stride;                 // tally stride per slot
+    unsigned long long *counters;     // [8] event counters (optional)
+    int *err;                         // EvCtl::err
+    int budget;                       // electron ustep iterations per event
+};
+
+template <class R> __device__ __forceinline__ void load(const Pool<R> &P, long i, Part<R> &p) {
+    p.e = P.e[i]; p.x = P.x[i]; p.y = P.y[i]; p.z = P.z[i];
+    p.u = P.u[i]; p.v = P.v[i]; p.w = P.w[i]; p.wt = P.wt[i];
+    p.ix = P.ix[i]; p.iy = P.iy[i]; p.iz = P.iz[i]; p.iq = P.iq[i];
+}
+
+template <class R>
+__device__ __forceinline__ void store(const Pool<R> &P, long i, const Part<R> &p, R demfp, R sig0, int phase,
+                                      uint64_t stream, uint32_t ctr, int slot) {
+    P.e[i] = p.e; P.x[i] = p.x; P.y[i] = p.y; P.z[i] = p.z;
+    P.u[i] = p.u; P.v[i] = p.v; P.w[i] = p.w; P.wt[i] = p.wt;
+    P.ix[i] = p.ix; P.iy[i] = p.iy; P.iz[i] = p.iz; P.iq[i] = p.iq;
+    P.demfp[i] = demfp; P.sig0[i] = sig0; P.phase[i] = phase;
+    P.stream[i] = stream; P.ctr[i] = ctr; P.slot[i] = slot;
+}
+
+// Reserves an output slot for every lane with `want`, one atomic per wave.
+// Reached by all lanes of the wavefront together.
+__device__ __forceinline__ long long wave_reserve(unsigned long long *counter, bool want) {
+    unsigned long long mask = __ballot(want);
+    if (mask == 0) return -1;
+    int lane = threadIdx.x & 63;
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader);
+    unsigned long long below = mask & ((1ull << lane) - 1ull);
+    return want ? (long long)(base + __popcll(below)) : -1;
+}
+
+// Appends a particle to the photon or the charged output pool.
+template <class R>
+__device__ __forceinline__ void emit(const EvArgs<R> &a, bool want, const Part<R> &p, R demfp, R sig0, int phase,
+                                     uint64_t stream, uint32_t ctr, int slot) {
+    bool ph = want && p.iq == 0, ch = want && p.iq != 0;
+    long long ip = wave_reserve(a.n_out + 0, ph);
+    long long ie = wave_reserve(a.n_out + 1, ch);
+    if (ph) {
+        if (ip < a.cap) store(a.outp, ip, p, demfp, sig0, phase, stream, ctr, slot);
+        else atomicOr(a.err, 1);
+    }
+    if (ch) {
+        if (ie < a.cap) store(a.oute, ie, p, demfp, sig0, phase, stream, ctr, slot);
+        else atomicOr(a.err, 1);
+    }
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void flush_counters(unsigned long long *dst, const Counters &c) {
+    if (!COUNT || !dst) return;
+    unsigned long long v[7] = {c.nhowfar, c.nhownear, c.ncross, c.ndep, c.nrn, c.nphoton, c.nelectron};
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        unsigned long long x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        if ((threadIdx.x & 63) == 0 && x) atomicAdd(dst + k, x);
+    }
+}
+
+// One event per queued particle: PHOTON selects photon(), otherwise up to
+// a.budget condensed-history steps of electron_run().  Waves stride over the
+// queue, so any grid covers any queue length.
+template <class R, bool PHOTON, bool COUNT>
+__global__ void __launch_bounds__(256) k_event(EvArgs<R> a) {
+    const long n = (long)*a.n_in;
+    const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
+    const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    Counters tot = {0, 0, 0, 0, 0, 0, 0};
+    for (long base = wave * 64; base < n; base += nwaves * 64) {
+        long i = base + lane;
+        bool have = i < n;
+        Part<R> p, q;
+        int d = -1;
+        uint64_t stream = 0;
+        int slot = 0, phase = 0;
+        R demfp = (R)0, sig0 = (R)0;
+        uint32_t qpos = 0;
+        if (have) {
+            load(a.in, i, p);
+            stream = a.in.stream[i];
+            slot = a.in.slot[i];
+            PhiloxAt rng;
+            rng.at(a.seed, stream, a.in.ctr[i]);
+            TallyAtomic tally;
+            tally.init(a.endep + (size_t)slot * a.slot_stride);
+            Engine<R, PhiloxAt, TallyAtomic> e{a.t, a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
+            if (PHOTON) {
+                d = e.photon(p, q);
+            } else {
+                typename Engine<R, PhiloxAt, TallyAtomic>::EState es;
+                es.demfp = a.in.demfp[i];
+                es.sig0 = a.in.sig0[i];
+                es.phase = a.in.phase[i];
+                d = e.electron_run(p, q, es, a.budget);
+                demfp = es.demfp;
+                sig0 = es.sig0;
+                phase = es.phase;
+            }
+            tally.flush();
+            qpos = rng.n;
+            if (COUNT) {
+                tot.nhowfar += e.cnt.nhowfar; tot.nhownear += e.cnt.nhownear; tot.ncross += e.cnt.ncross;
+                tot.ndep += e.cnt.ndep; tot.nrn += e.cnt.nrn; tot.nphoton += e.cnt.nphoton;
+                tot.nelectron += e.cnt.nelectron;
+            }
+        }
+        // d: -1 particle gone, 0 particle continues, +1 continues with a secondary
+        emit(a, have && d >= 0, p, demfp, sig0, phase, stream, qpos, slot);
+        emit(a, have && d == 1, q, (R)0, (R)0, 0, child_stream(stream, qpos), 0u, slot);
+    }
+    flush_counters<COUNT>(a.counters, tot);
+}
+
+// Primaries of the coming round: histories [inj_h0, inj_h0 + inj_n) of the
+// job, written to the input pool of their type from offset inj_at.
+template <class R> struct SrcArgs {
+    Tables<R> t;
+    Geom<R> g;
+    SrcDosxyz<R> dos;
+    SrcBeamlets<R> bix;
+    int use_beamlets;
+    const long *slot_off;        // beamlet source: [nslot+1] first history of each slot in the job
+    const int *slot_beamlet;     // beamlet of each slot
+    int nslot;
+    int batch;
+    long hist_base;              // dosxyz source: stream of history 0 of the job
+    uint64_t seed;
+    const EvCtl *ctl;
+    Pool<R> out;                 // destination pool (photons or charged, by source charge)
+};
+
+template <class R>
+__global__ void __launch_bounds__(256) k_source(SrcArgs<R> a) {
+    const long n = (long)a.ctl->inj_n, h0 = (long)a.ctl->inj_h0, at = (long)a.ctl->inj_at;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        long h = h0 + k;
+        int slot = 0, beamlet = 0;
+        uint64_t stream = (uint64_t)(a.hist_base + h);
+        if (a.use_beamlets) {
+            slot = find_slot(a.slot_off, a.nslot, h);
+            beamlet = a.slot_beamlet[slot];
+            stream = beamlet_stream(beamlet, a.batch, h - a.slot_off[slot]);
+        }
+        PhiloxAt rng;
+        rng.at(a.seed, stream, 0);
+        TallyNone tn;
+        Engine<R, PhiloxAt, TallyNone> e{a.t, a.g, rng, tn, {0, 0, 0, 0, 0, 0, 0}, false};
+        Part<R> p;
+        if (a.use_beamlets) init_history(e, a.bix, beamlet, p);
+        else init_history(e, a.dos, p);
+        store(a.out, at + k, p, (R)0, (R)0, 0, stream, rng.n, slot);
+    }
+}
+
+// Retires the round that read the pools of parity `par` and plans the
+// injection of the next round (pools of parity 1 - par).  One thread.
+__device__ __forceinline__ void round_end(EvCtl *c, int par) {
+    c->events[0] += c->cnt[par][0];
+    c->events[1] += c->cnt[par][1];
+    c->cnt[par][0] = 0;
+    c->cnt[par][1] = 0;
+    int nx = 1 - par;
+    for (int k = 0; k < 2; k++)
+        if ((long long)c->cnt[nx][k] > c->cap) {
+            c->err |= 1;
+            c->cnt[nx][k] = (unsigned long long)c->cap;
+        }
+    long long inflight = (long long)(c->cnt[nx][0] + c->cnt[nx][1]);
+    long long room = c->target - inflight;
+    long long left = c->nhist - c->next_hist;
+    long long space = c->cap - (long long)c->cnt[nx][c->inj_dest];
+    long long n = left < room ? left : room;
+    if (n > space) n = space;
+    if (n < 0) n = 0;
+    c->inj_h0 = c->next_hist;
+    c->inj_n = n;
+    c->inj_at = (long long)c->cnt[nx][c->inj_dest];
+    c->cnt[nx][c->inj_dest] += (unsigned long long)n;
+    c->next_hist += n;
+    if ((unsigned long long)(inflight + n) > c->peak) c->peak = (unsigned long long)(inflight + n);
+    c->rounds++;
+    if (inflight + n == 0 && c->done_round < 0) c->done_round = c->rounds;
+}
+
+static __global__ void k_round_end(EvCtl *c, int par) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) round_end(c, par);
+}
+
+// Starts a job: clears the queues and plans the first injection into the
+// pools of parity 0.
+static __global__ void k_job_start(EvCtl *c, long long nhist, long long target, long long cap, int dest) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    c->cnt[0][0] = c->cnt[0][1] = c->cnt[1][0] = c->cnt[1][1] = 0;
+    c->nhist = nhist;
+    c->next_hist = 0;
+    c->target = target;
+    c->cap = cap;
+    c->rounds = -1;
+    c->done_round = -1;
+    c->events[0] = c->events[1] = 0;
+    c->peak = 0;
+    c->inj_dest = dest;
+    c->err = 0;
+    round_end(c, 1);
+}
+
+}  // namespace omc

@@ -1,5 +1,5 @@
 // FP32-transport instantiations (FP64 tallies).
-#include "omc_launch.hpp"
+#include "omc_launch_impl.hpp"
 
 namespace omc {
 
@@ -23,26 +23,14 @@ hipError_t launch_accumulate(double *e, double *sum, double *sum2, long n, doubl
     return hipGetLastError();
 }
 
-template <class R> hipError_t launch_history(const HistArgs<R> &a, int rng, bool count, int grid, int block,
-                                             hipStream_t st) {
-    if (rng == RNG_REPLAY) {
-        if (count) hipLaunchKernelGGL((k_history<R, Replay, true>), dim3(grid), dim3(block), 0, st, a);
-        else hipLaunchKernelGGL((k_history<R, Replay, false>), dim3(grid), dim3(block), 0, st, a);
-    } else {
-        if (count) hipLaunchKernelGGL((k_history<R, Philox, true>), dim3(grid), dim3(block), 0, st, a);
-        else hipLaunchKernelGGL((k_history<R, Philox, false>), dim3(grid), dim3(block), 0, st, a);
-    }
-    return hipGetLastError();
-}
-
-template <class R> hipError_t launch_sampler(const SampArgs<R> &a, hipStream_t st) {
-    int block = 64;
-    int grid = (a.ncase + block - 1) / block;
-    hipLaunchKernelGGL((k_sampler<R>), dim3(grid), dim3(block), 0, st, a);
-    return hipGetLastError();
-}
-
 template hipError_t launch_history<float>(const HistArgs<float> &, int, bool, int, int, hipStream_t);
 template hipError_t launch_sampler<float>(const SampArgs<float> &, hipStream_t);
+template hipError_t launch_rounds<float>(const EvLaunch<float> &, long, int, bool, hipStream_t);
+template hipError_t launch_round_part<float>(const EvLaunch<float> &, long, int, bool, hipStream_t);
+
+hipError_t launch_job_start(EvCtl *c, long long nhist, long long target, long long cap, int dest, hipStream_t st) {
+    hipLaunchKernelGGL(k_job_start, dim3(1), dim3(64), 0, st, c, nhist, target, cap, dest);
+    return hipGetLastError();
+}
 
 }  // namespace omc

@@ -145,13 +145,17 @@ template <class R> struct HistArgs {
     SrcDosxyz<R> dos;
     SrcBeamlets<R> bix;
     int use_beamlets;            // 0: dosxyz source; 1: beamlet source
-    const int *hist_beamlet;     // beamlet of each history (beamlet source), or null -> beamlet0
-    int beamlet0;
+    const long *slot_off;        // beamlet source: [nslot+1] first history of each slot in this launch
+    const int *slot_beamlet;     // beamlet of each slot
+    int nslot;
+    long slot_stride;            // tally stride between slots (doubles)
+    int batch;                   // batch index (part of the beamlet random stream key)
     long nhist;                  // histories in this launch
     long hist_base;              // global index of history 0 (random stream id)
     uint64_t seed;
     const uint32_t *replay;      // RNG_REPLAY: 24-bit stream
     const long *replay_off;      // [nhist+1] first draw of each history
+    const long *replay_end;      // optional [nhist] end of each history's draws (else replay_off[h+1])
     double *endep;               // [nreg] FP64 tally, region 0 outside
     unsigned long long *counters;// [8] event counters (optional)
     int *err;                    // [4] 0 overflow, 1 replay underrun, 2 step guard
@@ -159,18 +163,43 @@ template <class R> struct HistArgs {
     int chunk;
 };
 
-template <class R, class G> __device__ __forceinline__ void make_rng(G &g, const HistArgs<R> &a, long h);
-template <> __device__ __forceinline__ void make_rng<float, Philox>(Philox &g, const HistArgs<float> &a, long h) {
-    g.init(a.seed, (uint64_t)(a.hist_base + h));
+// Random stream of history h_local of batch `batch` of a beamlet: the
+// beamlet, batch and history index occupy disjoint bit fields, so every
+// history of a Dij run has its own stream whatever the grouping of beamlets.
+__device__ __forceinline__ uint64_t beamlet_stream(int beamlet, int batch, long h_local) {
+    return ((uint64_t)(beamlet + 1) << 40) + ((uint64_t)batch << 32) + (uint64_t)h_local;
 }
-template <> __device__ __forceinline__ void make_rng<double, Philox>(Philox &g, const HistArgs<double> &a, long h) {
-    g.init(a.seed, (uint64_t)(a.hist_base + h));
+
+// Slot of history h: the largest s with off[s] <= h (off[0] = 0 <= h < off[n]).
+__device__ __forceinline__ int find_slot(const long *off, int n, long h) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (off[mid] <= h) lo = mid;
+        else hi = mid;
+    }
+    return lo;
 }
-template <> __device__ __forceinline__ void make_rng<float, Replay>(Replay &g, const HistArgs<float> &a, long h) {
-    g.base = a.replay; g.pos = a.replay_off[h]; g.end = a.replay_off[h + 1]; g.err = a.err + 1;
+
+template <class R, class G>
+__device__ __forceinline__ void make_rng(G &g, const HistArgs<R> &a, long h, uint64_t stream);
+template <> __device__ __forceinline__ void make_rng<float, Philox>(Philox &g, const HistArgs<float> &a, long,
+                                                                   uint64_t stream) {
+    g.init(a.seed, stream);
 }
-template <> __device__ __forceinline__ void make_rng<double, Replay>(Replay &g, const HistArgs<double> &a, long h) {
-    g.base = a.replay; g.pos = a.replay_off[h]; g.end = a.replay_off[h + 1]; g.err = a.err + 1;
+template <> __device__ __forceinline__ void make_rng<double, Philox>(Philox &g, const HistArgs<double> &a, long,
+                                                                    uint64_t stream) {
+    g.init(a.seed, stream);
+}
+template <> __device__ __forceinline__ void make_rng<float, Replay>(Replay &g, const HistArgs<float> &a, long h,
+                                                                   uint64_t) {
+    g.base = a.replay; g.pos = a.replay_off[h]; g.end = a.replay_end ? a.replay_end[h] : a.replay_off[h + 1];
+    g.err = a.err + 1;
+}
+template <> __device__ __forceinline__ void make_rng<double, Replay>(Replay &g, const HistArgs<double> &a, long h,
+                                                                    uint64_t) {
+    g.base = a.replay; g.pos = a.replay_off[h]; g.end = a.replay_end ? a.replay_end[h] : a.replay_off[h + 1];
+    g.err = a.err + 1;
 }
 
 // Plain history-by-history form: one lane runs whole showers, LIFO order,
@@ -186,13 +215,20 @@ __global__ void __launch_bounds__(256) k_history(HistArgs<R> a) {
         h0 = __shfl(h0, 0) + (long)(threadIdx.x % warpSize) * a.chunk;
         if (__shfl(h0, 0) >= a.nhist) break;
         for (long h = h0; h < h0 + a.chunk && h < a.nhist; h++) {
+            int slot = 0, beamlet = 0;
+            uint64_t stream = (uint64_t)(a.hist_base + h);
+            if (a.use_beamlets) {
+                slot = find_slot(a.slot_off, a.nslot, h);
+                beamlet = a.slot_beamlet[slot];
+                stream = beamlet_stream(beamlet, a.batch, h - a.slot_off[slot]);
+            }
             G rng;
-            make_rng<R, G>(rng, a, h);
+            make_rng<R, G>(rng, a, h, stream);
             TallyAtomic tally;
-            tally.init(a.endep);
+            tally.init(a.endep + (size_t)slot * a.slot_stride);
             Engine<R, G, TallyAtomic> e{a.t, a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
             Part<R> p, q;
-            if (a.use_beamlets) init_history(e, a.bix, a.hist_beamlet ? a.hist_beamlet[h] : a.beamlet0, p);
+            if (a.use_beamlets) init_history(e, a.bix, beamlet, p);
             else init_history(e, a.dos, p);
             int depth = 0;
             long guard = 0;

@@ -3,7 +3,8 @@
 // so that it rounds like the CPU reference).
 #pragma once
 #include <hip/hip_runtime.h>
-#include "omc_kernels.hpp"
+#include "omc_event.hpp"
+#include "omc_dij.hpp"
 
 namespace omc {
 
@@ -11,5 +12,39 @@ template <class R> hipError_t launch_history(const HistArgs<R> &a, int rng, bool
                                              hipStream_t st);
 template <class R> hipError_t launch_sampler(const SampArgs<R> &a, hipStream_t st);
 hipError_t launch_accumulate(double *e, double *sum, double *sum2, long n, double scale, hipStream_t st);
+
+// Everything the rounds of an event-queue job need.
+template <class R> struct EvLaunch {
+    Tables<R> t;
+    Geom<R> g;
+    uint64_t seed;
+    Pool<R> pool[2][2];          // [round parity][photon, charged]
+    EvCtl *ctl;
+    long long cap;
+    double *endep;
+    long slot_stride;
+    unsigned long long *counters;
+    int budget;
+    SrcArgs<R> src;              // source of the job (ctl and out are set per round)
+    int grid_event, grid_source;
+};
+
+// Enqueues rounds r0 .. r0+nround-1 of the job whose control block is L.ctl.
+template <class R> hipError_t launch_rounds(const EvLaunch<R> &L, long r0, int nround, bool count, hipStream_t st);
+// One part of round r: 0 source, 1 photon events, 2 charged events, 3 round end.
+template <class R> hipError_t launch_round_part(const EvLaunch<R> &L, long r, int part, bool count, hipStream_t st);
+hipError_t launch_job_start(EvCtl *c, long long nhist, long long target, long long cap, int dest, hipStream_t st);
+
+// Dij post-processing (omc_dij.hpp), FP64
+hipError_t launch_accum_slots(double *e, double *sum, double *sum2, long n, long stride, int nslot, const double *nper,
+                              hipStream_t st);
+hipError_t launch_dij_finish(double *sum, double *sum2, double *carry, int nslot, long stride, long nvox,
+                             const double *mass, const double *rho, int nbatch, double *blockmax, double *tot_dose,
+                             double *tot_var, int *nblk, hipStream_t st);
+hipError_t launch_dij_select(const double *dose, long stride, long nvox, int nslot, const double *blockmax, int nblk,
+                             double rel, double *thresh, int *counts, hipStream_t st);
+hipError_t launch_dij_compact(const double *dose, const double *var, long stride, long nvox, int nslot,
+                              const double *thresh, const int64_t *off, int64_t *ir, double *val, double *vval,
+                              hipStream_t st);
 
 }  // namespace omc

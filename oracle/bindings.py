@@ -37,6 +37,9 @@ def lib():
         L.orc_session_finish.argtypes = [vp, dp, dp, C.c_int, C.c_int, dp, dp]
         L.orc_session_ranmar.argtypes = [vp, C.c_long, ip]
         L.orc_ranmar_stream.argtypes = [C.c_int, C.c_int, C.c_long, C.c_long, ip]
+        L.orc_session_matrad.argtypes = [vp, C.c_int, C.c_double, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int,
+                                         C.c_double, ip, C.c_int, dp, dp, ip]
+        L.orc_session_matrad_finish.argtypes = [vp, dp, dp, C.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -127,3 +130,51 @@ class Session:
         out = np.zeros(n, np.int32)
         lib().orc_session_ranmar(self.h, n, _ip(out))
         return out
+
+    def matrad(self, spectrum, bix, nper, nbatch, want_counts=False):
+        """Serial matRad Dij transport (reference loop order).  spectrum: an
+        object with energy / cdfinv1 / cdfinv2 (cdfinv1 None for mono); bix:
+        dict with ibeam, src, corner, side1, side2 (float arrays), gaussian,
+        sigma; nper: histories per batch of each beamlet.  Returns per-beamlet
+        sums [nbix, nreg] and sums of squares (no carried variance), and the
+        draws per history when asked."""
+        nb = len(nper)
+        s = np.zeros((nb, self.nreg))
+        s2 = np.zeros((nb, self.nreg))
+        nper = np.ascontiguousarray(nper, np.int32)
+        cnt = np.zeros(int(nper.sum()) * nbatch, np.int32)
+        sp = spectrum.cdfinv1 is not None
+        c1 = np.ascontiguousarray(spectrum.cdfinv1 if sp else np.zeros(1000))
+        c2 = np.ascontiguousarray(spectrum.cdfinv2 if sp else np.zeros(1000))
+        arr = {k: np.ascontiguousarray(bix[k], np.float64) for k in ("src", "corner", "side1", "side2")}
+        ib = np.ascontiguousarray(bix["ibeam"], np.int32)
+        rc = lib().orc_session_matrad(self.h, int(sp), float(spectrum.energy), _dp(c1), _dp(c2), nb, _ip(ib),
+                                      _dp(arr["src"]), _dp(arr["corner"]), _dp(arr["side1"]), _dp(arr["side2"]),
+                                      int(bix.get("gaussian", 0)), float(bix.get("sigma", 0.0)), _ip(nper), nbatch,
+                                      _dp(s), _dp(s2), _ip(cnt) if want_counts else None)
+        if rc:
+            raise RuntimeError(f"oracle matrad run failed: {rc}")
+        return (s, s2, cnt) if want_counts else (s, s2)
+
+    def matrad_dij(self, s, s2, nbatch, rel):
+        """The reference's per-beamlet finish with the variance carried from
+        beamlet to beamlet (omc_matrad.c:1603-1709), thresholded strictly above
+        rel * max: CSC (jc, ir, val, var) with 0-based voxel rows."""
+        nb, nreg = s.shape
+        nv = nreg - 1
+        carry = np.zeros(nreg)
+        jc, irs, vals, vars_ = [0], [], [], []
+        for b in range(nb):
+            dose, var = np.zeros(nv), np.zeros(nv)
+            s2b = np.ascontiguousarray(s2[b] + carry)
+            lib().orc_session_matrad_finish(self.h, _dp(np.ascontiguousarray(s[b])), _dp(s2b), nbatch, _dp(dose),
+                                            _dp(var))
+            carry = np.concatenate([[0.0], var])
+            th = dose.max() * rel
+            idx = np.nonzero(dose > th)[0]
+            irs.append(idx)
+            vals.append(dose[idx])
+            vars_.append(var[idx])
+            jc.append(jc[-1] + len(idx))
+        return (np.array(jc, np.int64), np.concatenate(irs).astype(np.int64), np.concatenate(vals),
+                np.concatenate(vars_))

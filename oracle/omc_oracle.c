@@ -1797,3 +1797,39 @@ int orc_run_matrad_beamlet(orc *o, const omc_spectrum *spec, const orc_beamlets 
     }
     return OMC_OK;
 }
+
+/* Serial matRad Dij transport in the reference's loop order
+ * (omc_matrad.c:1576-1600): beamlet by beamlet, batch by batch, nper[b]
+ * histories per batch on one RANMAR stream.  sum/sum2 + b*nreg receive
+ * beamlet b's per-batch sums of endep/nper[b] and of its square (accumEndep,
+ * :897-918); rn_per_history (optional) the draws of each history. */
+int orc_run_matrad(const omc_tables *t, const omc_regions *r, const orc_geom *g, const omc_spectrum *spec,
+                   const orc_beamlets *b, int nsplit, int ixx, int jxx, const int *nper, int nbatch, double *sum,
+                   double *sum2, int *rn_per_history) {
+    orc *o = orc_create(t, r, g, nsplit, ixx, jxx);
+    if (!o) return OMC_ERR_NOMEM;
+    long h = 0;
+    int nreg = r->nreg;
+    for (int ib = 0; ib < b->n; ib++) {
+        double *sb = sum + (long)ib * nreg, *s2b = sum2 + (long)ib * nreg;
+        for (int bt = 0; bt < nbatch; bt++) {
+            for (int i = 0; i < nper[ib]; i++) {
+                long before = o->count;
+                orc_init_history_matrad(o, spec, b, ib);
+                orc_shower(o);
+                if (rn_per_history) rn_per_history[h] = (int)(o->count - before);
+                h++;
+                if (o->err) { orc_destroy(o); return OMC_ERR_STACK; }
+            }
+            for (int i = 0; i < nreg; i++) {
+                double e = o->endep[i];
+                e /= (double)nper[ib];
+                sb[i] += e;
+                s2b[i] += e * e;
+            }
+            memset(o->endep, 0, sizeof(double) * nreg);
+        }
+    }
+    orc_destroy(o);
+    return OMC_OK;
+}

@@ -107,6 +107,12 @@ double orc_run_dosxyz_omp(const omc_tables *t, const omc_regions *r, const orc_g
  * means accumulated into sum/sum2 [nreg] (sum2 must hold the carried variance). */
 int orc_run_matrad_beamlet(orc *o, const omc_spectrum *spec, const orc_beamlets *b, int ibeamlet,
                            int nperbatch, int nbatch, double *sum, double *sum2);
+/* Serial matRad Dij transport of all beamlets in the reference's loop order
+ * (beamlet, batch, history) on one RANMAR stream; per-beamlet sums [n*nreg]
+ * without the carried variance, draws per history optional. */
+int orc_run_matrad(const omc_tables *t, const omc_regions *r, const orc_geom *g, const omc_spectrum *spec,
+                   const orc_beamlets *b, int nsplit, int ixx, int jxx, const int *nper, int nbatch, double *sum,
+                   double *sum2, int *rn_per_history);
 
 #ifdef __cplusplus
 }

@@ -251,3 +251,37 @@ void orc_ranmar_stream(int ixx, int jxx, long skip, long n, int *out) {
     for (long i = 0; i < n; i++) out[i] = (int)(orc_rand(o) * 16777216.0);
     orc_destroy(o);
 }
+
+/* matRad Dij transport of the session's phantom and tables, serial on one
+ * RANMAR stream in the reference's loop order (omc_matrad.c:1576-1600):
+ * beamlet by beamlet, batch by batch, nper[b] histories per batch.  For
+ * beamlet b, sum/sum2 + b*nreg receive the per-batch sums of endep/nper[b]
+ * and of its square (accumEndep, :897-918) without the variance carried from
+ * the previous beamlet; rn_per_history (optional) the draws of each history
+ * in run order. */
+int orc_session_matrad(orc_session *s, int spectrum, double energy, const double *cdf1, const double *cdf2, int nbix,
+                       const int *ibeam, const double *src, const double *corner, const double *side1,
+                       const double *side2, int gaussian, double sigma, const int *nper, int nbatch, double *sum,
+                       double *sum2, int *rn_per_history) {
+    omc_spectrum sp;
+    memset(&sp, 0, sizeof sp);
+    if (spectrum) {
+        sp.spectrum = 1;
+        sp.deltak = OMC_INVDIM;
+        memcpy(sp.cdfinv1, cdf1, sizeof sp.cdfinv1);
+        memcpy(sp.cdfinv2, cdf2, sizeof sp.cdfinv2);
+    } else {
+        omc_spectrum_mono(&sp, energy);
+    }
+    orc_beamlets bl = {nbix, ibeam, src, corner, side1, side2, gaussian, sigma};
+    return orc_run_matrad(&s->t, &s->r, &s->g, &sp, &bl, s->nsplit, s->ixx, s->jxx, nper, nbatch, sum, sum2,
+                          rn_per_history);
+}
+
+/* The reference's per-beamlet finish (omc_matrad.c:920-1012) on the
+ * session's phantom: dose and variance [nvox] from sum/sum2 [nreg], where
+ * sum2 already holds the carried variance. */
+void orc_session_matrad_finish(orc_session *s, const double *sum, const double *sum2, int nbatch, double *dose,
+                               double *var) {
+    omc_matrad_finish(&s->ph, sum, sum2, nbatch, dose, var);
+}

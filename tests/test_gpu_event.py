@@ -1,0 +1,124 @@
+"""Event-queue form against the oracle and against the plain history form.
+
+The event form keys every particle's random stream by its history and the
+draw position of its parent, so (a) its dose is independent of the queue
+target, the grid and the electron step budget up to the order of FP64 atomic
+additions, and (b) it agrees with the oracle (RANMAR, the reference's CPU
+algorithm) statistically: per voxel above 10% of the maximum, the difference
+of the two batch means against their combined batch uncertainty."""
+import numpy as np
+import pytest
+
+from tests.conftest import DOSE_CASES, golden, write_case_inp
+
+pytestmark = pytest.mark.gpu
+
+omc = pytest.importorskip("ompmc_amd")
+B = pytest.importorskip("oracle.bindings")
+P = omc.phantoms
+
+
+def _spectrum(kw):
+    return omc.Spectrum.read(omc.os.path.join(omc.DATA, "spectra", "mohan6.spectrum")) \
+        if kw.get("spectrum") else omc.Spectrum.mono(kw["energy"])
+
+
+def _run(ctx, ph, name, nh, nb, **opts):
+    kw = DOSE_CASES[name][0]
+    return ctx.run_dosxyz(ph, _spectrum(kw), nh // nb, nb, collimator=kw.get("collimator", (-5, 5, -5, 5)),
+                          charge=kw.get("charge", 0), **opts)
+
+
+def _batch_stats(s1, s2, nb):
+    """Mean per batch and the uncertainty of that mean (batch method)."""
+    m = s1 / nb
+    var = np.maximum(s2 / nb - m * m, 0.0) / (nb - 1)
+    return m, np.sqrt(var)
+
+
+def _agreement(a1, a2, b1, b2, nb):
+    ma, ua = _batch_stats(a1[1:], a2[1:], nb)
+    mb, ub = _batch_stats(b1[1:], b2[1:], nb)
+    sel = np.maximum(ma, mb) > 0.1 * max(ma.max(), mb.max())
+    z = np.abs(ma[sel] - mb[sel]) / np.maximum(np.hypot(ua[sel], ub[sel]), 1e-300)
+    return float(np.mean(z <= 2.0)), float(np.mean(z <= 3.0)), int(sel.sum())
+
+
+@pytest.fixture(scope="module")
+def slab(tmp_path_factory):
+    ph_h = P.read_egsphant(golden("slab.egsphant"))
+    c = omc.Context(ph_h.media)
+    ph = c.phantom(ph_h, ecut=0.7, pcut=0.01)
+    yield c, ph
+    c.close()
+
+
+@pytest.mark.parametrize("name", ["slab", "electron", "positron"])
+def test_event_fp64_agrees_with_oracle(tmp_path, name):
+    """Event form (FP64, Philox) against the oracle (FP64, RANMAR) on the same
+    phantom, source and history count."""
+    nh, nb = 20000, 20
+    stem = write_case_inp(tmp_path, name)
+    s1, s2 = B.Session(stem).dose(nh // nb, nb)
+    ph_h = P.read_egsphant(golden(f"{name}.egsphant"))
+    c = omc.Context(ph_h.media)
+    ph = c.phantom(ph_h, ecut=0.7, pcut=0.01)
+    g1, g2, st = _run(c, ph, name, nh, nb, precision=omc.FP64, form=omc.FORM_EVENT, seed=11)
+    c.close()
+    assert st.errors == 0
+    f2, f3, n = _agreement(s1, s2, g1, g2, nb)
+    print(f"{name}: {n} voxels, {f2:.3f} within 2 sigma, {f3:.3f} within 3 sigma")
+    assert n > 20
+    assert f2 >= 0.88 and f3 >= 0.97, (f2, f3)
+    # total energy deposited in the phantom: within 4 sigma of the batch spread
+    ta, tb = s1[1:].sum(), g1[1:].sum()
+    assert abs(ta - tb) <= 0.02 * ta
+
+
+def test_event_result_independent_of_schedule(slab):
+    """Queue target, grid and electron step budget change the schedule, not the
+    particles: the dose agrees to FP64 summation order."""
+    c, ph = slab
+    ref = _run(c, ph, "slab", 4000, 4, precision=omc.FP64, form=omc.FORM_EVENT, seed=3)[0]
+    for kw in (dict(event_target=100, electron_budget=1), dict(event_target=777, electron_budget=3, grid=7),
+               dict(event_target=1 << 16, electron_budget=1000)):
+        g = _run(c, ph, "slab", 4000, 4, precision=omc.FP64, form=omc.FORM_EVENT, seed=3, **kw)[0]
+        assert np.allclose(g, ref, rtol=1e-10, atol=1e-12 * ref.max()), kw
+
+
+def test_event_matches_history_form_fp32(slab):
+    """Both GPU forms sample the same physics; FP32 transport."""
+    c, ph = slab
+    nh, nb = 200000, 20
+    h1, h2, sh = _run(c, ph, "slab", nh, nb, precision=omc.FP32, form=omc.FORM_HISTORY, seed=5)
+    e1, e2, se = _run(c, ph, "slab", nh, nb, precision=omc.FP32, form=omc.FORM_EVENT, seed=6)
+    assert sh.errors == 0 and se.errors == 0
+    f2, f3, n = _agreement(h1, h2, e1, e2, nb)
+    print(f"history vs event: {n} voxels, {f2:.3f} / {f3:.3f}")
+    assert f2 >= 0.88 and f3 >= 0.97
+
+
+def test_fp32_dose_within_half_percent_of_fp64(slab):
+    """FP32 transport against the FP64 GPU run: the dose summed over voxels
+    above 50% of the maximum differs by less than 0.5% (the statistical
+    uncertainty of that sum is about 0.1% here)."""
+    c, ph = slab
+    nh, nb = 400000, 20
+    a1, a2, _ = _run(c, ph, "slab", nh, nb, precision=omc.FP32, form=omc.FORM_EVENT, seed=21)
+    b1, b2, _ = _run(c, ph, "slab", nh, nb, precision=omc.FP64, form=omc.FORM_EVENT, seed=22)
+    ma, _ = _batch_stats(a1[1:], a2[1:], nb)
+    mb, _ = _batch_stats(b1[1:], b2[1:], nb)
+    sel = mb > 0.5 * mb.max()
+    rel = abs(ma[sel].sum() / mb[sel].sum() - 1.0)
+    print(f"FP32 vs FP64 dose (voxels > 50% max): {rel:.2e}")
+    assert rel < 5e-3
+
+
+def test_event_counts_and_kernel_times(slab):
+    c, ph = slab
+    _, _, st = _run(c, ph, "slab", 20000, 2, precision=omc.FP32, form=omc.FORM_EVENT, count_events=True)
+    assert st.n_howfar > 0 and st.n_hownear > 0 and st.n_dep > 0
+    assert st.ev_howfar[0] + st.ev_howfar[1] == st.n_howfar
+    assert st.events_photon > 0 and st.events_charged > 0 and st.rounds > 0
+    assert st.kernel_seconds[2] > 0 and st.kernel_seconds[3] > 0
+    assert st.peak_inflight >= 1
