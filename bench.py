@@ -1,20 +1,31 @@
 #!/usr/bin/env python3
 """Throughput benchmark of the GPU transport path.
 
-A "step" is one statistical batch of histories of the configured workload,
-with phantom and tables already resident in HBM.  Prints one JSON line:
-histories/s of the timed steps, the roofline of the transport kernel
-(algorithmic bytes per history from on-device event counters, over the
-kernel's time measured with HIP events on its stream), and a CPU baseline
-(the compiled reference, or the oracle restatement, on the host cores).
+Default workload (c3): the dose-influence matrix of a ~2000-beamlet plan on a
+160x160x100 synthetic CT (five beams, 0.5 cm bixels, lognormal histories per
+beamlet in 1e3..5e4, synthetic 6 MV spectrum, 10 batches, threshold 1e-4 of
+each beamlet's maximum), computed by the batched form (beamlet groups, event
+queues, FP32 transport with FP64 tallies).  A "step" is the whole matrix,
+phantom and tables resident in HBM.  Beside it the line reports:
 
-    python bench.py [--steps K] [--warmup W] [--config c2] [--form event|history]
-                    [--precision fp32|fp64] [--no-cpu]
+* the plain GPU history-by-history form running the reference's loop order
+  (one beamlet at a time) on every 20th beamlet, as histories/s;
+* the plan-dose statistical uncertainty in voxels above 50% of the maximum;
+* the FP32/FP64 plan-dose difference on every 5th beamlet;
+* p50/p99 latency and throughput of a short served request mix;
+* the roofline of the dominant transport kernel (algorithmic bytes from
+  on-device event counters over its duration, timed with HIP events on its
+  stream inside the timed steps);
+* a CPU baseline: the oracle restatement (OpenMP, the reference's loop) on a
+  sample of the same beamlets.
+
+    python bench.py [--steps K] [--warmup W] [--config c3|c2|c1] [--form event|history]
+                    [--precision fp32|fp64] [--no-cpu] [--no-serve] [--no-plain]
 """
 import argparse
 import json
+import math
 import os
-import subprocess
 import sys
 import tempfile
 import time
@@ -25,33 +36,11 @@ sys.path.insert(0, REPO)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+KERNEL_NAMES = {0: "k_history", 1: "k_source", 2: "k_event (photon)", 3: "k_event (charged)", 4: "post-processing"}
 
 
-def build_config(name):
-    from ompmc_amd import phantoms as P
-    import ompmc_amd as omc
-    if name == "c1":
-        ph = P.water_cube(60, 0.5)
-        spec = omc.Spectrum.mono(2.0)
-        return dict(workload="water 60^3 x 0.5 cm, 2 MeV photons, 10x10 cm parallel field",
-                    ph=ph, spec=spec, spec_file=None, energy=2.0, collimator=(-5, 5, -5, 5), nper=200000)
-    if name == "c2":
-        ph = P.slab_phantom(128, 0.3)
-        e, w = P.synthetic_6mv()
-        spec = omc.Spectrum.from_histogram(e, w)
-        return dict(workload="slab 128^3 x 0.3 cm (water/lung/bone/water), synthetic 6 MV, 10x10 cm parallel field",
-                    ph=ph, spec=spec, spec_hist=(e, w), energy=None, collimator=(-5, 5, -5, 5), nper=1000000)
-    raise SystemExit(f"unknown config {name}")
-
-
-def algorithmic_bytes(st):
-    """B = 72 N_ev + 5 N_cross + 16 N_dep per history: a read and a write of
-    the 36-byte particle state per transport event (howfar/hownear call), one
-    density word and medium byte per voxel entered, one FP64
-    read-modify-write per energy deposit."""
-    h = st.histories
-    n_ev = (st.n_howfar + st.n_hownear) / h
-    return 72.0 * n_ev + 5.0 * st.n_cross / h + 16.0 * st.n_dep / h, n_ev
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
 def cpu_cores():
@@ -62,139 +51,302 @@ def cpu_cores():
     return max(1, min(16, n))
 
 
-def cpu_baseline_reference(cfg, seconds_target=12.0):
-    """The compiled reference (oracle/_ref/omc_dosxyz, OpenMP) on a bounded
-    sample of the same workload, restricted with taskset to the host share."""
-    exe = os.path.join(REPO, "oracle", "_ref", "omc_dosxyz")
-    if not os.path.exists(exe):
-        return None
+def kernel_bytes(st, k):
+    """Algorithmic bytes of event kernel class k (0 photon, 1 charged):
+    72 per transport event (howfar/hownear call: read + write of the 36-byte
+    particle state), 5 per voxel entered (density + medium), 16 per FP64
+    deposit (read-modify-write)."""
+    return 72.0 * (st.ev_howfar[k] + st.ev_hownear[k]) + 5.0 * st.ev_cross[k] + 16.0 * st.ev_dep[k]
+
+
+def total_bytes(st):
+    return 72.0 * (st.n_howfar + st.n_hownear) + 5.0 * st.n_cross + 16.0 * st.n_dep
+
+
+def subset(bix, every, offset=0):
     from ompmc_amd import phantoms as P
-    cores = cpu_cores()
-    with tempfile.TemporaryDirectory() as tmp:
-        phf = os.path.join(tmp, "ph.egsphant")
-        P.write_egsphant(phf, cfg["ph"])
-        kw = {}
-        if cfg.get("spec_hist") is not None:
-            sf = os.path.join(tmp, "s.spectrum")
-            P.write_spectrum(sf, "synthetic", *cfg["spec_hist"])
-            kw["spectrum"] = sf
+    sel = np.arange(offset, bix.n, every)
+    return P.Beamlets(bix.ibeam[sel], bix.src, bix.corner[sel], bix.side1[sel], bix.side2[sel], bix.nhist[sel])
+
+
+def c3_config():
+    from ompmc_amd import phantoms as P
+    import ompmc_amd as omc
+    ph = P.synthetic_ct()
+    bix = P.beamlets()
+    e, w = P.synthetic_6mv()
+    return ph, bix, omc.Spectrum.from_histogram(e, w), (e, w)
+
+
+# ------------------------------------------------------------------ serving mix
+def serve_mix(ctx, rate_hist_per_s, nreq=12, load=0.8, seed=20261206, precision=32):
+    """A short Poisson-arrival request mix (phantoms 64^3 .. 192x192x128 at
+    0.2-0.5 cm, media water/lung/bone/air, half full-dose requests with
+    1e5..1e7 histories, half Dij requests with 1-9 beams); offered load is
+    `load` times the measured batched throughput."""
+    import ompmc_amd as omc
+    from ompmc_amd import phantoms as P
+    rng = np.random.default_rng(seed)
+    e, w = P.synthetic_6mv()
+    spec = omc.Spectrum.from_histogram(e, w)
+    reqs = []
+    for i in range(nreq):
+        nx = int(rng.integers(64, 193)); nz = int(rng.integers(64, 129))
+        vox = float(rng.uniform(0.2, 0.5))
+        ph = P.synthetic_ct(nx=nx, ny=nx, nz=nz, vx=vox, vy=vox, vz=vox, seed=int(rng.integers(1 << 30)),
+                            target_radius=min(2.0, 0.2 * nx * vox))
+        if i % 2 == 0:
+            nh = int(10 ** rng.uniform(5, 7))
+            half = min(5.0, 0.3 * nx * vox)
+            reqs.append(("dose", ph, nh, dict(collimator=(-half, half, -half, half))))
         else:
-            kw["energy"] = cfg["energy"]
+            nb = int(rng.integers(1, 10))
+            bix = P.beamlets(nbeams=nb, radius=min(2.0, 0.25 * nx * vox), hist_median=3000, hist_sigma_log10=0.3,
+                             hist_range=(1000, 20000), seed=int(rng.integers(1 << 30)))
+            reqs.append(("dij", ph, int(bix.nhist.sum()), dict(bix=bix)))
+    mean_h = np.mean([r[2] for r in reqs])
+    rate = load * rate_hist_per_s / mean_h               # requests per second
+    gaps = rng.exponential(1.0 / rate, nreq)
+    srv = omc.Server(ctx, max_inflight=4)
+    ids = []
+    t0 = time.perf_counter()
+    t_next = t0
+    for (kind, ph, nh, kw), gap in zip(reqs, gaps):
+        t_next += gap
+        dt = t_next - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        if kind == "dose":
+            ids.append(srv.submit_dose(ph, spec, max(1, nh // 10), 10, precision=precision, **kw))
+        else:
+            ids.append(srv.submit_dij(ph, spec, kw["bix"], nbatch=10, precision=precision))
+    res = [srv.wait(i) for i in ids]
+    wall = time.perf_counter() - t0
+    srv.close()
+    bad = [r for r in res if r["error"]]
+    lat = np.array([r["end_t"] - r["submit_t"] for r in res])
+    hist = sum(r["histories"] for r in res)
+    return dict(requests=nreq, failed=len(bad), offered_load=load, p50_s=float(np.percentile(lat, 50)),
+                p99_s=float(np.percentile(lat, 99)), histories_per_s=hist / wall, wall_s=wall,
+                mix="Poisson arrivals; phantoms 64^3..192x192x128 at 0.2-0.5 cm; half dose 1e5..1e7 histories, "
+                    "half Dij 1-9 beams")
 
-        def run(n):
-            d = P.default_inp(phf, tmp + "/", ncase=n, nbatch=10, collimator=cfg["collimator"], **kw)
-            stem = P.write_inp(os.path.join(tmp, "in"), d)[:-4]
-            cmd = ["taskset", "-c", f"0-{cores - 1}", exe, "-i", stem, "-o", "out"]
-            r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=600)
-            if r.returncode != 0:
-                return None
-            # transport phase: the two "Execution time up to this point" prints
-            ts = [float(l.split(":")[1].split()[0]) for l in r.stdout.splitlines()
-                  if "Execution time up to this point" in l]
-            return ts[-1] - ts[0] if len(ts) >= 2 else None
-        t = run(20000)
-        if not t or t <= 0:
-            return None
-        n = int(min(5e7, max(20000, 20000 * seconds_target / t)))
-        n -= n % 10
-        t = run(n)
-        if not t:
-            return None
-        return dict(value=n / t, unit="histories/s", cores=cores, kind="reference",
-                    sample=f"{n} histories of the same workload, reference omc_dosxyz (OpenMP, {cores} threads), "
-                           f"transport phase {t:.2f} s")
 
-
-def cpu_baseline_port(cfg, seconds_target=12.0):
-    """The oracle restatement (FP64, OpenMP) when the reference binary is absent."""
+# ------------------------------------------------------------------ CPU baseline
+def cpu_baseline_c3(ph, bix, spec, nbatch, seconds_target=12.0):
+    """The oracle restatement of the reference's matRad loop (OpenMP over the
+    histories of each batch, as omc_matrad.c:1578-1585) on the first beamlets
+    of the plan, on the host cores."""
     from oracle import bindings as OB
     from ompmc_amd import phantoms as P
     cores = cpu_cores()
     with tempfile.TemporaryDirectory() as tmp:
-        phf = os.path.join(tmp, "ph.egsphant")
-        P.write_egsphant(phf, cfg["ph"])
-        kw = {}
-        if cfg.get("spec_hist") is not None:
-            sf = os.path.join(tmp, "s.spectrum")
-            P.write_spectrum(sf, "synthetic", *cfg["spec_hist"])
-            kw["spectrum"] = sf
-        else:
-            kw["energy"] = cfg["energy"]
-        d = P.default_inp(phf, tmp + "/", ncase=1000, nbatch=10, collimator=cfg["collimator"], **kw)
-        s = OB.Session(P.write_inp(os.path.join(tmp, "in"), d)[:-4])
-        _, _, t = s.dose_omp(2000, 1, cores)
-        n = int(max(2000, 2000 * seconds_target / max(t, 1e-3)))
-        _, _, t = s.dose_omp(n, 1, cores)
-        return dict(value=n / t, unit="histories/s", cores=cores, kind="port",
-                    sample=f"{n} histories of the same workload, oracle restatement (OpenMP, {cores} threads)")
+        phf = os.path.join(tmp, "ct.egsphant")
+        P.write_egsphant(phf, ph)
+        d = P.default_inp(phf, tmp + "/", ncase=1000, nbatch=nbatch)
+        s = OB.Session(P.write_inp(os.path.join(tmp, "ct"), d)[:-4])
+        bd = dict(ibeam=bix.ibeam, src=bix.src.reshape(-1), corner=bix.corner.reshape(-1),
+                  side1=bix.side1.reshape(-1), side2=bix.side2.reshape(-1), gaussian=0, sigma=0.0)
+        nper = np.maximum(np.asarray(bix.nhist) // nbatch, 1).astype(np.int32)
+
+        def run(nb):
+            sub = {k: (v[:nb] if k == "ibeam" else v[:3 * nb] if k in ("corner", "side1", "side2") else v)
+                   for k, v in bd.items()}
+            _, _, secs = s.matrad_omp(spec, sub, nper[:nb], nbatch, cores)
+            return secs, int(nper[:nb].sum()) * nbatch
+        secs, nh = run(2)
+        nb = int(min(bix.n, max(2, 2 * seconds_target / max(secs, 1e-3))))
+        secs, nh = run(nb)
+        return dict(value=nh / secs, unit="histories/s", cores=cores, kind="port",
+                    sample=f"first {nb} beamlets of the plan ({nh} histories, {nbatch} batches), oracle "
+                           f"restatement of the reference's matRad loop, OpenMP {cores} threads, {secs:.1f} s")
+
+
+# ------------------------------------------------------------------ c3
+def bench_c3(a):
+    import ompmc_amd as omc
+    prec = omc.FP32 if a.precision == "fp32" else omc.FP64
+    form = omc.FORM_EVENT if a.form == "event" else omc.FORM_HISTORY
+    nbatch, rel = 10, 1e-4
+    t_setup = time.perf_counter()
+    ph, bix, spec, _ = c3_config()
+    ctx = omc.Context(ph.media)
+    gph = ctx.phantom(ph, ecut=0.7, pcut=0.01)
+    log(f"c3: {bix.n} beamlets, {int(np.sum(np.maximum(bix.nhist // nbatch, 1)) * nbatch)} histories per step, "
+        f"setup {time.perf_counter() - t_setup:.1f} s")
+    opts = dict(precision=prec, form=form, group=a.group, event_target=a.target, electron_budget=a.budget)
+    # event counts of the transport kernels (algorithmic bytes per history)
+    sub20 = subset(bix, 20)
+    _, _, _, stc = ctx.run_dij(gph, spec, sub20, nbatch=nbatch, rel_threshold=rel, count_events=True, **opts)
+    log(f"counts: {stc.histories:.0f} histories, {total_bytes(stc) / stc.histories:.0f} B/history")
+    for _ in range(a.warmup):
+        ctx.run_dij(gph, spec, bix, nbatch=nbatch, rel_threshold=rel, **opts)
+    log("warmup done")
+    walls, hist, plan, st_last = [], 0.0, None, None
+    ksec = np.zeros(5)
+    klaunch = np.zeros(5)
+    for k in range(a.steps):
+        t0 = time.perf_counter()
+        _, _, pl, st = ctx.run_dij(gph, spec, bix, nbatch=nbatch, rel_threshold=rel, want_plan=(k == a.steps - 1),
+                                   count_events="time", **opts)
+        walls.append(time.perf_counter() - t0)
+        hist += st.histories
+        ksec += np.array(st.kernel_seconds[:])
+        klaunch += np.array(st.kernel_launches[:])
+        if pl is not None:
+            plan = pl
+        st_last = st
+        log(f"step {k}: {st.histories / walls[-1]:.4g} histories/s, {st.groups} groups of {st.group_size}, "
+            f"nnz {st.nnz}")
+    wall = sum(walls)
+    value = hist / wall
+    pd, pv = plan
+    sel = pd > 0.5 * pd.max()
+    unc50 = float(np.mean(np.sqrt(pv[sel]) / pd[sel]))
+    # roofline of the dominant transport kernel
+    if form == omc.FORM_EVENT:
+        dom = 3 if ksec[3] >= ksec[2] else 2
+        bph = kernel_bytes(stc, dom - 2) / stc.histories
+    else:
+        dom = 0
+        bph = total_bytes(stc) / stc.histories
+    achieved = bph * hist / ksec[dom] / 1e9
+    traffic = read_traffic(dom)
+    out = {
+        "metric": "histories/s + p50/p99 request latency vs plain GPU form at <=1% dose unc.",
+        "value": value,
+        "unit": "histories/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 transport, f64 tallies" if prec == omc.FP32 else "f64",
+        "data": "synthetic CT, beamlets and 6 MV spectrum (seeded)",
+        "config": {"workload": "c3: Dij of a 5-beam plan, 160x160x100 synthetic CT at 0.25x0.25x0.3 cm, "
+                               f"{bix.n} beamlets of 0.5 cm, lognormal 1e3..5e4 histories each, 10 batches, "
+                               "threshold 1e-4", "beamlets": int(bix.n), "histories_per_step": hist / a.steps,
+                   "form": a.form, "precision": a.precision, "group": int(st_last.group_size)},
+        "plan_dose_rel_unc_gt50": unc50,
+        "nnz": int(st_last.nnz),
+        "kernels": {KERNEL_NAMES[k]: {"seconds": float(ksec[k]), "launches": int(klaunch[k])}
+                    for k in range(5) if klaunch[k] > 0},
+        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_history": bph,
+                     "per_launch_bytes": bph * hist / max(klaunch[dom], 1),
+                     "per_launch_seconds": ksec[dom] / max(klaunch[dom], 1)},
+    }
+    if not a.no_plain:
+        sub = subset(bix, 20)
+        _, _, _, sp = ctx.run_dij(gph, spec, sub, nbatch=nbatch, rel_threshold=rel, precision=prec,
+                                  form=omc.FORM_HISTORY, group=1)
+        out["plain_form"] = {"histories_per_s": sp.histories / sp.wall_seconds, "beamlets": int(sub.n),
+                             "what": "history kernel, one beamlet at a time (reference loop order), every 20th beamlet"}
+        out["speedup_vs_plain"] = value / out["plain_form"]["histories_per_s"]
+        log(f"plain: {out['plain_form']['histories_per_s']:.4g} histories/s")
+    if not a.no_fp64 and prec == omc.FP32:
+        sub = subset(bix, 5)
+        kw = dict(nbatch=nbatch, rel_threshold=rel, want_plan=True, form=form, group=a.group)
+        p32 = ctx.run_dij(gph, spec, sub, precision=omc.FP32, seed=77, **kw)[2]
+        p64 = ctx.run_dij(gph, spec, sub, precision=omc.FP64, seed=78, **kw)[2]
+        s = p64[0] > 0.5 * p64[0].max()
+        d = p32[0][s].sum() / p64[0][s].sum() - 1.0
+        sig = math.sqrt(p32[1][s].sum() + p64[1][s].sum()) / p64[0][s].sum()
+        out["fp64_plan_dose_rel_diff_gt50"] = {"value": d, "sigma": sig, "beamlets": int(sub.n)}
+        log(f"fp32/fp64: {d:+.3e} +- {sig:.1e}")
+    if not a.no_serve:
+        out["serving"] = serve_mix(ctx, value, precision=prec)
+        log(f"serving: {out['serving']}")
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline_c3(ph, bix, spec, nbatch)
+        except Exception as e:          # the baseline is reported, not required
+            log(f"cpu baseline failed: {e}")
+    out["cpu_baseline"] = cpu
+    ctx.close()
+    return out
+
+
+def read_traffic(kernel_class):
+    """HBM bytes per launch of the dominant kernel from the committed PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel), if present."""
+    path = os.path.join(REPO, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(KERNEL_NAMES[kernel_class])
+    except (OSError, ValueError):
+        return None
+
+
+# ------------------------------------------------------------------ c1 / c2 (single dose runs)
+def bench_dose(a):
+    import ompmc_amd as omc
+    from ompmc_amd import phantoms as P
+    if a.config == "c1":
+        ph, spec, nper = P.water_cube(60, 0.5), omc.Spectrum.mono(2.0), 100000
+        wl = "c1: water 60^3 x 0.5 cm, 2 MeV photons, 10x10 cm parallel field"
+    else:
+        ph = P.slab_phantom(128, 0.3)
+        spec, nper = omc.Spectrum.from_histogram(*P.synthetic_6mv()), 1000000
+        wl = "c2: slab 128^3 x 0.3 cm (water/lung/bone/water), synthetic 6 MV, 10x10 cm parallel field"
+    prec = omc.FP32 if a.precision == "fp32" else omc.FP64
+    form = omc.FORM_EVENT if a.form == "event" else omc.FORM_HISTORY
+    ctx = omc.Context(ph.media)
+    gph = ctx.phantom(ph, ecut=0.7, pcut=0.01)
+    kw = dict(collimator=(-5, 5, -5, 5), precision=prec, form=form, event_target=a.target, electron_budget=a.budget)
+    _, _, stc = ctx.run_dosxyz(gph, spec, max(10000, nper // 10), 2, seed=7, count_events=True, **kw)
+    for _ in range(a.warmup):
+        ctx.run_dosxyz(gph, spec, nper, 1, seed=5, **kw)
+    t0 = time.perf_counter()
+    s1, s2, st = ctx.run_dosxyz(gph, spec, nper, max(2, a.steps), seed=1802, count_events="time", **kw)
+    wall = time.perf_counter() - t0
+    ksec = np.array(st.kernel_seconds[:])
+    dom = 0 if form == omc.FORM_HISTORY else (3 if ksec[3] >= ksec[2] else 2)
+    bph = (total_bytes(stc) if dom == 0 else kernel_bytes(stc, dom - 2)) / stc.histories
+    achieved = bph * st.histories / ksec[dom] / 1e9
+    dose, unc = ctx.dose(gph, s1, s2, nper, max(2, a.steps))
+    m = dose > 0.5 * dose.max()
+    out = {"metric": "histories/s (dose run)", "value": st.histories / wall, "unit": "histories/s", "n_gpus": 1,
+           "steps": max(2, a.steps), "warmup": a.warmup, "ms_per_step": 1e3 * wall / max(2, a.steps),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 transport, f64 tallies" if prec == omc.FP32 else "f64", "data": "synthetic (seeded)",
+           "config": {"workload": wl, "histories_per_step": nper, "form": a.form, "precision": a.precision},
+           "mean_rel_unc_gt50": float(np.mean(unc[m])),
+           "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "bytes_per_history": bph},
+           "cpu_baseline": None}
+    ctx.close()
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2")
-    ap.add_argument("--form", default="history", choices=("event", "history"))
+    ap.add_argument("--config", default="c3", choices=("c1", "c2", "c3"))
+    ap.add_argument("--form", default="event", choices=("event", "history"))
     ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64"))
-    ap.add_argument("--nper", type=int, default=0, help="histories per step (default: config's)")
+    ap.add_argument("--group", type=int, default=0, help="beamlets per group (0: by HBM budget)")
+    ap.add_argument("--target", type=int, default=0, help="event form: particles in flight (0: automatic)")
+    ap.add_argument("--budget", type=int, default=0, help="event form: electron steps per event (0: automatic)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              "(histories shard across GPUs by batch, see the design notes)", file=sys.stderr)
+              "(beamlets and batches would shard across GPUs with no reduction; see the design notes)",
+              file=sys.stderr)
         sys.exit(2)
-    import ompmc_amd as omc
-    cfg = build_config(a.config)
-    nper = a.nper or cfg["nper"]
-    prec = omc.FP32 if a.precision == "fp32" else omc.FP64
-    form = omc.FORM_EVENT if a.form == "event" else omc.FORM_HISTORY
-    ph = cfg["ph"]
-    ctx = omc.Context(ph.media)
-    gph = ctx.phantom(ph, ecut=0.7, pcut=0.01)
-    kw = dict(collimator=cfg["collimator"], precision=prec, form=form)
-    # warmup (module load, first-touch) and event counts for the roofline
-    _, _, stc = ctx.run_dosxyz(gph, cfg["spec"], max(10000, nper // 10), max(2, a.warmup), seed=7, count_events=True, **kw)
-    bytes_per_hist, n_ev = algorithmic_bytes(stc)
-    t0 = time.perf_counter()
-    s1, s2, st = ctx.run_dosxyz(gph, cfg["spec"], nper, max(2, a.steps), seed=1802, **kw)
-    wall = time.perf_counter() - t0
-    steps = max(2, a.steps)
-    hist = st.histories
-    value = hist / wall
-    kernel_rate = hist / st.gpu_seconds
-    achieved = bytes_per_hist * kernel_rate / 1e9
-    # statistical quality of the timed run (voxels above 50% of the maximum)
-    dose, unc = ctx.dose(gph, s1, s2, nper, steps)
-    m = dose > 0.5 * dose.max()
-    unc50 = float(np.mean(unc[m])) if m.any() else float("nan")
-    cpu = None
-    if not a.no_cpu:
-        cpu = cpu_baseline_reference(cfg) or cpu_baseline_port(cfg)
-    out = {
-        "metric": "histories/s (transport, one GPU)",
-        "value": value,
-        "unit": "histories/s",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": a.warmup,
-        "ms_per_step": 1e3 * wall / steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 transport, f64 tallies" if prec == omc.FP32 else "f64",
-        "data": "synthetic phantom and spectrum (seeded)",
-        "config": {"workload": cfg["workload"], "histories_per_step": nper, "form": a.form,
-                   "precision": a.precision},
-        "kernel_histories_per_s": kernel_rate,
-        "mean_rel_unc_gt50": unc50,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_history": bytes_per_hist, "events_per_history": n_ev},
-        "cpu_baseline": cpu,
-    }
+    out = bench_c3(a) if a.config == "c3" else bench_dose(a)
     print(json.dumps(out))
-    ctx.close()
 
 
 if __name__ == "__main__":

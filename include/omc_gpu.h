@@ -111,7 +111,8 @@ typedef struct omc_run_opts {
     const int64_t *replay_offsets;/* first draw of each history, nhist+1 entries */
     int64_t replay_len;
     int grid, block;        /* 0 = automatic */
-    int count_events;       /* fill the event counters and per-kernel times of omc_run_stats */
+    int count_events;       /* 1: fill the event counters and per-kernel times of omc_run_stats;
+                               2: per-kernel times only (events around every transport kernel) */
     int64_t event_target;   /* event form: particles in flight per round (0 = automatic) */
     int electron_budget;    /* event form: condensed-history steps per charged-particle event (0 = automatic) */
     int group;              /* Dij: beamlets transported together (0 = as many as hbm_budget allows,

@@ -183,7 +183,8 @@ def run_opts(precision=FP32, form=FORM_EVENT, seed=1802, rng=RNG_PHILOX, replay=
     o = RunOpts()
     lib().omc_run_opts_default(C.byref(o))
     o.precision, o.form, o.seed, o.rng, o.nsplit = precision, form, seed, rng, nsplit
-    o.grid, o.block, o.count_events = grid, block, int(bool(count_events))
+    o.grid, o.block = grid, block
+    o.count_events = 2 if count_events == "time" else int(bool(count_events))
     o.event_target, o.electron_budget, o.group, o.hbm_budget = event_target, electron_budget, group, hbm_budget
     keep = []
     if replay is not None:

@@ -364,6 +364,7 @@ static int run_dosxyz_history(omc_ctx *c, hipStream_t stream, PhantomDev *pd, co
     uint32_t *d_rn = nullptr;
     long *d_off = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> kev;
     float ms = 0.f;
     auto wall0 = std::chrono::steady_clock::now();
     if (!A.ok) { rc = omc_seterr(OMC_ERR_NOMEM, "device allocation failed"); goto done; }
@@ -391,6 +392,10 @@ static int run_dosxyz_history(omc_ctx *c, hipStream_t stream, PhantomDev *pd, co
     a.chunk = 1;
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
+    if (o->count_events) {
+        kev.assign(2 * nbatch, nullptr);
+        for (auto &e : kev) HIPCHK(hipEventCreate(&e));
+    }
     HIPCHK(hipEventRecord(ev0, stream));
     for (int b = 0; b < nbatch; b++) {
         a.nhist = nper;
@@ -400,8 +405,10 @@ static int run_dosxyz_history(omc_ctx *c, hipStream_t stream, PhantomDev *pd, co
         int block = o->block > 0 ? o->block : 256;
         long want = (nper + block - 1) / block;
         int grid = o->grid > 0 ? o->grid : (int)(want < 2048 ? want : 2048);
-        HIPCHK(launch_history<R>(a, o->rng == OMC_RNG_REPLAY ? RNG_REPLAY : RNG_PHILOX, o->count_events != 0, grid,
-                                 block, stream));
+        if (o->count_events) HIPCHK(hipEventRecord(kev[2 * b], stream));
+        HIPCHK(launch_history<R>(a, o->rng == OMC_RNG_REPLAY ? RNG_REPLAY : RNG_PHILOX, (o->count_events & 1) != 0,
+                                 grid, block, stream));
+        if (o->count_events) HIPCHK(hipEventRecord(kev[2 * b + 1], stream));
         HIPCHK(launch_accumulate(d_e, d_s, d_s2, nreg, 1.0, stream));
     }
     HIPCHK(hipEventRecord(ev1, stream));
@@ -419,13 +426,21 @@ static int run_dosxyz_history(omc_ctx *c, hipStream_t stream, PhantomDev *pd, co
         st->n_howfar = cnt[0]; st->n_hownear = cnt[1]; st->n_cross = cnt[2]; st->n_dep = cnt[3];
         st->n_rn = cnt[4]; st->n_photon = cnt[5]; st->n_electron = cnt[6];
         st->errors = (err[0] ? 1 : 0) | (err[1] ? 2 : 0) | (err[2] ? 4 : 0);
-        st->kernel_seconds[OMC_K_HISTORY] = ms * 1e-3;
+        double tk = 0;
+        for (int b = 0; b < nbatch && o->count_events; b++) {
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, kev[2 * b], kev[2 * b + 1]));
+            tk += t * 1e-3;
+        }
+        st->kernel_seconds[OMC_K_HISTORY] = o->count_events ? tk : ms * 1e-3;
         st->kernel_launches[OMC_K_HISTORY] = (uint64_t)nbatch;
         if (err[0]) rc = omc_seterr(OMC_ERR_STACK, "particle stack overflow on the device");
     }
 done:
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (auto e : kev)
+        if (e) (void)hipEventDestroy(e);
     A.release();
     if (st) st->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
     return rc;

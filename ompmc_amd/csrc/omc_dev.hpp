@@ -57,6 +57,10 @@ template <> struct M<double> {
     // large share of cases, and the FP64 path must round as the reference does
     static __device__ __forceinline__ double pow3(double x) { return ::pow(x, 3.0); }
     static __device__ __forceinline__ double pow4(double x) { return ::pow(x, 4.0); }
+    // remaining path (cm at the medium's nominal density) below which an
+    // electron step counts as having reached its interaction site: the
+    // reference's 1e-9 (src/ompmc.c:5355)
+    static __device__ __forceinline__ double reach_tol(double) { return 1.0E-9; }
 };
 template <> struct M<float> {
     static __device__ __forceinline__ float log(float x) { return ::logf(x); }
@@ -69,6 +73,12 @@ template <> struct M<float> {
     static __device__ __forceinline__ float fabs(float a) { return ::fabsf(a); }
     static __device__ __forceinline__ float pow3(float x) { return x * x * x; }
     static __device__ __forceinline__ float pow4(float x) { float y = x * x; return y * y; }
+    // In float the path to the interaction, t, goes through t/rhof*rhof on a
+    // step that reaches the site and keeps a residual of up to two roundings
+    // (about 1.2e-7 t), far above the reference's absolute 1e-9 whenever the
+    // voxel density differs from the nominal one; below 4 times that residual
+    // the site counts as reached, as it does exactly in FP64.
+    static __device__ __forceinline__ float reach_tol(float t) { return ::fmaxf(1.0E-9f, 5.0E-7f * t); }
 };
 
 // ------------------------------------------------------------------ tables
@@ -1443,8 +1453,9 @@ template <class R, class G, class T> struct Engine {
                 if (eie <= ecut_of(imed)) return electron_stop(p, q, iq, eie, false);
                 if (imed != medold) break;
                 demfp -= de * sig0;
+                R tt0 = total_tstep;
                 total_tstep -= tvstep * rhof;
-                if (total_tstep < (R)1.0E-9) demfp = (R)0.0;
+                if (total_tstep < M<R>::reach_tol(tt0)) demfp = (R)0.0;
                 if (demfp >= (R)1.0E-5 && --budget <= 0) {
                     es.demfp = demfp;
                     es.sig0 = sig0;

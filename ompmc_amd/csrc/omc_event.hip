@@ -102,55 +102,61 @@ struct JobStats {
 };
 
 // Runs one job (nhist primaries) to completion.  Rounds are enqueued in
-// chunks; the host reads a snapshot of the control block after each chunk
-// while the next chunk runs.  With `timing`, every round is bracketed by
-// events per kernel class instead (slower; for profiling runs).
+// chunks of K; the host reads a snapshot of the control block after each
+// chunk while the next chunk runs, so the device never waits for the host.
+// With `timing`, every kernel of every round is bracketed by events on the
+// job's stream and its duration is added to the job statistics when the
+// chunk has completed.
 template <class R>
 int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int dest, bool count, bool timing,
             JobStats &js) {
+    constexpr int K = 4;
     CK(launch_job_start(S.ctl, nhist, S.target, S.cap, dest, st));
     Pinned snapbuf(2 * sizeof(EvCtl));
     if (!snapbuf.p) return omc_seterr(OMC_ERR_NOMEM, "pinned host buffer");
     EvCtl *snap = (EvCtl *)snapbuf.p;
-    const EvCtl *done = nullptr;
-    if (timing) {
-        Events ev(5);
-        for (long r = 0;; r++) {
-            CK(hipEventRecord(ev.ev[0], st));
+    Events chunk_done(2);
+    Events kev(timing ? 2 * K * 5 : 0);
+    auto enqueue = [&](long r0, int slot) -> hipError_t {
+        for (int j = 0; j < K; j++) {
+            hipEvent_t *e = timing ? &kev.ev[(size_t)(slot * K + j) * 5] : nullptr;
+            hipError_t x;
+            if (e && (x = hipEventRecord(e[0], st)) != hipSuccess) return x;
             for (int part = 0; part < 4; part++) {
-                CK(launch_round_part(L, r, part, count, st));
-                CK(hipEventRecord(ev.ev[part + 1], st));
+                if ((x = launch_round_part(L, r0 + j, part, count, st)) != hipSuccess) return x;
+                if (e && (x = hipEventRecord(e[part + 1], st)) != hipSuccess) return x;
             }
-            CK(hipMemcpyAsync(&snap[0], S.ctl, sizeof(EvCtl), hipMemcpyDeviceToHost, st));
-            CK(hipStreamSynchronize(st));
-            js.t_source += ev.ms(0, 1) * 1e-3;
-            js.t_photon += ev.ms(1, 2) * 1e-3;
-            js.t_charged += ev.ms(2, 3) * 1e-3;
-            js.t_other += ev.ms(3, 4) * 1e-3;
-            js.n_source++; js.n_photon++; js.n_charged++; js.n_other++;
-            if (snap[0].err || snap[0].done_round >= 0 || r > MAX_ROUNDS) { done = &snap[0]; break; }
         }
-    } else {
-        Events ev(2);
-        const int K = 4;
-        long r = 0;
-        int cur = 0;
-        CK(launch_rounds(L, r, K, count, st));
-        CK(hipMemcpyAsync(&snap[0], S.ctl, sizeof(EvCtl), hipMemcpyDeviceToHost, st));
-        CK(hipEventRecord(ev.ev[0], st));
+        return hipMemcpyAsync(&snap[slot], S.ctl, sizeof(EvCtl), hipMemcpyDeviceToHost, st) == hipSuccess
+                   ? hipEventRecord(chunk_done.ev[slot], st)
+                   : hipErrorUnknown;
+    };
+    auto harvest = [&](int slot) {
+        if (!timing) return;
+        for (int j = 0; j < K; j++) {
+            int b = (slot * K + j) * 5;
+            js.t_source += kev.ms(b, b + 1) * 1e-3;
+            js.t_photon += kev.ms(b + 1, b + 2) * 1e-3;
+            js.t_charged += kev.ms(b + 2, b + 3) * 1e-3;
+            js.t_other += kev.ms(b + 3, b + 4) * 1e-3;
+        }
+        js.n_source += K; js.n_photon += K; js.n_charged += K; js.n_other += K;
+    };
+    long r = 0;
+    int cur = 0;
+    CK(enqueue(r, 0));
+    r += K;
+    for (;;) {
+        CK(enqueue(r, 1 - cur));
         r += K;
-        for (;;) {
-            CK(launch_rounds(L, r, K, count, st));
-            CK(hipMemcpyAsync(&snap[1 - cur], S.ctl, sizeof(EvCtl), hipMemcpyDeviceToHost, st));
-            CK(hipEventRecord(ev.ev[1 - cur], st));
-            r += K;
-            CK(hipEventSynchronize(ev.ev[cur]));
-            if (snap[cur].err || snap[cur].done_round >= 0 || r > MAX_ROUNDS) break;
-            cur = 1 - cur;
-        }
-        CK(hipStreamSynchronize(st));
-        done = &snap[1 - cur];   // the last snapshot, taken after every enqueued round
+        CK(hipEventSynchronize(chunk_done.ev[cur]));
+        harvest(cur);
+        if (snap[cur].err || snap[cur].done_round >= 0 || r > MAX_ROUNDS) break;
+        cur = 1 - cur;
     }
+    CK(hipStreamSynchronize(st));
+    harvest(1 - cur);
+    const EvCtl *done = &snap[1 - cur];   // the last snapshot, taken after every enqueued round
     js.ev_photon += done->events[0];
     js.ev_charged += done->events[1];
     js.rounds += (uint64_t)(done->done_round >= 0 ? done->done_round : done->rounds);
@@ -249,7 +255,7 @@ int omc_run_dosxyz_event(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const o
     CK(hipEventRecord(ev.ev[0], stream));
     for (int b = 0; b < nbatch; b++) {
         L.src.hist_base = (long)b * nper;
-        rc = run_job(stream, S, L, nper, dest, o->count_events != 0, o->count_events != 0, js);
+        rc = run_job(stream, S, L, nper, dest, (o->count_events & 1) != 0, o->count_events != 0, js);
         if (rc) return rc;
         CK(launch_accumulate(d_e, d_s, d_s2, nreg, 1.0, stream));
     }
@@ -352,7 +358,7 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
     const int nbix = bix->nbixels;
     const long nreg = pd->nreg, nvox = nreg - 1;
     const bool event = o->form == OMC_FORM_EVENT;
-    const bool count = o->count_events != 0;
+    const bool count = (o->count_events & 1) != 0, timing = o->count_events != 0;
     std::vector<long> nper = beamlet_nper(bix, nbatch);
     for (long n : nper)
         if (n >= (1L << 32)) return omc_seterr(OMC_ERR_LIMIT, "more than 2^32 histories per batch of a beamlet");
@@ -482,7 +488,7 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
             if (event) {
                 L.src.nslot = ng;
                 L.src.batch = bt;
-                rc = run_job(stream, S, L, njob, 0, count, count, js);
+                rc = run_job(stream, S, L, njob, 0, count, timing, js);
                 if (rc) return rc;
             } else {
                 H.nslot = ng;
@@ -505,9 +511,9 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
                 int block = o->block > 0 ? o->block : 256;
                 long want = (njob + block - 1) / block;
                 int grid = o->grid > 0 ? o->grid : (int)std::min<long>(want, 2048);
-                if (count) CK(hipEventRecord(ev.ev[2], stream));
+                if (timing) CK(hipEventRecord(ev.ev[2], stream));
                 CK(launch_history<R>(H, replay ? RNG_REPLAY : RNG_PHILOX, count, grid, block, stream));
-                if (count) {
+                if (timing) {
                     CK(hipEventRecord(ev.ev[3], stream));
                     CK(hipEventSynchronize(ev.ev[3]));
                     t_hist += ev.ms(2, 3) * 1e-3;
@@ -517,7 +523,7 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
             CK(launch_accum_slots(d_e, d_s, d_s2, nreg, nreg, ng, d_nper, stream));
             histories += (double)njob;
         }
-        if (count) CK(hipEventRecord(ev.ev[2], stream));
+        if (timing) CK(hipEventRecord(ev.ev[2], stream));
         int nb = 0;
         CK(launch_dij_finish(d_s, d_s2, d_carry, ng, nreg, nvox, pd->dmass, pd->drho, nbatch, d_bmax, d_tdose, d_tvar,
                              &nb, stream));
@@ -553,7 +559,7 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
             if (var)
                 CK(hipMemcpyAsync(out.var.data() + base, d_vval, sizeof(double) * tot, hipMemcpyDeviceToHost, stream));
         }
-        if (count) {
+        if (timing) {
             CK(hipEventRecord(ev.ev[3], stream));
             CK(hipEventSynchronize(ev.ev[3]));
             t_other += ev.ms(2, 3) * 1e-3;

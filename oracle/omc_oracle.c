@@ -1833,3 +1833,50 @@ int orc_run_matrad(const omc_tables *t, const omc_regions *r, const orc_geom *g,
     orc_destroy(o);
     return OMC_OK;
 }
+
+/* OpenMP matRad Dij transport as the reference runs it (omc_matrad.c:1576-1600):
+ * for each beamlet and batch, the histories of the batch are shared by the
+ * team (thread k seeds (ixx, jxx+k)), then the batch sums are folded in.
+ * Thread-private tallies replace the reference's atomic updates.  Used for
+ * CPU timing only; returns the transport seconds. */
+double orc_run_matrad_omp(const omc_tables *t, const omc_regions *r, const orc_geom *g, const omc_spectrum *spec,
+                          const orc_beamlets *b, int nsplit, int ixx, int jxx, const int *nper, int nbatch,
+                          int nthreads, double *sum, double *sum2) {
+    double t0 = 0.0, t1 = 0.0;
+#ifdef _OPENMP
+    int nreg = r->nreg;
+    if (nthreads < 1) nthreads = omp_get_num_procs();
+    orc **ctx = calloc(nthreads, sizeof(orc *));
+    for (int k = 0; k < nthreads; k++) ctx[k] = orc_create(t, r, g, nsplit, ixx, jxx + k);
+    t0 = omp_get_wtime();
+    for (int ib = 0; ib < b->n; ib++) {
+        double *sb = sum + (long)ib * nreg, *s2b = sum2 + (long)ib * nreg;
+        for (int bt = 0; bt < nbatch; bt++) {
+#pragma omp parallel for schedule(guided) num_threads(nthreads)
+            for (int i = 0; i < nper[ib]; i++) {
+                orc *o = ctx[omp_get_thread_num()];
+                orc_init_history_matrad(o, spec, b, ib);
+                orc_shower(o);
+            }
+#pragma omp parallel for num_threads(nthreads)
+            for (int i = 0; i < nreg; i++) {
+                double e = 0.0;
+                for (int k = 0; k < nthreads; k++) {
+                    e += ctx[k]->endep[i];
+                    ctx[k]->endep[i] = 0.0;
+                }
+                e /= (double)nper[ib];
+                sb[i] += e;
+                s2b[i] += e * e;
+            }
+        }
+    }
+    t1 = omp_get_wtime();
+    for (int k = 0; k < nthreads; k++) orc_destroy(ctx[k]);
+    free(ctx);
+#else
+    (void)t; (void)r; (void)g; (void)spec; (void)b; (void)nsplit; (void)ixx; (void)jxx; (void)nper; (void)nbatch;
+    (void)nthreads; (void)sum; (void)sum2;
+#endif
+    return t1 - t0;
+}

@@ -113,6 +113,10 @@ int orc_run_matrad_beamlet(orc *o, const omc_spectrum *spec, const orc_beamlets 
 int orc_run_matrad(const omc_tables *t, const omc_regions *r, const orc_geom *g, const omc_spectrum *spec,
                    const orc_beamlets *b, int nsplit, int ixx, int jxx, const int *nper, int nbatch, double *sum,
                    double *sum2, int *rn_per_history);
+/* OpenMP variant for CPU timing (thread k seeds (ixx, jxx+k)); returns seconds. */
+double orc_run_matrad_omp(const omc_tables *t, const omc_regions *r, const orc_geom *g, const omc_spectrum *spec,
+                          const orc_beamlets *b, int nsplit, int ixx, int jxx, const int *nper, int nbatch,
+                          int nthreads, double *sum, double *sum2);
 
 #ifdef __cplusplus
 }

@@ -285,3 +285,23 @@ void orc_session_matrad_finish(orc_session *s, const double *sum, const double *
                                double *var) {
     omc_matrad_finish(&s->ph, sum, sum2, nbatch, dose, var);
 }
+
+/* OpenMP matRad transport for CPU timing; returns the transport seconds. */
+double orc_session_matrad_omp(orc_session *s, int spectrum, double energy, const double *cdf1, const double *cdf2,
+                              int nbix, const int *ibeam, const double *src, const double *corner,
+                              const double *side1, const double *side2, int gaussian, double sigma, const int *nper,
+                              int nbatch, int nthreads, double *sum, double *sum2) {
+    omc_spectrum sp;
+    memset(&sp, 0, sizeof sp);
+    if (spectrum) {
+        sp.spectrum = 1;
+        sp.deltak = OMC_INVDIM;
+        memcpy(sp.cdfinv1, cdf1, sizeof sp.cdfinv1);
+        memcpy(sp.cdfinv2, cdf2, sizeof sp.cdfinv2);
+    } else {
+        omc_spectrum_mono(&sp, energy);
+    }
+    orc_beamlets bl = {nbix, ibeam, src, corner, side1, side2, gaussian, sigma};
+    return orc_run_matrad_omp(&s->t, &s->r, &s->g, &sp, &bl, s->nsplit, s->ixx, s->jxx, nper, nbatch, nthreads, sum,
+                              sum2);
+}

@@ -64,15 +64,21 @@ def test_event_fp64_agrees_with_oracle(tmp_path, name):
     c = omc.Context(ph_h.media)
     ph = c.phantom(ph_h, ecut=0.7, pcut=0.01)
     g1, g2, st = _run(c, ph, name, nh, nb, precision=omc.FP64, form=omc.FORM_EVENT, seed=11)
-    c.close()
     assert st.errors == 0
     f2, f3, n = _agreement(s1, s2, g1, g2, nb)
     print(f"{name}: {n} voxels, {f2:.3f} within 2 sigma, {f3:.3f} within 3 sigma")
     assert n > 20
     assert f2 >= 0.88 and f3 >= 0.97, (f2, f3)
-    # total energy deposited in the phantom: within 4 sigma of the batch spread
-    ta, tb = s1[1:].sum(), g1[1:].sum()
-    assert abs(ta - tb) <= 0.02 * ta
+    # total energy deposited in the phantom: the spread of the GPU total over
+    # independent seeds gives its standard deviation for nh histories
+    tots = [g1[1:].sum()]
+    for k in range(7):
+        tots.append(_run(c, ph, name, nh, nb, precision=omc.FP64, form=omc.FORM_EVENT, seed=100 + k)[0][1:].sum())
+    c.close()
+    sd = float(np.std(tots, ddof=1))
+    ta = s1[1:].sum()
+    print(f"{name}: oracle total {ta:.1f}, GPU {np.mean(tots):.1f} +- {sd:.1f} per run")
+    assert abs(ta - np.mean(tots)) <= 4.0 * sd * np.sqrt(1 + 1 / len(tots))
 
 
 def test_event_result_independent_of_schedule(slab):
@@ -98,20 +104,36 @@ def test_event_matches_history_form_fp32(slab):
     assert f2 >= 0.88 and f3 >= 0.97
 
 
-def test_fp32_dose_within_half_percent_of_fp64(slab):
-    """FP32 transport against the FP64 GPU run: the dose summed over voxels
-    above 50% of the maximum differs by less than 0.5% (the statistical
-    uncertainty of that sum is about 0.1% here)."""
-    c, ph = slab
-    nh, nb = 400000, 20
-    a1, a2, _ = _run(c, ph, "slab", nh, nb, precision=omc.FP32, form=omc.FORM_EVENT, seed=21)
-    b1, b2, _ = _run(c, ph, "slab", nh, nb, precision=omc.FP64, form=omc.FORM_EVENT, seed=22)
-    ma, _ = _batch_stats(a1[1:], a2[1:], nb)
-    mb, _ = _batch_stats(b1[1:], b2[1:], nb)
-    sel = mb > 0.5 * mb.max()
-    rel = abs(ma[sel].sum() / mb[sel].sum() - 1.0)
-    print(f"FP32 vs FP64 dose (voxels > 50% max): {rel:.2e}")
-    assert rel < 5e-3
+@pytest.fixture(scope="module")
+def ct():
+    """Synthetic CT: every voxel's density differs from its medium's nominal
+    one, so FP32 rounding of density-scaled path lengths is exercised."""
+    ph_h = P.synthetic_ct(nx=64, ny=64, nz=48, vx=0.3, vy=0.3, vz=0.3, seed=9, target_radius=2.0)
+    c = omc.Context(ph_h.media)
+    ph = c.phantom(ph_h, ecut=0.7, pcut=0.01)
+    yield c, ph
+    c.close()
+
+
+def test_fp32_dose_within_half_percent_of_fp64(ct):
+    """FP32 transport against the FP64 GPU run on the synthetic CT: the dose
+    summed over voxels above 50% of the maximum (region fixed by an
+    independent pilot run) differs by less than 0.5%; eight independent seeds
+    per precision put the statistical uncertainty of the difference near 0.15%."""
+    c, ph = ct
+    spec = omc.Spectrum.read(omc.os.path.join(omc.DATA, "spectra", "mohan6.spectrum"))
+
+    def run(prec, seed, n=200000):
+        return c.run_dosxyz(ph, spec, n // 4, 4, collimator=(-3, 3, -3, 3), precision=prec, form=omc.FORM_EVENT,
+                            seed=seed)[0][1:]
+    pilot = run(omc.FP64, 999)
+    sel = pilot > 0.5 * pilot.max()
+    reg = {prec: np.array([run(prec, 200 + k)[sel].sum() for k in range(8)]) for prec in (omc.FP32, omc.FP64)}
+    a, b = reg[omc.FP32], reg[omc.FP64]
+    rel = a.mean() / b.mean() - 1.0
+    sem = np.hypot(a.std(ddof=1), b.std(ddof=1)) / np.sqrt(8) / b.mean()
+    print(f"FP32 vs FP64 dose (voxels > 50% max): {rel:+.3e} +- {sem:.1e}")
+    assert abs(rel) < 5e-3
 
 
 def test_event_counts_and_kernel_times(slab):
