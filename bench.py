@@ -11,11 +11,13 @@ phantom and tables resident in HBM.  Beside it the line reports:
 * the plain GPU history-by-history form running the reference's loop order
   (one beamlet at a time) on every 20th beamlet, as histories/s;
 * the plan-dose statistical uncertainty in voxels above 50% of the maximum;
-* the FP32/FP64 plan-dose difference on every 5th beamlet;
+* the FP32/FP64 plan-dose difference on every 10th beamlet (three seeds
+  per precision, region fixed by an independent FP64 pilot run);
 * p50/p99 latency and throughput of a short served request mix;
 * the roofline of the dominant transport kernel (algorithmic bytes from
   on-device event counters over its duration, timed with HIP events on its
-  stream inside the timed steps);
+  stream inside the timed steps) and its measured HBM traffic per launch
+  (from the committed rocprofv3 PMC summary under profiles/);
 * a CPU baseline: the oracle restatement (OpenMP, the reference's loop) on a
   sample of the same beamlets.
 
@@ -170,6 +172,8 @@ def bench_c3(a):
     nbatch, rel = 10, 1e-4
     t_setup = time.perf_counter()
     ph, bix, spec, _ = c3_config()
+    if a.subset > 1:
+        bix = subset(bix, a.subset)
     ctx = omc.Context(ph.media)
     gph = ctx.phantom(ph, ecut=0.7, pcut=0.01)
     log(f"c3: {bix.n} beamlets, {int(np.sum(np.maximum(bix.nhist // nbatch, 1)) * nbatch)} histories per step, "
@@ -211,7 +215,8 @@ def bench_c3(a):
         dom = 0
         bph = total_bytes(stc) / stc.histories
     achieved = bph * hist / ksec[dom] / 1e9
-    traffic = read_traffic(dom)
+    traffic = read_traffic(dom, hist, klaunch[dom]) if (a.config == "c3" and form == omc.FORM_EVENT
+                                                        and prec == omc.FP32) else None
     out = {
         "metric": "histories/s + p50/p99 request latency vs plain GPU form at <=1% dose unc.",
         "value": value,
@@ -248,14 +253,19 @@ def bench_c3(a):
         out["speedup_vs_plain"] = value / out["plain_form"]["histories_per_s"]
         log(f"plain: {out['plain_form']['histories_per_s']:.4g} histories/s")
     if not a.no_fp64 and prec == omc.FP32:
-        sub = subset(bix, 5)
+        # FP32 against FP64 on every 10th beamlet: three independent seeds per
+        # precision; region (voxels above 20% of the plan maximum) fixed by an
+        # FP64 pilot run with its own seed
+        sub = subset(bix, 10)
         kw = dict(nbatch=nbatch, rel_threshold=rel, want_plan=True, form=form, group=a.group)
-        p32 = ctx.run_dij(gph, spec, sub, precision=omc.FP32, seed=77, **kw)[2]
-        p64 = ctx.run_dij(gph, spec, sub, precision=omc.FP64, seed=78, **kw)[2]
-        s = p64[0] > 0.5 * p64[0].max()
-        d = p32[0][s].sum() / p64[0][s].sum() - 1.0
-        sig = math.sqrt(p32[1][s].sum() + p64[1][s].sum()) / p64[0][s].sum()
-        out["fp64_plan_dose_rel_diff_gt50"] = {"value": d, "sigma": sig, "beamlets": int(sub.n)}
+        pilot = ctx.run_dij(gph, spec, sub, precision=omc.FP64, seed=70, **kw)[2][0]
+        reg = pilot > 0.2 * pilot.max()
+        sums = {p: np.array([ctx.run_dij(gph, spec, sub, precision=p, seed=71 + 3 * i + (p == omc.FP64), **kw)[2][0][reg].sum()
+                             for i in range(3)]) for p in (omc.FP32, omc.FP64)}
+        d = sums[omc.FP32].mean() / sums[omc.FP64].mean() - 1.0
+        sig = math.hypot(sums[omc.FP32].std(ddof=1), sums[omc.FP64].std(ddof=1)) / math.sqrt(3) / sums[omc.FP64].mean()
+        out["fp64_plan_dose_rel_diff"] = {"value": d, "sigma": sig, "beamlets": int(sub.n), "region": "> 20% of max",
+                                          "seeds": 3}
         log(f"fp32/fp64: {d:+.3e} +- {sig:.1e}")
     if not a.no_serve:
         out["serving"] = serve_mix(ctx, value, precision=prec)
@@ -271,15 +281,21 @@ def bench_c3(a):
     return out
 
 
-def read_traffic(kernel_class):
-    """HBM bytes per launch of the dominant kernel from the committed PMC
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel), if present."""
-    path = os.path.join(REPO, "profiles", "traffic.json")
+def read_traffic(kernel_class, histories, launches):
+    """HBM bytes per launch of a kernel from the committed PMC summary:
+    (FETCH_SIZE + WRITE_SIZE) per history, each scaled by what the counter
+    reads for a copy of known size, times the histories of the timed steps
+    over their launches.  None when no summary covers the kernel."""
+    path = os.path.join(REPO, "profiles", "c3_event_fp32_pmc_traffic.md")
     try:
-        with open(path) as f:
-            return json.load(f).get(KERNEL_NAMES[kernel_class])
-    except (OSError, ValueError):
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        t = json.loads(block).get(KERNEL_NAMES[kernel_class])
+    except (OSError, IndexError, ValueError):
         return None
+    if not t or launches <= 0:
+        return None
+    return (t["read_bytes_per_history"] + t["write_bytes_per_history"]) * histories / launches
 
 
 # ------------------------------------------------------------------ c1 / c2 (single dose runs)
@@ -339,7 +355,19 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--subset", type=int, default=1, help="c3: every n-th beamlet only (profiling runs)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device once (PMC scale of FETCH_SIZE / WRITE_SIZE) and exit")
     a = ap.parse_args()
+    if a.calibrate_copy:
+        import torch
+        x = torch.ones(a.calibrate_copy << 18, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.calibrate_copy << 20, "copy_bytes_written": a.calibrate_copy << 20}))
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(beamlets and batches would shard across GPUs with no reduction; see the design notes)",

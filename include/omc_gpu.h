@@ -92,6 +92,12 @@ typedef struct omc_spectrum_desc {
     const double *cdfinv1, *cdfinv2;
 } omc_spectrum_desc;
 
+/* Spectrum file reader (omc_dosxyz.c:396-510; omc_matrad.c:627-769): fills
+ * cdfinv1/cdfinv2 (OMC_INVDIM = 1000 entries each) from a spectrum file of
+ * the reference's format (title line; nbins, emin, imode; upper edge and
+ * weight per bin). */
+int omc_spectrum_file(const char *path, double *cdfinv1, double *cdfinv2);
+
 /* omc_dosxyz collimated point source (omc_dosxyz.c:521-617, 964-1068). */
 typedef struct omc_dosxyz_source_desc {
     omc_spectrum_desc spec;

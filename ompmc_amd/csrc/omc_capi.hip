@@ -105,7 +105,7 @@ template <class R> static int upload_tables(omc_ctx *c, DevArena &A, Tables<R> &
     T.q2cp = pw(h.q2cp_ms0, h.q2cp_ms1, nek);
     T.range_ep = vec(h.range_ep, 2 * nek);
     T.e_array = vec(h.e_array, nek);
-    T.spin_rej = vec(h.spin_rej, (size_t)nm * OMC_NSPIN_MED);
+    T.spin_rej = vec(h.spin_rej, (size_t)nm * OMC_NSPIN_MED + 1);   // + the zero entry of quirk Q17
     T.ums = vec(h.ums, OMC_NMS);
     T.fms = vec(h.fms, OMC_NMS);
     T.wms = vec(h.wms, OMC_NMS);
@@ -139,6 +139,13 @@ extern "C" int omc_ctx_create(omc_ctx **out, int nmed, const char *const *media,
     }
     rc = upload_tables<float>(c, c->tabmem, c->t32);
     if (!rc) rc = upload_tables<double>(c, c->tabmem, c->t64);
+    if (!rc) {
+        // the descriptors themselves live in device memory: kernels take
+        // pointers to them (no per-lane copy of a by-value argument)
+        c->dt32 = c->tabmem.upload(std::vector<Tables<float>>{c->t32});
+        c->dt64 = c->tabmem.upload(std::vector<Tables<double>>{c->t64});
+        if (!c->tabmem.ok) rc = omc_seterr(OMC_ERR_NOMEM, "table descriptors");
+    }
     if (rc) { omc_ctx_destroy(c); return rc; }
     *out = c;
     return OMC_OK;
@@ -267,6 +274,8 @@ int omc_phantom_make(omc_ctx *c, const omc_phantom_desc *d, PhantomDev **out) {
     pd->dmed = pd->mem.upload(m8);
     fill_geom<float>(c, pd, pd->g32);
     fill_geom<double>(c, pd, pd->g64);
+    pd->dg32 = pd->mem.upload(std::vector<Geom<float>>{pd->g32});
+    pd->dg64 = pd->mem.upload(std::vector<Geom<double>>{pd->g64});
     {
         // voxel masses as the reference's accumulateResults forms them (omc_matrad.c:944-950)
         std::vector<double> mass(nvox), rho(ph.rho, ph.rho + nvox);
@@ -323,6 +332,16 @@ PhantomDev *omc_get_phantom(omc_ctx *c, int id) {
 }
 
 // ------------------------------------------------------------------ runs
+extern "C" int omc_spectrum_file(const char *path, double *cdfinv1, double *cdfinv2) {
+    if (!path || !cdfinv1 || !cdfinv2) return omc_seterr(OMC_ERR_ARG, "omc_spectrum_file: bad arguments");
+    omc_spectrum sp;
+    int rc = omc_spectrum_read(&sp, path);
+    if (rc) return rc;
+    memcpy(cdfinv1, sp.cdfinv1, sizeof(double) * OMC_INVDIM);
+    memcpy(cdfinv2, sp.cdfinv2, sizeof(double) * OMC_INVDIM);
+    return OMC_OK;
+}
+
 extern "C" void omc_run_opts_default(omc_run_opts *o) {
     memset(o, 0, sizeof *o);
     o->precision = OMC_FP32;

@@ -40,8 +40,10 @@ struct PhantomDev {
     const uint8_t *dmed = nullptr;
     const double *dmass = nullptr;   // voxel mass (g): the reference's dx*dy*dz*rho, for the Dij finish
     const double *drho = nullptr;
-    omc::Geom<float> g32;
+    omc::Geom<float> g32;            // host copies of the descriptors
     omc::Geom<double> g64;
+    const omc::Geom<float> *dg32 = nullptr;   // device copies passed to kernels
+    const omc::Geom<double> *dg64 = nullptr;
 };
 
 struct omc_ctx {
@@ -50,18 +52,21 @@ struct omc_ctx {
     omc_tables host;
     DevArena tabmem;
     DevArena scratch;
-    omc::Tables<float> t32;
+    omc::Tables<float> t32;          // host copies of the descriptors
     omc::Tables<double> t64;
+    const omc::Tables<float> *dt32 = nullptr;  // device copies passed to kernels
+    const omc::Tables<double> *dt64 = nullptr;
     std::map<int, PhantomDev *> phantoms;
     int next_id = 1;
 };
 
-template <class R> const omc::Tables<R> &tables_of(const omc_ctx *c);
-template <> inline const omc::Tables<float> &tables_of<float>(const omc_ctx *c) { return c->t32; }
-template <> inline const omc::Tables<double> &tables_of<double>(const omc_ctx *c) { return c->t64; }
-template <class R> const omc::Geom<R> &geom_of(const PhantomDev *p);
-template <> inline const omc::Geom<float> &geom_of<float>(const PhantomDev *p) { return p->g32; }
-template <> inline const omc::Geom<double> &geom_of<double>(const PhantomDev *p) { return p->g64; }
+// device-resident table and geometry descriptors of a precision
+template <class R> const omc::Tables<R> *tables_of(const omc_ctx *c);
+template <> inline const omc::Tables<float> *tables_of<float>(const omc_ctx *c) { return c->dt32; }
+template <> inline const omc::Tables<double> *tables_of<double>(const omc_ctx *c) { return c->dt64; }
+template <class R> const omc::Geom<R> *geom_of(const PhantomDev *p);
+template <> inline const omc::Geom<float> *geom_of<float>(const PhantomDev *p) { return p->dg32; }
+template <> inline const omc::Geom<double> *geom_of<double>(const PhantomDev *p) { return p->dg64; }
 
 int omc_seterr(int code, const char *fmt, ...);
 int omc_check_opts(const omc_run_opts *o);

@@ -161,7 +161,10 @@ struct Philox {
     }
     template <class R> __device__ __forceinline__ R next() {
         if (pos >= 4) round4();
-        return (R)(buf[pos++] >> 8) * (R)(1.0 / 16777216.0);
+        // selects instead of a dynamic index keep buf in registers
+        uint32_t v = pos == 0 ? buf[0] : pos == 1 ? buf[1] : pos == 2 ? buf[2] : buf[3];
+        pos++;
+        return (R)(v >> 8) * (R)(1.0 / 16777216.0);
     }
 };
 

@@ -59,7 +59,7 @@ int device_cus(int dev) {
 }
 
 // ------------------------------------------------------------------ event-form state
-constexpr int DEFAULT_BUDGET = 16;           // condensed-history steps per charged event
+constexpr int DEFAULT_BUDGET = 32;           // condensed-history steps per charged event (measured best on gfx950)
 constexpr long MAX_ROUNDS = 2000000;         // a job that has not drained by then is reported as a step-guard error
 
 template <class R> size_t pool_entry_bytes() { return sizeof(R) * 10 + sizeof(int) * 6 + 4 + 8; }
@@ -68,7 +68,8 @@ template <class R> size_t event_bytes(long long target) {
     return (size_t)4 * (size_t)(4 * target) * pool_entry_bytes<R>() + sizeof(EvCtl);
 }
 
-long long default_target(int dev) { return (long long)device_cus(dev) * 4096; }
+// particles in flight: 2048 per compute unit, 8 per lane at 4 waves per SIMD
+long long default_target(int dev) { return (long long)device_cus(dev) * 2048; }
 
 template <class R> struct EvState {
     DevArena A;
@@ -719,9 +720,13 @@ size_t request_bytes(omc_server *s, const omc_request &r) {
     long long tgt = default_target(s->ctx->device);
     size_t ev = r.precision == OMC_FP64 ? event_bytes<double>(tgt) : event_bytes<float>(tgt);
     if (r.kind == OMC_REQ_DOSE) return b + (size_t)3 * nreg * 8 + ev;
-    // Dij: the group is sized to what is left of the request's share
-    return b + dij_fixed_bytes(nreg, r.precision, OMC_FORM_EVENT, tgt) + (size_t)std::min(r.bix.nbixels, 8) *
-                                                                             dij_slot_bytes(nreg);
+    // Dij: as many beamlets per group as the request's share of the budget
+    // holds (at least one), so that each transport job is large
+    size_t fixed = b + dij_fixed_bytes(nreg, r.precision, OMC_FORM_EVENT, tgt);
+    size_t share = (size_t)s->budget / (size_t)s->max_inflight;
+    size_t slots = share > fixed ? (share - fixed) / (dij_slot_bytes(nreg) + 64) : 1;
+    slots = std::max<size_t>(1, std::min<size_t>(slots, (size_t)r.bix.nbixels));
+    return fixed + slots * (dij_slot_bytes(nreg) + 64);
 }
 
 void run_request(omc_server *s, omc_server::Req *q) {

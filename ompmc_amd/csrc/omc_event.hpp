@@ -74,8 +74,8 @@ struct EvCtl {
 };
 
 template <class R> struct EvArgs {
-    Tables<R> t;
-    Geom<R> g;
+    const Tables<R> *t;
+    const Geom<R> *g;
     uint64_t seed;
     Pool<R> in;                       // input pool of this kernel's type
     const unsigned long long *n_in;   // its length
@@ -151,8 +151,15 @@ __device__ __forceinline__ void flush_counters(unsigned long long *dst, const Co
 // One event per queued particle: PHOTON selects photon(), otherwise up to
 // a.budget condensed-history steps of electron_run().  Waves stride over the
 // queue, so any grid covers any queue length.
+// The charged-particle kernel needs about 250 VGPRs unconstrained (2 waves
+// per SIMD); capping it at 128 (4 waves) spills some state to scratch and
+// still runs faster, as measured on gfx950.
+#ifndef OMC_CHARGED_WAVES_PER_EU
+#define OMC_CHARGED_WAVES_PER_EU 4
+#endif
 template <class R, bool PHOTON, bool COUNT>
-__global__ void __launch_bounds__(256) k_event(EvArgs<R> a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PHOTON ? 1 : OMC_CHARGED_WAVES_PER_EU)))
+k_event(EvArgs<R> a) {
     const long n = (long)*a.n_in;
     const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
     const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -175,7 +182,7 @@ __global__ void __launch_bounds__(256) k_event(EvArgs<R> a) {
             rng.at(a.seed, stream, a.in.ctr[i]);
             TallyAtomic tally;
             tally.init(a.endep + (size_t)slot * a.slot_stride);
-            Engine<R, PhiloxAt, TallyAtomic> e{a.t, a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
+            Engine<R, PhiloxAt, TallyAtomic> e{*a.t, *a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
             if (PHOTON) {
                 d = e.photon(p, q);
             } else {
@@ -206,8 +213,8 @@ __global__ void __launch_bounds__(256) k_event(EvArgs<R> a) {
 // Primaries of the coming round: histories [inj_h0, inj_h0 + inj_n) of the
 // job, written to the input pool of their type from offset inj_at.
 template <class R> struct SrcArgs {
-    Tables<R> t;
-    Geom<R> g;
+    const Tables<R> *t;
+    const Geom<R> *g;
     SrcDosxyz<R> dos;
     SrcBeamlets<R> bix;
     int use_beamlets;
@@ -236,7 +243,7 @@ __global__ void __launch_bounds__(256) k_source(SrcArgs<R> a) {
         PhiloxAt rng;
         rng.at(a.seed, stream, 0);
         TallyNone tn;
-        Engine<R, PhiloxAt, TallyNone> e{a.t, a.g, rng, tn, {0, 0, 0, 0, 0, 0, 0}, false};
+        Engine<R, PhiloxAt, TallyNone> e{*a.t, *a.g, rng, tn, {0, 0, 0, 0, 0, 0, 0}, false};
         Part<R> p;
         if (a.use_beamlets) init_history(e, a.bix, beamlet, p);
         else init_history(e, a.dos, p);

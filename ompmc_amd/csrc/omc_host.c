@@ -1164,7 +1164,10 @@ static int alloc_tables(omc_tables *t) {
                      &t->q2cp_ms0, &t->q2cp_ms1, &t->e_array};
     for (size_t i = 0; i < sizeof ek / sizeof ek[0]; i++) *ek[i] = calloc(nm * MXEKE, sizeof(double));
     t->range_ep = calloc(2 * nm * MXEKE, sizeof(double));
-    t->spin_rej = calloc(nm * OMC_NSPIN_MED, sizeof(double));
+    /* one zero entry past the end: the rejection lookup interpolates with
+     * the next entry, which for cos(theta) = -1 on the last row of the last
+     * medium lies one past the table (reference quirk Q17) */
+    t->spin_rej = calloc(nm * OMC_NSPIN_MED + 1, sizeof(double));
     t->ums = calloc(OMC_NMS, sizeof(double));
     t->fms = calloc(OMC_NMS, sizeof(double));
     t->wms = calloc(OMC_NMS, sizeof(double));

@@ -140,8 +140,8 @@ constexpr int HIST_STACK = 40;        // deepest LIFO depth below the two regist
 enum RngMode { RNG_PHILOX = 0, RNG_REPLAY = 1 };
 
 template <class R> struct HistArgs {
-    Tables<R> t;
-    Geom<R> g;
+    const Tables<R> *t;          // device copies (global memory), shared by all lanes
+    const Geom<R> *g;
     SrcDosxyz<R> dos;
     SrcBeamlets<R> bix;
     int use_beamlets;            // 0: dosxyz source; 1: beamlet source
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(256) k_history(HistArgs<R> a) {
             make_rng<R, G>(rng, a, h, stream);
             TallyAtomic tally;
             tally.init(a.endep + (size_t)slot * a.slot_stride);
-            Engine<R, G, TallyAtomic> e{a.t, a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
+            Engine<R, G, TallyAtomic> e{*a.t, *a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
             Part<R> p, q;
             if (a.use_beamlets) init_history(e, a.bix, beamlet, p);
             else init_history(e, a.dos, p);
@@ -273,8 +273,8 @@ enum SampKind {
 };
 
 template <class R> struct SampArgs {
-    Tables<R> t;
-    Geom<R> g;
+    const Tables<R> *t;          // device copies (global memory), shared by all lanes
+    const Geom<R> *g;
     int ncase;
     const int *kind;
     const double *in;        // [ncase*in_stride]
@@ -324,30 +324,30 @@ __global__ void __launch_bounds__(64) k_sampler(SampArgs<R> a) {
     rng.base = a.rn; rng.pos = a.rn_off[c]; rng.end = a.rn_off[c + 1]; rng.err = a.err;
     TallyRec tally;
     tally.buf = out + 64; tally.cap = (a.out_stride - 64) / 2; tally.n = 0;
-    Engine<R, Replay, TallyRec> e{a.t, a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, true};
+    Engine<R, Replay, TallyRec> e{*a.t, *a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, true};
     Part<R> p, q;
     int kind = a.kind[c];
     int d = 0;
     out[0] = 0;
     switch (kind) {
-    case S_COMPTON: load_part(in, p, a.g); e.compton(p, q); d = 1; break;
-    case S_PAIR: load_part(in, p, a.g); e.pair((int)in[11], p, q); d = 1; break;
-    case S_PHOTO: load_part(in, p, a.g); e.photo(p); d = 0; break;
-    case S_RAYLEIGH: load_part(in, p, a.g); e.rayleigh(p, (int)in[11], (R)in[2], (R)in[12], (int)in[13]); d = 0; break;
-    case S_BREMS: load_part(in, p, a.g); e.brems(p, q); d = 1; break;
-    case S_MOLLER: load_part(in, p, a.g); d = e.moller(p, q); break;
-    case S_BHABHA: load_part(in, p, a.g); e.bhabha(p, q); d = 1; break;
-    case S_ANNIH: load_part(in, p, a.g); e.annih(p, q); d = 1; break;
-    case S_RANNIH: load_part(in, p, a.g); e.rannih(p, q); d = 1; break;
+    case S_COMPTON: load_part(in, p, *a.g); e.compton(p, q); d = 1; break;
+    case S_PAIR: load_part(in, p, *a.g); e.pair((int)in[11], p, q); d = 1; break;
+    case S_PHOTO: load_part(in, p, *a.g); e.photo(p); d = 0; break;
+    case S_RAYLEIGH: load_part(in, p, *a.g); e.rayleigh(p, (int)in[11], (R)in[2], (R)in[12], (int)in[13]); d = 0; break;
+    case S_BREMS: load_part(in, p, *a.g); e.brems(p, q); d = 1; break;
+    case S_MOLLER: load_part(in, p, *a.g); d = e.moller(p, q); break;
+    case S_BHABHA: load_part(in, p, *a.g); e.bhabha(p, q); d = 1; break;
+    case S_ANNIH: load_part(in, p, *a.g); e.annih(p, q); d = 1; break;
+    case S_RANNIH: load_part(in, p, *a.g); e.rannih(p, q); d = 1; break;
     case S_PHOTON:
         // the reference's photon() keeps transporting the same photon after a
         // Rayleigh scatter within one call
-        load_part(in, p, a.g);
+        load_part(in, p, *a.g);
         do { d = e.photon(p, q); } while (d == 0 && p.iq == 0);
         break;
-    case S_ELECTRON: load_part(in, p, a.g); d = e.electron(p, q); break;
+    case S_ELECTRON: load_part(in, p, *a.g); d = e.electron(p, q); break;
     case S_MSDIST: {
-        load_part(in + 6, p, a.g);
+        load_part(in + 6, p, *a.g);
         Part<R> f;
         R us = e.msdist(p, (int)in[0], (int)in[1], (R)in[2], (R)in[3], (R)in[4], (R)in[5], f);
         out[1] = us; out[2] = f.x; out[3] = f.y; out[4] = f.z; out[5] = f.u; out[6] = f.v; out[7] = f.w;
@@ -386,8 +386,8 @@ __global__ void __launch_bounds__(64) k_sampler(SampArgs<R> a) {
     // scalar cases; particles at out[1..11] and out[12..22]
     out[0] = d;
     if (d <= 1) {
-        if (d >= 0) store_part(out + 1, p, a.g);
-        if (d == 1) store_part(out + 12, q, a.g);
+        if (d >= 0) store_part(out + 1, p, *a.g);
+        if (d == 1) store_part(out + 12, q, *a.g);
     }
     out[23] = tally.n;
     out[24] = (double)e.cnt.nhowfar;

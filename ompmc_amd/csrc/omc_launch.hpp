@@ -15,8 +15,8 @@ hipError_t launch_accumulate(double *e, double *sum, double *sum2, long n, doubl
 
 // Everything the rounds of an event-queue job need.
 template <class R> struct EvLaunch {
-    Tables<R> t;
-    Geom<R> g;
+    const Tables<R> *t;
+    const Geom<R> *g;
     uint64_t seed;
     Pool<R> pool[2][2];          // [round parity][photon, charged]
     EvCtl *ctl;
