@@ -10,7 +10,9 @@ phantom and tables resident in HBM.  Beside it the line reports:
 
 * the plain GPU history-by-history form running the reference's loop order
   (one beamlet at a time) on every 20th beamlet, as histories/s;
-* the plan-dose statistical uncertainty in voxels above 50% of the maximum;
+* the plan-dose statistical uncertainty in voxels above 50% of the maximum,
+  and both forms again at the metric's operating point: every beamlet's
+  histories scaled so that this uncertainty reaches 1% (beamlet subsets);
 * the FP32/FP64 plan-dose difference on every 10th beamlet (three seeds
   per precision, region fixed by an independent FP64 pilot run);
 * p50/p99 latency and throughput of a short served request mix;
@@ -69,6 +71,12 @@ def subset(bix, every, offset=0):
     from ompmc_amd import phantoms as P
     sel = np.arange(offset, bix.n, every)
     return P.Beamlets(bix.ibeam[sel], bix.src, bix.corner[sel], bix.side1[sel], bix.side2[sel], bix.nhist[sel])
+
+
+def scaled(bix, k):
+    from ompmc_amd import phantoms as P
+    return P.Beamlets(bix.ibeam, bix.src, bix.corner, bix.side1, bix.side2,
+                      (np.asarray(bix.nhist, np.int64) * k).astype(np.int64))
 
 
 def c3_config():
@@ -252,6 +260,23 @@ def bench_c3(a):
                              "what": "history kernel, one beamlet at a time (reference loop order), every 20th beamlet"}
         out["speedup_vs_plain"] = value / out["plain_form"]["histories_per_s"]
         log(f"plain: {out['plain_form']['histories_per_s']:.4g} histories/s")
+        # the metric's operating point: every beamlet's histories scaled by k so
+        # that the plan dose reaches <= 1% statistical uncertainty in voxels above
+        # 50% of its maximum (uncertainty ~ N^-1/2); both forms at that scale,
+        # the batched one on every 20th beamlet, the plain one on every 400th
+        k = int(math.ceil(max(1.0, (unc50 / 0.01) ** 2)))
+        sb = scaled(subset(bix, 20), k)
+        sp_ = scaled(subset(bix, 400, offset=7), k)
+        _, _, _, stb = ctx.run_dij(gph, spec, sb, nbatch=nbatch, rel_threshold=rel, **opts)
+        _, _, _, stp = ctx.run_dij(gph, spec, sp_, nbatch=nbatch, rel_threshold=rel, precision=prec,
+                                   form=omc.FORM_HISTORY, group=1)
+        rb, rp = stb.histories / stb.wall_seconds, stp.histories / stp.wall_seconds
+        out["at_1pct_unc"] = {"history_scale": k, "histories_for_plan": float(hist / a.steps * k),
+                              "batched_histories_per_s": rb, "batched_beamlets": int(sb.n),
+                              "plain_histories_per_s": rp, "plain_beamlets": int(sp_.n),
+                              "speedup_vs_plain": rb / rp,
+                              "plan_seconds_batched": hist / a.steps * k / rb}
+        log(f"at 1% unc (x{k} histories): batched {rb:.4g}, plain {rp:.4g}, ratio {rb / rp:.1f}")
     if not a.no_fp64 and prec == omc.FP32:
         # FP32 against FP64 on every 10th beamlet: three independent seeds per
         # precision; region (voxels above 20% of the plan maximum) fixed by an

@@ -124,6 +124,8 @@ typedef struct omc_run_opts {
     int group;              /* Dij: beamlets transported together (0 = as many as hbm_budget allows,
                                1 = one beamlet at a time, the reference's loop order) */
     int64_t hbm_budget;     /* device bytes a Dij run may use for its tallies (0 = automatic) */
+    int voxel_walk;         /* event form: 1 = photons fly voxel by voxel as the reference's photon()
+                               does; 0 = delta tracking with the phantom's majorant (same distribution) */
 } omc_run_opts;
 
 /* Kernel classes of omc_run_stats.kernel_seconds / kernel_launches */
@@ -199,9 +201,10 @@ int omc_run_dij(omc_ctx *ctx, int phantom_id, const omc_spectrum_desc *spec, con
                 int nbatch, double rel_threshold, const omc_run_opts *opts, omc_csc *dij, omc_csc *var,
                 omc_run_stats *stats);
 /* omc_run_dij with two optional outputs of nvox doubles each: the dose of
- * all beamlets summed with unit weights and the variance of that sum (sum of
- * the per-beamlet batch variances without the carry-over); no counterpart in
- * the reference, for plan-level statistics. */
+ * all beamlets summed with unit weights, and the variance of that sum (the
+ * sum over beamlets of the batch estimate of the variance of the mean,
+ * (sum x^2/n - mean^2)/(n-1), without the reference's carry-over); no
+ * counterpart in the reference, for plan-level statistics. */
 int omc_run_dij_ex(omc_ctx *ctx, int phantom_id, const omc_spectrum_desc *spec, const omc_beamlets_desc *bix,
                    int nbatch, double rel_threshold, const omc_run_opts *opts, omc_csc *dij, omc_csc *var,
                    double *plan_dose, double *plan_var, omc_run_stats *stats);

@@ -58,7 +58,8 @@ class RunOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("form", C.c_int), ("rng", C.c_int), ("seed", C.c_uint64),
                 ("nsplit", C.c_int), ("replay", _u32), ("replay_offsets", _i64), ("replay_len", C.c_int64),
                 ("grid", C.c_int), ("block", C.c_int), ("count_events", C.c_int), ("event_target", C.c_int64),
-                ("electron_budget", C.c_int), ("group", C.c_int), ("hbm_budget", C.c_int64)]
+                ("electron_budget", C.c_int), ("group", C.c_int), ("hbm_budget", C.c_int64),
+                ("voxel_walk", C.c_int)]
 
 
 class RunStats(C.Structure):
@@ -179,13 +180,14 @@ def default_paths():
 
 def run_opts(precision=FP32, form=FORM_EVENT, seed=1802, rng=RNG_PHILOX, replay=None, replay_offsets=None,
              grid=0, block=0, count_events=False, nsplit=1, event_target=0, electron_budget=0, group=0,
-             hbm_budget=0):
+             hbm_budget=0, voxel_walk=False):
     o = RunOpts()
     lib().omc_run_opts_default(C.byref(o))
     o.precision, o.form, o.seed, o.rng, o.nsplit = precision, form, seed, rng, nsplit
     o.grid, o.block = grid, block
     o.count_events = 2 if count_events == "time" else int(bool(count_events))
     o.event_target, o.electron_budget, o.group, o.hbm_budget = event_target, electron_budget, group, hbm_budget
+    o.voxel_walk = int(bool(voxel_walk))
     keep = []
     if replay is not None:
         rp = np.ascontiguousarray(replay, np.uint32)

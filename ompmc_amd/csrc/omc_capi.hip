@@ -236,7 +236,27 @@ template <class R> static void fill_geom(omc_ctx *c, PhantomDev *pd, Geom<R> &G)
     for (int m = 0; m < OMC_MXMED; m++) {
         G.pcut[m] = m < h.nmed ? (R)(h.ap[m] <= pd->pcut ? pd->pcut : h.ap[m]) : (R)0;
         G.ecut[m] = m < h.nmed ? (R)(h.ae[m] <= pd->ecut ? pd->ecut : h.ae[m]) : (R)0;
+        G.rhof_max[m] = (R)0;
     }
+    // largest density ratio per medium (majorant of the photon delta tracking)
+    for (long v = 0; v < nvox; v++) {
+        int im = ph.med[v] - 1;
+        if (im >= 0 && rhof[v] > G.rhof_max[im]) G.rhof_max[im] = rhof[v];
+    }
+    // equal spacing per axis: voxel lookup by multiplication
+    auto even = [](const double *b, int n, double &inv) {
+        double d = (b[n] - b[0]) / n;
+        inv = 1.0 / d;
+        for (int i = 0; i < n; i++)
+            if (fabs((b[i + 1] - b[i]) - d) > 1e-9 * fabs(d)) return false;
+        return true;
+    };
+    double ix, iy, iz;
+    bool u = even(ph.xb, ph.nx, ix) & even(ph.yb, ph.ny, iy) & even(ph.zb, ph.nz, iz);
+    G.uniform = u ? 1 : 0;
+    G.inv_dx = (R)ix;
+    G.inv_dy = (R)iy;
+    G.inv_dz = (R)iz;
 }
 
 size_t omc_phantom_bytes(const omc_phantom_desc *d) {

@@ -121,6 +121,9 @@ template <class R> struct Geom {
     const uint8_t *med;
     const R *rhof;
     R pcut[MXMED], ecut[MXMED];    // region cuts per medium: global cut, or the PEGS AP/AE when higher
+    R rhof_max[MXMED];             // largest density ratio of each medium in the phantom (0: absent)
+    int uniform;                   // 1: equal spacing per axis, voxel lookup by multiplication
+    R inv_dx, inv_dy, inv_dz;      // 1 / spacing when uniform
 };
 
 // ------------------------------------------------------------------ particles
@@ -648,6 +651,14 @@ template <class R, class G, class T> struct Engine {
             }
             if (imed != -1 && dpmfp <= (R)1.0E-05) break;
         }
+        return interact(p, q, imed, eig, gle, lgle, cohfac);
+    }
+
+    // interaction at the end of a photon flight (src/ompmc.c:2044-2086):
+    // Rayleigh with probability 1 - cohfac, else pair / Compton / photo by the
+    // branching ratios of the medium at the site
+    __device__ __forceinline__ int interact(Part<R> &p, Part<R> &q, int imed, R eig, R gle, int lgle, R cohfac) {
+        const R RM = (R)0.5109989461;
         R r = rnd();
         if (r <= (R)1.0 - cohfac) {
             rayleigh(p, imed, eig, gle, lgle);
@@ -666,6 +677,84 @@ template <class R, class G, class T> struct Engine {
         }
         photo(p);
         return 0;
+    }
+
+    // voxel index along one axis: direct for equal spacing, else bisection
+    __device__ __forceinline__ int locate(const R *b, int n, R x, R inv) const {
+        int i;
+        if (g.uniform) {
+            i = (int)((x - b[0]) * inv);
+        } else {
+            int lo = 0, hi = n;
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                if (b[mid] <= x) lo = mid;
+                else hi = mid;
+            }
+            i = lo;
+        }
+        return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+    }
+
+    // Photon flight by delta (Woodcock) tracking, for the event form.  With
+    // the photon energy fixed, the attenuation coefficient of a voxel is
+    // rhof / (gmfp(E) cohfac(E)) of its medium, so the largest over the
+    // phantom is the largest over media of rhof_max / (gmfp cohfac): an exact
+    // majorant.  Flights are sampled with it and a tentative site is real
+    // with probability mu(site) / mu_max; the interaction then follows the
+    // reference exactly.  Same distribution of sites as the voxel-by-voxel
+    // flight of photon() (not the same draws), without a dependent load per
+    // voxel crossed.
+    __device__ int photon_delta(Part<R> &p, Part<R> &q) {
+        if (count) cnt.nphoton++;
+        int imed = medium(p);
+        R eig = p.e;
+        if (eig <= pcut_of(imed) || p.wt == (R)0) {
+            ausgab(p, eig);
+            return -1;
+        }
+        R gle = M<R>::log(eig);
+        R mu_max = (R)0;
+        for (int m = 0; m < t.nmed; m++) {
+            R rm = g.rhof_max[m];
+            if (rm <= (R)0) continue;
+            const Med<R> &md = t.med[m];
+            int lg = pwlf_i(gle, md.ge1, md.ge0) - 1;
+            R mu = rm / (pwlf(t.gmfp[m * MXGE + lg], gle) * pwlf(t.cohe[m * MXGE + lg], gle));
+            mu_max = M<R>::fmax(mu_max, mu);
+        }
+        if (!(mu_max > (R)0)) {          // nothing to interact with: leaves
+            p.ix = -1;
+            return -1;
+        }
+        const R xlo = g.xb[0], xhi = g.xb[g.nx], ylo = g.yb[0], yhi = g.yb[g.ny], zlo = g.zb[0], zhi = g.zb[g.nz];
+        for (;;) {
+            R rn = rnd();
+            R s = -M<R>::log((R)1.0 - rn) / mu_max;
+            if (count) cnt.nhowfar++;
+            p.x += s * p.u;
+            p.y += s * p.v;
+            p.z += s * p.w;
+            if (!(p.x >= xlo && p.x <= xhi && p.y >= ylo && p.y <= yhi && p.z >= zlo && p.z <= zhi)) {
+                p.ix = -1;               // left the phantom (howfar discards it)
+                return -1;
+            }
+            p.ix = locate(g.xb, g.nx, p.x, g.inv_dx);
+            p.iy = locate(g.yb, g.ny, p.y, g.inv_dy);
+            p.iz = locate(g.zb, g.nz, p.z, g.inv_dz);
+            if (count) cnt.ncross++;
+            size_t v = p.ix + (size_t)g.nx * (p.iy + (size_t)g.ny * p.iz);
+            uint8_t mm = g.med[v];
+            R r2 = rnd();
+            if (mm == VACUUM) continue;
+            imed = (int)mm;
+            const Med<R> &md = t.med[imed];
+            int lgle = pwlf_i(gle, md.ge1, md.ge0) - 1;
+            R gmfp = pwlf(t.gmfp[imed * MXGE + lgle], gle) / g.rhof[v];
+            R cohfac = pwlf(t.cohe[imed * MXGE + lgle], gle);
+            R mu = (R)1.0 / (gmfp * cohfac);
+            if (r2 * mu_max <= mu) return interact(p, q, imed, eig, gle, lgle, cohfac);
+        }
     }
 
     // ---------------------------------------------------------------- electron physics

@@ -59,27 +59,38 @@ int device_cus(int dev) {
 }
 
 // ------------------------------------------------------------------ event-form state
-constexpr int DEFAULT_BUDGET = 32;           // condensed-history steps per charged event (measured best on gfx950)
+constexpr int DEFAULT_BUDGET = 8;            // condensed-history steps between lane refills
 constexpr long MAX_ROUNDS = 2000000;         // a job that has not drained by then is reported as a step-guard error
 
 template <class R> size_t pool_entry_bytes() { return sizeof(R) * 10 + sizeof(int) * 6 + 4 + 8; }
 
 template <class R> size_t event_bytes(long long target) {
-    return (size_t)4 * (size_t)(4 * target) * pool_entry_bytes<R>() + sizeof(EvCtl);
+    return (size_t)4 * (size_t)(8 * target) * pool_entry_bytes<R>() + sizeof(EvCtl);
 }
 
 // particles in flight: 2048 per compute unit, 8 per lane at 4 waves per SIMD
 long long default_target(int dev) { return (long long)device_cus(dev) * 2048; }
 
+// run_job's code for a pool overflow: the caller zeroes the job's tallies,
+// doubles the pools (EvState::grow) and runs the job again.  Every particle's
+// random stream is keyed by its history, so the rerun is the same job.
+constexpr int RC_POOL_OVERFLOW = -100;
+constexpr int MAX_POOL_GROWTH = 5;
+
 template <class R> struct EvState {
     DevArena A;
     Pool<R> pool[2][2];
     EvCtl *ctl = nullptr;
-    long long target = 0, cap = 0;
+    long long target = 0, cap = 0, capmul = 8;   // pool capacity = capmul x target (peak in flight measured near 4x)
     ~EvState() { A.release(); }
+    int grow() {
+        A.release();
+        capmul *= 2;
+        return alloc(target);
+    }
     int alloc(long long tgt) {
         target = tgt < 64 ? 64 : tgt;
-        cap = 4 * target;
+        cap = capmul * target;
         for (int p = 0; p < 2; p++)
             for (int k = 0; k < 2; k++) {
                 Pool<R> &P = pool[p][k];
@@ -162,7 +173,10 @@ int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int 
     js.ev_charged += done->events[1];
     js.rounds += (uint64_t)(done->done_round >= 0 ? done->done_round : done->rounds);
     js.peak = std::max<uint64_t>(js.peak, done->peak);
-    if (done->err & 1) return omc_seterr(OMC_ERR_STACK, "event pool overflow (target %lld, capacity %lld)", S.target, S.cap);
+    if (done->err & 1) {
+        omc_seterr(OMC_ERR_STACK, "event pool overflow (target %lld, capacity %lld)", S.target, S.cap);
+        return RC_POOL_OVERFLOW;
+    }
     if (done->done_round < 0) return omc_seterr(OMC_ERR_LIMIT, "event job did not drain within %ld rounds", MAX_ROUNDS);
     return OMC_OK;
 }
@@ -178,12 +192,33 @@ template <class R> void fill_launch(omc_ctx *c, const PhantomDev *pd, EvState<R>
     L.ctl = S.ctl;
     L.cap = S.cap;
     L.budget = o->electron_budget > 0 ? o->electron_budget : DEFAULT_BUDGET;
+    L.delta = o->voxel_walk ? 0 : 1;
     int cus = device_cus(c->device);
     L.grid_event = o->grid > 0 ? o->grid : cus * 8;
     L.grid_source = cus * 4;
     L.src.t = L.t;
     L.src.g = L.g;
     L.src.seed = o->seed;
+}
+
+// Runs a job, growing the pools and rerunning it from zeroed tallies
+// (nslot slots of nreg doubles at `endep`) when the queues overflow.
+template <class R>
+int run_job_grow(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int dest, bool count, bool timing,
+                 JobStats &js, double *endep, size_t ndouble) {
+    for (int attempt = 0;; attempt++) {
+        int rc = run_job(st, S, L, nhist, dest, count, timing, js);
+        if (rc != RC_POOL_OVERFLOW) return rc;
+        if (attempt >= MAX_POOL_GROWTH) return OMC_ERR_STACK;
+        CK(hipMemsetAsync(endep, 0, sizeof(double) * ndouble, st));
+        CK(hipStreamSynchronize(st));
+        rc = S.grow();
+        if (rc) return rc;
+        for (int p = 0; p < 2; p++)
+            for (int k = 0; k < 2; k++) L.pool[p][k] = S.pool[p][k];
+        L.ctl = S.ctl;
+        L.cap = S.cap;
+    }
 }
 
 void stats_from_job(omc_run_stats *st, const JobStats &js) {
@@ -256,7 +291,8 @@ int omc_run_dosxyz_event(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const o
     CK(hipEventRecord(ev.ev[0], stream));
     for (int b = 0; b < nbatch; b++) {
         L.src.hist_base = (long)b * nper;
-        rc = run_job(stream, S, L, nper, dest, (o->count_events & 1) != 0, o->count_events != 0, js);
+        rc = run_job_grow(stream, S, L, nper, dest, (o->count_events & 1) != 0, o->count_events != 0, js, d_e,
+                          (size_t)nreg);
         if (rc) return rc;
         CK(launch_accumulate(d_e, d_s, d_s2, nreg, 1.0, stream));
     }
@@ -489,7 +525,7 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
             if (event) {
                 L.src.nslot = ng;
                 L.src.batch = bt;
-                rc = run_job(stream, S, L, njob, 0, count, timing, js);
+                rc = run_job_grow(stream, S, L, njob, 0, count, timing, js, d_e, (size_t)ng * nreg);
                 if (rc) return rc;
             } else {
                 H.nslot = ng;

@@ -4,9 +4,9 @@
 // (photons, charged particles), double-buffered by round parity.  A round is
 // four launches on one stream:
 //   k_source     samples this round's injected primaries into the input pool,
-//   k_event<1>   one photon() call per queued photon,
-//   k_event<0>   up to `budget` condensed-history steps (electron_run) per
-//                queued electron/positron,
+//   k_event<1>   transports the queued photons (persistent lanes, below),
+//   k_event<0>   transports the queued electrons/positrons, `budget`
+//                condensed-history steps (electron_run) between refills,
 //   k_round_end  retires the consumed input pools and plans the next
 //                injection so that the number of particles in flight stays
 //                near a target.
@@ -67,8 +67,9 @@ struct EvCtl {
     long long inj_h0, inj_n, inj_at;  // injection of the coming round
     long long target, cap;
     long long rounds, done_round;     // rounds retired; round at which the job drained (-1 while running)
-    unsigned long long events[2];     // photon / charged events processed
+    unsigned long long events[2];     // photon / charged particles taken from the queues
     unsigned long long peak;          // largest number of particles in flight
+    unsigned long long claim[2][2];   // [round parity][photon, charged] queue entries claimed by lanes
     int inj_dest;                     // pool of the primaries: 0 photons, 1 charged
     int err;                          // bit 0: pool overflow
 };
@@ -81,12 +82,14 @@ template <class R> struct EvArgs {
     const unsigned long long *n_in;   // its length
     Pool<R> outp, oute;               // photon and charged output pools
     unsigned long long *n_out;        // [2] their lengths
+    unsigned long long *claim;        // entries of the input queue claimed so far (zero at launch)
     long long cap;                    // capacity of each pool
     double *endep;                    // tally base
     long slot_stride;                 // tally stride per slot
     unsigned long long *counters;     // [8] event counters (optional)
     int *err;                         // EvCtl::err
-    int budget;                       // electron ustep iterations per event
+    int budget;                       // electron ustep iterations between refills of a wave
+    int delta;                        // photons: 1 delta tracking, 0 voxel-by-voxel flight
 };
 
 template <class R> __device__ __forceinline__ void load(const Pool<R> &P, long i, Part<R> &p) {
@@ -148,64 +151,79 @@ __device__ __forceinline__ void flush_counters(unsigned long long *dst, const Co
     }
 }
 
-// One event per queued particle: PHOTON selects photon(), otherwise up to
-// a.budget condensed-history steps of electron_run().  Waves stride over the
-// queue, so any grid covers any queue length.
 // The charged-particle kernel needs about 250 VGPRs unconstrained (2 waves
 // per SIMD); capping it at 128 (4 waves) spills some state to scratch and
 // still runs faster, as measured on gfx950.
 #ifndef OMC_CHARGED_WAVES_PER_EU
 #define OMC_CHARGED_WAVES_PER_EU 4
 #endif
+
+// Persistent transport of one particle type.  Every lane holds one particle
+// in registers and advances it -- one photon() call, or up to a.budget
+// condensed-history steps of electron_run() -- then the wave meets: lanes
+// whose particle left, or turned into the other type, hand it and any
+// secondary to the output queues and claim the next queued particle (one
+// atomic per wave).  A particle of this kernel's type stays in its lane until
+// it is gone, so the queues only carry new secondaries and type changes, and
+// no lane idles while the queue still holds work.  The wave exits when its
+// lanes are empty and the queue is exhausted.
 template <class R, bool PHOTON, bool COUNT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PHOTON ? 1 : OMC_CHARGED_WAVES_PER_EU)))
 k_event(EvArgs<R> a) {
-    const long n = (long)*a.n_in;
-    const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
-    const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
+    const long long n = (long long)*a.n_in;
     Counters tot = {0, 0, 0, 0, 0, 0, 0};
-    for (long base = wave * 64; base < n; base += nwaves * 64) {
-        long i = base + lane;
-        bool have = i < n;
-        Part<R> p, q;
+    bool have = false;
+    Part<R> p, q;
+    uint64_t stream = 0;
+    int slot = 0, phase = 0;
+    R demfp = (R)0, sig0 = (R)0;
+    PhiloxAt rng;
+    TallyAtomic tally;
+    tally.init(a.endep);
+    for (;;) {
+        long long idx = wave_reserve(a.claim, !have);
+        if (!have && idx >= 0 && idx < n) {
+            load(a.in, idx, p);
+            stream = a.in.stream[idx];
+            slot = a.in.slot[idx];
+            demfp = a.in.demfp[idx];
+            sig0 = a.in.sig0[idx];
+            phase = a.in.phase[idx];
+            rng.at(a.seed, stream, a.in.ctr[idx]);
+            tally.init(a.endep + (size_t)slot * a.slot_stride);
+            have = true;
+        }
+        if (!__any(have)) break;
         int d = -1;
-        uint64_t stream = 0;
-        int slot = 0, phase = 0;
-        R demfp = (R)0, sig0 = (R)0;
+        bool done = false;
         uint32_t qpos = 0;
         if (have) {
-            load(a.in, i, p);
-            stream = a.in.stream[i];
-            slot = a.in.slot[i];
-            PhiloxAt rng;
-            rng.at(a.seed, stream, a.in.ctr[i]);
-            TallyAtomic tally;
-            tally.init(a.endep + (size_t)slot * a.slot_stride);
             Engine<R, PhiloxAt, TallyAtomic> e{*a.t, *a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
             if (PHOTON) {
-                d = e.photon(p, q);
+                d = a.delta ? e.photon_delta(p, q) : e.photon(p, q);
             } else {
                 typename Engine<R, PhiloxAt, TallyAtomic>::EState es;
-                es.demfp = a.in.demfp[i];
-                es.sig0 = a.in.sig0[i];
-                es.phase = a.in.phase[i];
+                es.demfp = demfp;
+                es.sig0 = sig0;
+                es.phase = phase;
                 d = e.electron_run(p, q, es, a.budget);
                 demfp = es.demfp;
                 sig0 = es.sig0;
                 phase = es.phase;
             }
-            tally.flush();
             qpos = rng.n;
+            // d: -1 particle gone, 0 particle continues, +1 continues with a secondary
+            done = d < 0 || ((p.iq == 0) != PHOTON);
+            if (done) tally.flush();
             if (COUNT) {
                 tot.nhowfar += e.cnt.nhowfar; tot.nhownear += e.cnt.nhownear; tot.ncross += e.cnt.ncross;
                 tot.ndep += e.cnt.ndep; tot.nrn += e.cnt.nrn; tot.nphoton += e.cnt.nphoton;
                 tot.nelectron += e.cnt.nelectron;
             }
         }
-        // d: -1 particle gone, 0 particle continues, +1 continues with a secondary
-        emit(a, have && d >= 0, p, demfp, sig0, phase, stream, qpos, slot);
+        emit(a, have && done && d >= 0, p, demfp, sig0, phase, stream, qpos, slot);
         emit(a, have && d == 1, q, (R)0, (R)0, 0, child_stream(stream, qpos), 0u, slot);
+        if (done) have = false;
     }
     flush_counters<COUNT>(a.counters, tot);
 }
@@ -258,6 +276,8 @@ __device__ __forceinline__ void round_end(EvCtl *c, int par) {
     c->events[1] += c->cnt[par][1];
     c->cnt[par][0] = 0;
     c->cnt[par][1] = 0;
+    c->claim[par][0] = 0;
+    c->claim[par][1] = 0;
     int nx = 1 - par;
     for (int k = 0; k < 2; k++)
         if ((long long)c->cnt[nx][k] > c->cap) {
@@ -290,6 +310,7 @@ static __global__ void k_round_end(EvCtl *c, int par) {
 static __global__ void k_job_start(EvCtl *c, long long nhist, long long target, long long cap, int dest) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     c->cnt[0][0] = c->cnt[0][1] = c->cnt[1][0] = c->cnt[1][1] = 0;
+    c->claim[0][0] = c->claim[0][1] = c->claim[1][0] = c->claim[1][1] = 0;
     c->nhist = nhist;
     c->next_hist = 0;
     c->target = target;

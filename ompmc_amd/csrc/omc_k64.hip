@@ -64,8 +64,13 @@ k_dij_finish(double *sum, double *sum2, double *carry, int nslot, long stride, l
             if (e != 0.0) {
                 u = e2 - e * e;
                 u /= nbatch;
-                double f2 = e2raw * (factor * factor) / (double)(nbatch - 1);
-                u0 = (f2 - e * e) / nbatch;
+                // plan statistics: the batch estimate of the variance of the
+                // mean, (sum x^2 / n - mean^2) / (n - 1), without the
+                // reference's carry-over and without its 1/(n-1) on the sum
+                // of squares (which leaves a floor of mean^2 / (n (n-1)))
+                double m2 = e2raw * (factor * factor) / (double)nbatch;
+                u0 = (m2 - e * e) / (double)(nbatch - 1);
+                if (u0 < 0.0) u0 = 0.0;
             } else {
                 e = 0.0;
                 u = 0.0;

@@ -45,8 +45,10 @@ template <class R> hipError_t launch_round_part(const EvLaunch<R> &L, long r, in
         a.counters = L.counters ? L.counters + (part == 1 ? 0 : 8) : nullptr;
         a.err = &L.ctl->err;
         a.budget = L.budget;
+        a.delta = L.delta;
         a.in = L.pool[par][part - 1];
         a.n_in = &L.ctl->cnt[par][part - 1];
+        a.claim = &L.ctl->claim[par][part - 1];
         if (part == 1) {
             if (count) hipLaunchKernelGGL((k_event<R, true, true>), dim3(L.grid_event), dim3(256), 0, st, a);
             else hipLaunchKernelGGL((k_event<R, true, false>), dim3(L.grid_event), dim3(256), 0, st, a);

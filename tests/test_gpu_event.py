@@ -92,6 +92,21 @@ def test_event_result_independent_of_schedule(slab):
         assert np.allclose(g, ref, rtol=1e-10, atol=1e-12 * ref.max()), kw
 
 
+def test_event_pool_growth_reruns_job():
+    """6 MeV electrons with a tiny queue target overflow the pools (each lane
+    runs its electron to the end and emits every bremsstrahlung photon); the
+    job is rerun with larger pools from zeroed tallies and gives the dose of
+    the default run to FP64 summation order."""
+    ph_h = P.read_egsphant(golden("electron.egsphant"))
+    c = omc.Context(ph_h.media)
+    ph = c.phantom(ph_h, ecut=0.7, pcut=0.01)
+    ref, _, st0 = _run(c, ph, "electron", 4000, 4, precision=omc.FP64, form=omc.FORM_EVENT, seed=8)
+    g, _, st1 = _run(c, ph, "electron", 4000, 4, precision=omc.FP64, form=omc.FORM_EVENT, seed=8, event_target=64)
+    c.close()
+    assert st0.errors == 0 and st1.errors == 0
+    assert np.allclose(g, ref, rtol=1e-10, atol=1e-12 * ref.max())
+
+
 def test_event_matches_history_form_fp32(slab):
     """Both GPU forms sample the same physics; FP32 transport."""
     c, ph = slab
@@ -134,6 +149,31 @@ def test_fp32_dose_within_half_percent_of_fp64(ct):
     sem = np.hypot(a.std(ddof=1), b.std(ddof=1)) / np.sqrt(8) / b.mean()
     print(f"FP32 vs FP64 dose (voxels > 50% max): {rel:+.3e} +- {sem:.1e}")
     assert abs(rel) < 5e-3
+
+
+def test_delta_tracking_matches_voxel_walk(ct):
+    """Photon delta tracking (the event form's default) against the
+    reference's voxel-by-voxel flight, on a CT with air, lung, bone and
+    per-voxel densities: per-voxel agreement of the batch means, and the dose
+    summed over voxels above 20% of the maximum (region fixed by a pilot run)
+    compared through the spread of independent seeds."""
+    c, ph = ct
+    spec = omc.Spectrum.read(omc.os.path.join(omc.DATA, "spectra", "mohan6.spectrum"))
+    kw = dict(collimator=(-3, 3, -3, 3), precision=omc.FP64, form=omc.FORM_EVENT)
+    a1, a2, _ = c.run_dosxyz(ph, spec, 20000, 20, seed=41, **kw)
+    b1, b2, _ = c.run_dosxyz(ph, spec, 20000, 20, seed=42, voxel_walk=True, **kw)
+    f2, f3, n = _agreement(a1, a2, b1, b2, 20)
+    print(f"delta vs voxel walk: {n} voxels, {f2:.3f} within 2 sigma, {f3:.3f} within 3 sigma")
+    assert n > 100 and f2 >= 0.88 and f3 >= 0.97
+    pilot = c.run_dosxyz(ph, spec, 50000, 4, seed=40, **kw)[0][1:]
+    sel = pilot > 0.2 * pilot.max()
+    sums = {vw: np.array([c.run_dosxyz(ph, spec, 50000, 4, seed=50 + 2 * k + vw, voxel_walk=vw, **kw)[0][1:][sel].sum()
+                          for k in range(6)]) for vw in (False, True)}
+    a, b = sums[False], sums[True]
+    d = a.mean() / b.mean() - 1.0
+    sem = np.hypot(a.std(ddof=1), b.std(ddof=1)) / np.sqrt(6) / b.mean()
+    print(f"delta vs voxel walk, dose > 20% of max: {d:+.2e} +- {sem:.1e}")
+    assert abs(d) < 4.0 * sem + 1e-3
 
 
 def test_event_counts_and_kernel_times(slab):
