@@ -374,8 +374,9 @@ extern "C" void omc_run_opts_default(omc_run_opts *o) {
 int omc_check_opts(const omc_run_opts *o) {
     if (!o) return omc_seterr(OMC_ERR_ARG, "null run options");
     if (o->precision != OMC_FP32 && o->precision != OMC_FP64) return omc_seterr(OMC_ERR_ARG, "precision %d", o->precision);
-    if (o->nsplit != 1)
-        return omc_seterr(OMC_ERR_ARG, "nsplit %d: the GPU transport supports photon splitting factor 1 only", o->nsplit);
+    if (o->nsplit < 1 || o->nsplit > 64) return omc_seterr(OMC_ERR_ARG, "nsplit %d (1..64)", o->nsplit);
+    if (o->nsplit > 1 && o->form != OMC_FORM_HISTORY)
+        return omc_seterr(OMC_ERR_ARG, "photon splitting (nsplit %d) runs in the history form", o->nsplit);
     if (o->form != OMC_FORM_HISTORY && o->form != OMC_FORM_EVENT) return omc_seterr(OMC_ERR_ARG, "form %d", o->form);
     if (o->rng == OMC_RNG_REPLAY && (!o->replay || !o->replay_offsets))
         return omc_seterr(OMC_ERR_ARG, "replay random numbers requested without a stream");
@@ -429,6 +430,7 @@ static int run_dosxyz_history(omc_ctx *c, hipStream_t stream, PhantomDev *pd, co
     a.seed = o->seed;
     a.replay = d_rn;
     a.chunk = 1;
+    a.nsplit = o->nsplit;
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
     if (o->count_events) {

@@ -225,6 +225,7 @@ template <class R, class G, class T> struct Engine {
     T &tally;
     Counters cnt;
     bool count;
+    int nsplit = 1;                 // photon splitting factor (vrt.nsplit)
 
     __device__ __forceinline__ R rnd() {
         if (count) cnt.nrn++;
@@ -757,6 +758,150 @@ template <class R, class G, class T> struct Engine {
         }
     }
 
+    // photon() with splitting (src/ompmc.c:1867-2139) on a stack array: the
+    // photon on top of st[0..np] is split into nsplit sub-photons with
+    // stratified numbers of mean free paths along its ray; each interacts at
+    // its site, and of the scattered photons only those of one randomly chosen
+    // sub-photon survive, with the full weight.  Returns the new np (the
+    // reference's stack order, including its in-place compaction, which
+    // copies energy, charge, direction and weight but not the position).
+    __device__ int photon_split(Part<R> *st, int np, int cap, int *err) {
+        const R RM = (R)0.5109989461;
+        if (count) cnt.nphoton++;
+        int imed = medium(st[np]);
+        R eig = st[np].e;
+        if (eig <= pcut_of(imed) || st[np].wt == (R)0) {
+            ausgab(st[np], eig);
+            return np - 1;
+        }
+        R gle = M<R>::log(eig);
+        int lgle = 0;
+        R cohfac = (R)0, gmfp = (R)0;
+        const int ns = nsplit;
+        for (;;) {                                  // start of the splitting of the top photon
+            R rnno = rnd();
+            rnno /= (R)ns;
+            R d_eta = (R)1.0 / (R)ns;
+            R eta_prime = (R)1.0 - rnno + d_eta;
+            Part<R> save = st[np];
+            R wtsave = save.wt / (R)ns;
+            np -= 1;
+            rnno = rnd();
+            R a_survive = rnno * (R)ns;
+            int i_survive = (int)a_survive;
+            R dpmfp_old = (R)0.0;
+            for (int isplit = 0; isplit < ns; isplit++) {
+                eta_prime -= d_eta;
+                if (eta_prime <= (R)0.0) break;
+                R dpmfp = -M<R>::log(eta_prime) - dpmfp_old;
+                dpmfp_old += dpmfp;
+                np += 1;
+                if (np >= cap - 1) {
+                    *err = 1;
+                    return -1;
+                }
+                Part<R> &p = st[np];
+                p.x = save.x; p.y = save.y; p.z = save.z;
+                p.u = save.u; p.v = save.v; p.w = save.w;
+                p.e = save.e; p.wt = wtsave;
+                p.ix = save.ix; p.iy = save.iy; p.iz = save.iz; p.iq = 0;
+                imed = medium(p);
+                bool discard = false;
+                for (;;) {
+                    R tstep;
+                    if (imed != -1) {
+                        const Med<R> &m = t.med[imed];
+                        lgle = pwlf_i(gle, m.ge1, m.ge0) - 1;
+                        R gmfpr0 = pwlf(t.gmfp[imed * MXGE + lgle], gle);
+                        gmfp = gmfpr0 / rhof_of(p);
+                        cohfac = pwlf(t.cohe[imed * MXGE + lgle], gle);
+                        gmfp *= cohfac;
+                        tstep = gmfp * dpmfp;
+                    } else {
+                        tstep = (R)1.0E8;
+                    }
+                    R ustep = tstep;
+                    if (p.ix < 0) {       // howfar: outside the geometry -> discard
+                        if (count) cnt.nhowfar++;
+                        p.x += ustep * p.u; p.y += ustep * p.v; p.z += ustep * p.w;
+                        discard = true;
+                        break;
+                    }
+                    bool leaves;
+                    int mv = howfar(p, ustep, leaves);
+                    p.x += ustep * p.u;
+                    p.y += ustep * p.v;
+                    p.z += ustep * p.w;
+                    if (imed != -1) dpmfp = M<R>::fmax((R)0.0, dpmfp - ustep / gmfp);
+                    if (mv != 0) {
+                        apply_move(p, mv, leaves);
+                        imed = medium(p);
+                    }
+                    if (imed != -1 && dpmfp <= (R)1.0E-05) break;
+                }
+                if (discard) {
+                    np -= 1;
+                    if (np < 0) return np;
+                    break;                          // end of the splitting loop
+                }
+                save.x = p.x; save.y = p.y; save.z = p.z;
+                save.ix = p.ix; save.iy = p.iy; save.iz = p.iz;
+                R r = rnd();
+                if (r <= (R)1.0 - cohfac) {
+                    if (isplit != i_survive) {
+                        np -= 1;
+                    } else {
+                        st[np].wt *= (R)ns;
+                        rayleigh(st[np], imed, eig, gle, lgle);
+                    }
+                    continue;
+                }
+                r = rnd();
+                int npold = np;
+                R gbr1 = pwlf(t.gbr1[imed * MXGE + lgle], gle);
+                if (r <= gbr1 && eig > (R)2.0 * RM) {
+                    pair(imed, st[np], st[np + 1]);
+                    np += 1;
+                } else {
+                    R gbr2 = pwlf(t.gbr2[imed * MXGE + lgle], gle);
+                    if (r < gbr2) {
+                        compton(st[np], st[np + 1]);
+                        np += 1;
+                    } else {
+                        photo(st[np]);
+                    }
+                }
+                int ip = npold;
+                do {
+                    if (st[ip].iq == 0) {
+                        if (isplit != i_survive) {
+                            if (ip < np) {
+                                st[ip].e = st[np].e; st[ip].iq = st[np].iq;
+                                st[ip].u = st[np].u; st[ip].v = st[np].v;
+                                st[ip].w = st[np].w; st[ip].wt = st[np].wt;
+                            }
+                            np -= 1;
+                        } else {
+                            st[ip].wt *= (R)ns;
+                            ip += 1;
+                        }
+                    } else {
+                        ip += 1;
+                    }
+                } while (ip <= np);
+            }
+            if (np < 0) return np;
+            if (st[np].iq != 0) return np;
+            eig = st[np].e;
+            imed = medium(st[np]);
+            if (eig <= pcut_of(imed)) {
+                ausgab(st[np], eig);
+                return np - 1;
+            }
+            gle = M<R>::log(eig);
+        }
+    }
+
     // ---------------------------------------------------------------- electron physics
     struct Spinr { int i, j; };
     struct Mscats { int i, j; R omega2; };
@@ -1075,6 +1220,22 @@ template <class R, class G, class T> struct Engine {
         return de;
     }
 
+    // Russian roulette of a photon produced by an electron interaction when
+    // splitting is on (src/ompmc.c:4167-4182, 4356-4371, 4645-4660): killed
+    // (weight and energy 0) with probability 1 - 1/nsplit, else its weight
+    // times nsplit.  Called in stack order, one draw per photon.
+    __device__ __forceinline__ void roulette(Part<R> &x) {
+        if (nsplit > 1 && x.iq == 0) {
+            R rn = rnd();
+            if (rn * (R)nsplit > (R)1.0) {
+                x.wt = (R)0.0;
+                x.e = (R)0.0;
+            } else {
+                x.wt *= (R)nsplit;
+            }
+        }
+    }
+
     // two-photon annihilation at rest (src/ompmc.c:4129-4185)
     __device__ void rannih(Part<R> &a, Part<R> &b) {
         const R RM = (R)0.5109989461;
@@ -1089,6 +1250,8 @@ template <class R, class G, class T> struct Engine {
         b.e = RM; b.iq = 0;
         transfer(b, a);
         b.u = (R)-1.0 * a.u; b.v = (R)-1.0 * a.v; b.w = (R)-1.0 * a.w;
+        roulette(a);
+        roulette(b);
     }
 
     // bremsstrahlung (src/ompmc.c:4188-4374)
@@ -1171,6 +1334,8 @@ template <class R, class G, class T> struct Engine {
             pg.w = c * costhe;
         }
         pe.e = ese;
+        roulette(pe);
+        roulette(pg);
     }
 
     // Moller (src/ompmc.c:4377-4453); nothing happens at or below 2*TE (quirk Q4)
@@ -1299,6 +1464,8 @@ template <class R, class G, class T> struct Engine {
             p1.v = sinthe * sphi;
             p1.w = cc * costhe;
         }
+        roulette(p0);
+        roulette(p1);
     }
 
     // electron below cut-off or discarded: deposit, annihilate a positron

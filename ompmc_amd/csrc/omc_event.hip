@@ -489,6 +489,7 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
         H.replay = d_rn;
         H.replay_off = d_rstart;
         H.replay_end = d_rend;
+        H.nsplit = o->nsplit;
     }
     HostCSC out;
     out.jc.assign(nbix + 1, 0);

@@ -161,6 +161,7 @@ template <class R> struct HistArgs {
     int *err;                    // [4] 0 overflow, 1 replay underrun, 2 step guard
     unsigned long long *next;    // dynamic work counter (zeroed by the host)
     int chunk;
+    int nsplit;                  // photon splitting factor; > 1 selects k_history_split
 };
 
 // Random stream of history h_local of batch `batch` of a beamlet: the
@@ -244,6 +245,72 @@ __global__ void __launch_bounds__(256) k_history(HistArgs<R> a) {
                 }
                 if (++guard > 100000000L) { atomicOr(a.err + 2, 1); break; }
             }
+            tally.flush();
+            if (COUNT) {
+                tot.nhowfar += e.cnt.nhowfar; tot.nhownear += e.cnt.nhownear; tot.ncross += e.cnt.ncross;
+                tot.ndep += e.cnt.ndep; tot.nrn += e.cnt.nrn; tot.nphoton += e.cnt.nphoton;
+                tot.nelectron += e.cnt.nelectron;
+            }
+        }
+    }
+    if (COUNT && a.counters) {
+        atomicAdd(a.counters + 0, tot.nhowfar); atomicAdd(a.counters + 1, tot.nhownear);
+        atomicAdd(a.counters + 2, tot.ncross); atomicAdd(a.counters + 3, tot.ndep);
+        atomicAdd(a.counters + 4, tot.nrn); atomicAdd(a.counters + 5, tot.nphoton);
+        atomicAdd(a.counters + 6, tot.nelectron);
+    }
+}
+
+// History form with photon splitting: the whole LIFO stack of a history
+// lives in an array (scratch), indexed like the reference's particle stack,
+// because the splitting photon routine pushes sub-photons and compacts the
+// stack in place.
+constexpr int SPLIT_STACK = 96;
+
+template <class R, class G, bool COUNT>
+__global__ void __launch_bounds__(256) k_history_split(HistArgs<R> a) {
+    Part<R> st[SPLIT_STACK];
+    Counters tot = {0, 0, 0, 0, 0, 0, 0};
+    for (;;) {
+        long h0 = 0;
+        if (threadIdx.x % warpSize == 0) h0 = (long)atomicAdd(a.next, (unsigned long long)a.chunk * warpSize);
+        h0 = __shfl(h0, 0) + (long)(threadIdx.x % warpSize) * a.chunk;
+        if (__shfl(h0, 0) >= a.nhist) break;
+        for (long h = h0; h < h0 + a.chunk && h < a.nhist; h++) {
+            int slot = 0, beamlet = 0;
+            uint64_t stream = (uint64_t)(a.hist_base + h);
+            if (a.use_beamlets) {
+                slot = find_slot(a.slot_off, a.nslot, h);
+                beamlet = a.slot_beamlet[slot];
+                stream = beamlet_stream(beamlet, a.batch, h - a.slot_off[slot]);
+            }
+            G rng;
+            make_rng<R, G>(rng, a, h, stream);
+            TallyAtomic tally;
+            tally.init(a.endep + (size_t)slot * a.slot_stride);
+            Engine<R, G, TallyAtomic> e{*a.t, *a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
+            e.nsplit = a.nsplit;
+            if (a.use_beamlets) init_history(e, a.bix, beamlet, st[0]);
+            else init_history(e, a.dos, st[0]);
+            int np = 0, err = 0;
+            long guard = 0;
+            while (np >= 0) {
+                if (st[np].iq == 0) {
+                    np = e.photon_split(st, np, SPLIT_STACK, &err);
+                } else {
+                    Part<R> q;
+                    int d = e.electron(st[np], q);
+                    if (d < 0) {
+                        np -= 1;
+                    } else if (d > 0) {
+                        if (np + 1 >= SPLIT_STACK) { err = 1; break; }
+                        st[++np] = q;
+                    }
+                }
+                if (err) break;
+                if (++guard > 100000000L) { atomicOr(a.err + 2, 1); break; }
+            }
+            if (err) atomicOr(a.err, 1);
             tally.flush();
             if (COUNT) {
                 tot.nhowfar += e.cnt.nhowfar; tot.nhownear += e.cnt.nhownear; tot.ncross += e.cnt.ncross;

@@ -7,6 +7,16 @@ namespace omc {
 
 template <class R> hipError_t launch_history(const HistArgs<R> &a, int rng, bool count, int grid, int block,
                                              hipStream_t st) {
+    if (a.nsplit > 1) {
+        if (rng == RNG_REPLAY) {
+            if (count) hipLaunchKernelGGL((k_history_split<R, Replay, true>), dim3(grid), dim3(block), 0, st, a);
+            else hipLaunchKernelGGL((k_history_split<R, Replay, false>), dim3(grid), dim3(block), 0, st, a);
+        } else {
+            if (count) hipLaunchKernelGGL((k_history_split<R, Philox, true>), dim3(grid), dim3(block), 0, st, a);
+            else hipLaunchKernelGGL((k_history_split<R, Philox, false>), dim3(grid), dim3(block), 0, st, a);
+        }
+        return hipGetLastError();
+    }
     if (rng == RNG_REPLAY) {
         if (count) hipLaunchKernelGGL((k_history<R, Replay, true>), dim3(grid), dim3(block), 0, st, a);
         else hipLaunchKernelGGL((k_history<R, Replay, false>), dim3(grid), dim3(block), 0, st, a);

@@ -5,7 +5,8 @@ random numbers, reproduces the reference's outputs (FP64 within 1e-6
 relative; FP32 within a stated float tolerance) with the same number of draws.
 Level 2: whole runs of the plain history kernel in FP64, each history fed its
 own slice of the reference's RANMAR stream, reproduce the oracle's per-voxel
-energy (the oracle equals the reference bit for bit)."""
+energy (the oracle equals the reference bit for bit), with and without photon
+splitting."""
 import numpy as np
 import pytest
 
@@ -121,7 +122,7 @@ def _replay_inputs(session, nh, nb):
     return s1, s2, stream, off
 
 
-@pytest.mark.parametrize("name", ["w20", "small", "spec", "electron", "positron", "slab"])
+@pytest.mark.parametrize("name", ["w20", "small", "spec", "electron", "positron", "slab", "split", "split_slab"])
 def test_history_replay_fp64_matches_oracle(tmp_path, name):
     """Plain history kernel, FP64, each history on its own slice of the
     reference's RANMAR stream: per-region batch sums agree with the oracle."""
@@ -137,7 +138,7 @@ def test_history_replay_fp64_matches_oracle(tmp_path, name):
         if kw.get("spectrum") else omc.Spectrum.mono(kw["energy"])
     g1, g2, st = c.run_dosxyz(ph, spec, nh // nb, nb, collimator=kw.get("collimator", (-5, 5, -5, 5)),
                               charge=kw.get("charge", 0), precision=omc.FP64, form=omc.FORM_HISTORY,
-                              rng=omc.RNG_REPLAY, replay=stream, replay_offsets=off)
+                              rng=omc.RNG_REPLAY, replay=stream, replay_offsets=off, nsplit=kw.get("nsplit", 1))
     c.close()
     assert st.errors & 1 == 0
     inside = s1[1:] > 0
