@@ -40,6 +40,8 @@ def lib():
         L.orc_session_matrad.argtypes = [vp, C.c_int, C.c_double, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int,
                                          C.c_double, ip, C.c_int, dp, dp, ip]
         L.orc_session_matrad_finish.argtypes = [vp, dp, dp, C.c_int, dp, dp]
+        L.orc_session_matrad_source.argtypes = [vp, C.c_int, C.c_double, dp, dp, C.c_int, ip, dp, dp, dp, dp,
+                                                C.c_int, C.c_double, C.c_int, C.c_int, dp]
         L.orc_session_matrad_omp.restype = C.c_double
         L.orc_session_matrad_omp.argtypes = [vp, C.c_int, C.c_double, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int,
                                              C.c_double, ip, C.c_int, C.c_int, dp, dp]
@@ -158,6 +160,22 @@ class Session:
         if rc:
             raise RuntimeError(f"oracle matrad run failed: {rc}")
         return (s, s2, cnt) if want_counts else (s, s2)
+
+    def matrad_source(self, spectrum, bix, ib, n):
+        """n primaries of beamlet ib: rows iq ir e x y z u v w wt dnear."""
+        out = np.zeros((n, 11))
+        sp = spectrum.cdfinv1 is not None
+        c1 = np.ascontiguousarray(spectrum.cdfinv1 if sp else np.zeros(1000))
+        c2 = np.ascontiguousarray(spectrum.cdfinv2 if sp else np.zeros(1000))
+        arr = {k: np.ascontiguousarray(bix[k], np.float64) for k in ("src", "corner", "side1", "side2")}
+        ibm = np.ascontiguousarray(bix["ibeam"], np.int32)
+        rc = lib().orc_session_matrad_source(self.h, int(sp), float(spectrum.energy), _dp(c1), _dp(c2), len(ibm),
+                                             _ip(ibm), _dp(arr["src"]), _dp(arr["corner"]), _dp(arr["side1"]),
+                                             _dp(arr["side2"]), int(bix.get("gaussian", 0)),
+                                             float(bix.get("sigma", 0.0)), int(ib), int(n), _dp(out))
+        if rc:
+            raise RuntimeError(f"oracle source sampling failed: {rc}")
+        return out
 
     def matrad_omp(self, spectrum, bix, nper, nbatch, nthreads=0):
         """OpenMP matRad transport (CPU timing); returns (sum, sum2, seconds)."""

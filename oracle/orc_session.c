@@ -305,3 +305,31 @@ double orc_session_matrad_omp(orc_session *s, int spectrum, double energy, const
     return orc_run_matrad_omp(&s->t, &s->r, &s->g, &sp, &bl, s->nsplit, s->ixx, s->jxx, nper, nbatch, nthreads, sum,
                               sum2);
 }
+
+/* Samples n primaries of beamlet ib on the session's RANMAR stream with the
+ * matRad source (omc_matrad.c:1177-1413); out[11*n]: iq ir e x y z u v w wt
+ * dnear per primary. */
+int orc_session_matrad_source(orc_session *s, int spectrum, double energy, const double *cdf1, const double *cdf2,
+                              int nbix, const int *ibeam, const double *src, const double *corner,
+                              const double *side1, const double *side2, int gaussian, double sigma, int ib, int n,
+                              double *out) {
+    omc_spectrum sp;
+    memset(&sp, 0, sizeof sp);
+    if (spectrum) {
+        sp.spectrum = 1;
+        sp.deltak = OMC_INVDIM;
+        memcpy(sp.cdfinv1, cdf1, sizeof sp.cdfinv1);
+        memcpy(sp.cdfinv2, cdf2, sizeof sp.cdfinv2);
+    } else {
+        omc_spectrum_mono(&sp, energy);
+    }
+    orc_beamlets bl = {nbix, ibeam, src, corner, side1, side2, gaussian, sigma};
+    orc *o = orc_create(&s->t, &s->r, &s->g, s->nsplit, s->ixx, s->jxx);
+    if (!o) return OMC_ERR_NOMEM;
+    for (int i = 0; i < n; i++) {
+        orc_init_history_matrad(o, &sp, &bl, ib);
+        orc_get_particle(o, 0, out + 11 * (long)i);
+    }
+    orc_destroy(o);
+    return OMC_OK;
+}

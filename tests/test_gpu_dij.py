@@ -29,9 +29,9 @@ def _case(tmp_path, nbeams=2, radius=1.0, nhist=(150, 400), seed=3):
     return P.read_egsphant(phf), bix, stem
 
 
-def _bix_dict(bix):
+def _bix_dict(bix, gaussian=0, sigma=0.0):
     return dict(ibeam=bix.ibeam, src=bix.src.reshape(-1), corner=bix.corner.reshape(-1),
-                side1=bix.side1.reshape(-1), side2=bix.side2.reshape(-1), gaussian=0, sigma=0.0)
+                side1=bix.side1.reshape(-1), side2=bix.side2.reshape(-1), gaussian=gaussian, sigma=sigma)
 
 
 def _nper(bix, nbatch):
@@ -61,14 +61,14 @@ def _compare_csc(g, o, what):
     assert frac >= 0.99, f"{what}: {frac:.4f} of entries within 1e-9, worst {rel.max():.3g}"
 
 
-@pytest.mark.parametrize("group", [1, 3])
-def test_dij_replay_fp64_matches_oracle(tmp_path, group):
-    nbatch, rel = 4, 1e-4
+@pytest.mark.parametrize("group,gaussian", [(1, 0), (3, 0), (3, 1)])
+def test_dij_replay_fp64_matches_oracle(tmp_path, group, gaussian):
+    nbatch, rel, sigma = 4, 1e-4, 0.3
     ph_h, bix, stem = _case(tmp_path)
     sess = B.Session(stem)
     spec = omc.Spectrum.read(os.path.join(omc.DATA, "spectra", "mohan6.spectrum"))
     nper = _nper(bix, nbatch)
-    s, s2, cnt = sess.matrad(spec, _bix_dict(bix), nper, nbatch, want_counts=True)
+    s, s2, cnt = sess.matrad(spec, _bix_dict(bix, gaussian, sigma), nper, nbatch, want_counts=True)
     jc, ir, val, var = sess.matrad_dij(s, s2, nbatch, rel)
     off = np.zeros(len(cnt) + 1, np.int64)
     off[1:] = np.cumsum(cnt)
@@ -77,7 +77,7 @@ def test_dij_replay_fp64_matches_oracle(tmp_path, group):
     ph = c.phantom(ph_h, ecut=0.7, pcut=0.01)
     (g, gv, _, st) = c.run_dij(ph, spec, bix, nbatch=nbatch, rel_threshold=rel, want_var=True,
                                precision=omc.FP64, form=omc.FORM_HISTORY, group=group, rng=omc.RNG_REPLAY,
-                               replay=stream, replay_offsets=off)
+                               replay=stream, replay_offsets=off, gaussian=gaussian, sigma=sigma)
     c.close()
     assert st.errors == 0
     assert st.group_size == min(group, bix.n)

@@ -10,6 +10,7 @@ from tests.conftest import DOSE_CASES, golden, write_case_inp
 from tests.golden import golden_io
 
 B = pytest.importorskip("oracle.bindings")
+omc = pytest.importorskip("ompmc_amd")
 
 RM = 0.5109989461
 
@@ -178,3 +179,77 @@ def test_drange_matches_quadrature(small_session):
         f = 1.0 / (c1[i] * np.log(e) + c0[i])
         quad = float(np.sum(0.5 * (f[1:] + f[:-1]) * np.diff(e)))
         assert out[0] == pytest.approx(quad, rel=2e-6, abs=1e-12)
+
+
+# ---------------------------------------------------------------- matRad source
+def _matrad_session(tmp_path):
+    P = omc.phantoms
+    ph = P.synthetic_ct(nx=20, ny=20, nz=12, vx=0.5, vy=0.5, vz=0.5, seed=3, target_radius=1.0)
+    phf = str(tmp_path / "ct.egsphant")
+    P.write_egsphant(phf, ph)
+    d = P.default_inp(phf, str(tmp_path) + "/", ncase=1000, nbatch=10)
+    return B.Session(P.write_inp(str(tmp_path / "ct"), d)[:-4]), ph
+
+
+@pytest.mark.parametrize("gaussian", [0, 1])
+def test_matrad_source_geometry(tmp_path, gaussian):
+    """The restated matRad source (omc_matrad.c:1177-1413) against the geometry
+    it must satisfy: every primary starts on the phantom surface facing the
+    source, moves along the line from the focal spot through a point of its
+    bixel, that point is uniform over the bixel rectangle, and the focal spot
+    offsets along the two bixel side directions are normal with the given
+    width.  Recovers the bixel point and spot offset by intersecting each
+    start ray with the bixel plane and the source plane."""
+    sess, ph = _matrad_session(tmp_path)
+    P = omc.phantoms
+    bix = P.beamlets(nbeams=5, radius=1.0, seed=4)
+    sigma = 2.0
+    bd = dict(ibeam=bix.ibeam, src=bix.src.reshape(-1), corner=bix.corner.reshape(-1),
+              side1=bix.side1.reshape(-1), side2=bix.side2.reshape(-1), gaussian=gaussian, sigma=sigma)
+    spec = omc.Spectrum.mono(2.0)
+    n = 4000
+    lo = np.array([ph.xb[0], ph.yb[0], ph.zb[0]])
+    hi = np.array([ph.xb[-1], ph.yb[-1], ph.zb[-1]])
+    for ib in (0, bix.n // 3, bix.n - 1):
+        out = sess.matrad_source(spec, bd, ib, n)
+        assert np.all(out[:, 0] == 0) and np.all(out[:, 2] == 2.0) and np.all(out[:, 9] == 1.0)
+        pos, d = out[:, 3:6], out[:, 6:9]
+        assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-12)
+        # on the surface: one coordinate on a face, all within the box
+        face = np.minimum(np.abs(pos - lo), np.abs(pos - hi)).min(axis=1)
+        assert np.all(face < 1e-9)
+        assert np.all(pos >= lo - 1e-9) and np.all(pos <= hi + 1e-9)
+        # voxel of the start point contains it (x fastest, 1-based region)
+        nx, ny = len(ph.xb) - 1, len(ph.yb) - 1
+        r = out[:, 1].astype(int) - 1
+        ix, iy, iz = r % nx, (r // nx) % ny, r // (nx * ny)
+        for b, i, c in ((ph.xb, ix, 0), (ph.yb, iy, 1), (ph.zb, iz, 2)):
+            assert np.all(pos[:, c] >= b[i] - 1e-9) and np.all(pos[:, c] <= b[i + 1] + 1e-9)
+        src = bix.src[bix.ibeam[ib]]
+        c0, s1, s2 = bix.corner[ib], bix.side1[ib], bix.side2[ib]
+        p1, p2 = s1 / np.linalg.norm(s1), s2 / np.linalg.norm(s2)
+        r12, g12 = np.zeros((n, 2)), np.zeros((n, 2))
+        for k in range(n):
+            # pos + t d = corner + r1 s1 + r2 s2   (bixel point on the ray)
+            a = np.column_stack([d[k], -s1, -s2])
+            t, r1, r2 = np.linalg.solve(a, c0 - pos[k])
+            r12[k] = (r1, r2)
+            # pos + t' d = src + g1 p1 + g2 p2     (focal spot on the ray)
+            a = np.column_stack([d[k], -p1, -p2])
+            t2, g1, g2 = np.linalg.solve(a, src - pos[k])
+            g12[k] = (g1, g2)
+            # the spot lies behind the entry point, the bixel (isocentre plane,
+            # inside the phantom) ahead of it
+            assert t > 0 and t2 < 0
+        assert np.all(r12 > -1e-9) and np.all(r12 < 1 + 1e-9)
+        assert np.all(np.abs(r12.mean(axis=0) - 0.5) < 4 * np.sqrt(1 / 12 / n))
+        assert np.all(np.abs(r12.var(axis=0) - 1 / 12) < 4 * np.sqrt(1 / 180 / n))
+        if gaussian:
+            assert np.all(np.abs(g12.mean(axis=0)) < 4 * sigma / np.sqrt(n))
+            assert np.all(np.abs(g12.std(axis=0) / sigma - 1) < 4 / np.sqrt(2 * n))
+            kurt = ((g12 - g12.mean(axis=0)) ** 4).mean(axis=0) / g12.var(axis=0) ** 2
+            assert np.all(np.abs(kurt - 3) < 0.5)
+            # the two offsets are independent (Box-Muller pair)
+            assert abs(np.corrcoef(g12[:, 0], g12[:, 1])[0, 1]) < 4 / np.sqrt(n)
+        else:
+            assert np.allclose(g12, 0.0, atol=1e-7)
