@@ -42,6 +42,13 @@ constexpr int MXU_MS = 31;
 constexpr int MXMED = 9;
 constexpr uint8_t VACUUM = 255;
 
+// Engine methods are inlined into the kernels: an out-of-line call takes the
+// particles, the random-number state and the tally by reference, which keeps
+// them in scratch memory (flat loads and stores) for the whole kernel.
+#ifndef OMC_INL
+#define OMC_INL __forceinline__
+#endif
+
 // ------------------------------------------------------------------ math
 template <class R> struct M;
 template <> struct M<double> {
@@ -85,6 +92,20 @@ template <> struct M<float> {
 // Piecewise-linear function of a log energy: value = lv * c1 + c0, one
 // interleaved pair per (medium, interval) so one 8/16-byte load serves a lookup.
 template <class R> struct PW { R c1, c0; };
+
+// Loads through pointers into device tables and phantom arrays, which live
+// in global memory: addressed as such they compile to global loads instead
+// of flat accesses.
+template <class T> __device__ __forceinline__ T gld(const T *p) {
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+template <class R> __device__ __forceinline__ PW<R> gld(const PW<R> *p) {
+    const __attribute__((address_space(1))) R *q = (const __attribute__((address_space(1))) R *)p;
+    PW<R> r;
+    r.c1 = q[0];
+    r.c0 = q[1];
+    return r;
+}
 
 template <class R> struct Med {
     R ge0, ge1, eke0, eke1;
@@ -234,17 +255,21 @@ template <class R, class G, class T> struct Engine {
 
     static __device__ __forceinline__ int pwlf_i(R lv, R c1, R c0) { return (int)(lv * c1 + c0); }
     static __device__ __forceinline__ R pwlf(const PW<R> &q, R lv) { return lv * q.c1 + q.c0; }
+    static __device__ __forceinline__ R pwlf(const PW<R> *tab, R lv) {
+        PW<R> q = gld(tab);
+        return lv * q.c1 + q.c0;
+    }
 
     __device__ __forceinline__ int region(const Part<R> &p) const {
         return p.ix < 0 ? 0 : 1 + p.ix + g.nx * (p.iy + g.ny * p.iz);
     }
     __device__ __forceinline__ int medium(const Part<R> &p) const {
         if (p.ix < 0) return -1;
-        uint8_t m = g.med[p.ix + g.nx * (p.iy + g.ny * p.iz)];
+        uint8_t m = gld(g.med + (p.ix + g.nx * (p.iy + g.ny * p.iz)));
         return m == VACUUM ? -1 : (int)m;
     }
     __device__ __forceinline__ R rhof_of(const Part<R> &p) const {
-        return p.ix < 0 ? (R)0 : g.rhof[p.ix + g.nx * (p.iy + g.ny * p.iz)];
+        return p.ix < 0 ? (R)0 : gld(g.rhof + (p.ix + g.nx * (p.iy + g.ny * p.iz)));
     }
     __device__ __forceinline__ R pcut_of(int imed) const { return imed < 0 ? (R)0 : g.pcut[imed]; }
     __device__ __forceinline__ R ecut_of(int imed) const { return imed < 0 ? (R)0 : g.ecut[imed]; }
@@ -265,24 +290,24 @@ template <class R, class G, class T> struct Engine {
         R dist;
         leaves = false;
         if (p.w > (R)0) {
-            dist = (g.zb[p.iz + 1] - p.z) / p.w;
+            dist = (gld(g.zb + (p.iz + 1)) - p.z) / p.w;
             if (dist < ustep) { ustep = dist; mv = 3; leaves = p.iz == g.nz - 1; }
         } else if (p.w < (R)0) {
-            dist = -(p.z - g.zb[p.iz]) / p.w;
+            dist = -(p.z - gld(g.zb + (p.iz))) / p.w;
             if (dist < ustep) { ustep = dist; mv = -3; leaves = p.iz == 0; }
         }
         if (p.u > (R)0) {
-            dist = (g.xb[p.ix + 1] - p.x) / p.u;
+            dist = (gld(g.xb + (p.ix + 1)) - p.x) / p.u;
             if (dist < ustep) { ustep = dist; mv = 1; leaves = p.ix == g.nx - 1; }
         } else if (p.u < (R)0) {
-            dist = -(p.x - g.xb[p.ix]) / p.u;
+            dist = -(p.x - gld(g.xb + (p.ix))) / p.u;
             if (dist < ustep) { ustep = dist; mv = -1; leaves = p.ix == 0; }
         }
         if (p.v > (R)0) {
-            dist = (g.yb[p.iy + 1] - p.y) / p.v;
+            dist = (gld(g.yb + (p.iy + 1)) - p.y) / p.v;
             if (dist < ustep) { ustep = dist; mv = 2; leaves = p.iy == g.ny - 1; }
         } else if (p.v < (R)0) {
-            dist = -(p.y - g.yb[p.iy]) / p.v;
+            dist = -(p.y - gld(g.yb + (p.iy))) / p.v;
             if (dist < ustep) { ustep = dist; mv = -2; leaves = p.iy == 0; }
         }
         return mv;
@@ -301,12 +326,12 @@ template <class R, class G, class T> struct Engine {
         if (count) cnt.nhownear++;
         if (p.ix < 0) return (R)0;
         R tp = (R)1.0E10;
-        tp = M<R>::fmin(tp, g.xb[p.ix + 1] - p.x);
-        tp = M<R>::fmin(tp, p.x - g.xb[p.ix]);
-        tp = M<R>::fmin(tp, g.yb[p.iy + 1] - p.y);
-        tp = M<R>::fmin(tp, p.y - g.yb[p.iy]);
-        tp = M<R>::fmin(tp, g.zb[p.iz + 1] - p.z);
-        tp = M<R>::fmin(tp, p.z - g.zb[p.iz]);
+        tp = M<R>::fmin(tp, gld(g.xb + (p.ix + 1)) - p.x);
+        tp = M<R>::fmin(tp, p.x - gld(g.xb + (p.ix)));
+        tp = M<R>::fmin(tp, gld(g.yb + (p.iy + 1)) - p.y);
+        tp = M<R>::fmin(tp, p.y - gld(g.yb + (p.iy)));
+        tp = M<R>::fmin(tp, gld(g.zb + (p.iz + 1)) - p.z);
+        tp = M<R>::fmin(tp, p.z - gld(g.zb + (p.iz)));
         return tp;
     }
 
@@ -370,8 +395,8 @@ template <class R, class G, class T> struct Engine {
     // ---------------------------------------------------------------- photon interactions
     // Rayleigh (src/ompmc.c:1120-1163): form-factor CDF of medium 0 with bin
     // index 0, as the reference reads them (quirk Q1)
-    __device__ void rayleigh(Part<R> &p, int imed, R eig, R gle, int lgle) {
-        R pmax = pwlf(t.pmax[imed * MXGE + lgle], gle);
+    __device__ OMC_INL void rayleigh(Part<R> &p, int imed, R eig, R gle, int lgle) {
+        R pmax = pwlf(t.pmax + (imed * MXGE + lgle), gle);
         R xmax = (R)80.65506856998 * eig;
         R xv, costhe, csqthe, r0, r1;
         do {
@@ -404,7 +429,7 @@ template <class R, class G, class T> struct Engine {
     }
 
     // pair production (src/ompmc.c:1447-1685)
-    __device__ void pair(int imed, Part<R> &p0, Part<R> &p1) {
+    __device__ OMC_INL void pair(int imed, Part<R> &p0, Part<R> &p1) {
         const Med<R> &m = t.med[imed];
         const R RM = (R)0.5109989461;
         R eig = p0.e, ese1, ese2;
@@ -503,7 +528,7 @@ template <class R, class G, class T> struct Engine {
     }
 
     // Klein-Nishina Compton (src/ompmc.c:1688-1801)
-    __device__ void compton(Part<R> &p0, Part<R> &p1) {
+    __device__ OMC_INL void compton(Part<R> &p0, Part<R> &p1) {
         const R RM = (R)0.5109989461;
         R eig = p0.e, ko = p0.e / RM;
         R broi = (R)1.0 + (R)2.0 * ko, bro = (R)1.0 / broi;
@@ -563,7 +588,7 @@ template <class R, class G, class T> struct Engine {
     }
 
     // photo-electric (src/ompmc.c:1804-1864); no binding energy (quirk Q3)
-    __device__ void photo(Part<R> &p) {
+    __device__ OMC_INL void photo(Part<R> &p) {
         const R RM = (R)0.5109989461;
         p.e += RM;
         p.iq = -1;
@@ -602,7 +627,7 @@ template <class R, class G, class T> struct Engine {
 
     // photon step (src/ompmc.c:1867-2139) for nsplit = 1: transport to the
     // next interaction (or out of the phantom / below PCUT), interact, return.
-    __device__ int photon(Part<R> &p, Part<R> &q) {
+    __device__ OMC_INL int photon(Part<R> &p, Part<R> &q) {
         const R RM = (R)0.5109989461;
         if (count) cnt.nphoton++;
         int imed = medium(p);
@@ -626,9 +651,9 @@ template <class R, class G, class T> struct Engine {
             if (imed != -1) {
                 const Med<R> &m = t.med[imed];
                 lgle = pwlf_i(gle, m.ge1, m.ge0) - 1;
-                R gmfpr0 = pwlf(t.gmfp[imed * MXGE + lgle], gle);
+                R gmfpr0 = pwlf(t.gmfp + (imed * MXGE + lgle), gle);
                 gmfp = gmfpr0 / rhof_of(p);
-                cohfac = pwlf(t.cohe[imed * MXGE + lgle], gle);
+                cohfac = pwlf(t.cohe + (imed * MXGE + lgle), gle);
                 gmfp *= cohfac;
                 tstep = gmfp * dpmfp;
             } else {
@@ -666,12 +691,12 @@ template <class R, class G, class T> struct Engine {
             return 0;
         }
         r = rnd();
-        R gbr1 = pwlf(t.gbr1[imed * MXGE + lgle], gle);
+        R gbr1 = pwlf(t.gbr1 + (imed * MXGE + lgle), gle);
         if (r <= gbr1 && eig > (R)2.0 * RM) {
             pair(imed, p, q);
             return 1;
         }
-        R gbr2 = pwlf(t.gbr2[imed * MXGE + lgle], gle);
+        R gbr2 = pwlf(t.gbr2 + (imed * MXGE + lgle), gle);
         if (r < gbr2) {
             compton(p, q);
             return 1;
@@ -706,7 +731,7 @@ template <class R, class G, class T> struct Engine {
     // reference exactly.  Same distribution of sites as the voxel-by-voxel
     // flight of photon() (not the same draws), without a dependent load per
     // voxel crossed.
-    __device__ int photon_delta(Part<R> &p, Part<R> &q) {
+    __device__ OMC_INL int photon_delta(Part<R> &p, Part<R> &q) {
         if (count) cnt.nphoton++;
         int imed = medium(p);
         R eig = p.e;
@@ -721,14 +746,14 @@ template <class R, class G, class T> struct Engine {
             if (rm <= (R)0) continue;
             const Med<R> &md = t.med[m];
             int lg = pwlf_i(gle, md.ge1, md.ge0) - 1;
-            R mu = rm / (pwlf(t.gmfp[m * MXGE + lg], gle) * pwlf(t.cohe[m * MXGE + lg], gle));
+            R mu = rm / (pwlf(t.gmfp + (m * MXGE + lg), gle) * pwlf(t.cohe + (m * MXGE + lg), gle));
             mu_max = M<R>::fmax(mu_max, mu);
         }
         if (!(mu_max > (R)0)) {          // nothing to interact with: leaves
             p.ix = -1;
             return -1;
         }
-        const R xlo = g.xb[0], xhi = g.xb[g.nx], ylo = g.yb[0], yhi = g.yb[g.ny], zlo = g.zb[0], zhi = g.zb[g.nz];
+        const R xlo = gld(g.xb + (0)), xhi = gld(g.xb + (g.nx)), ylo = gld(g.yb + (0)), yhi = gld(g.yb + (g.ny)), zlo = gld(g.zb + (0)), zhi = gld(g.zb + (g.nz));
         for (;;) {
             R rn = rnd();
             R s = -M<R>::log((R)1.0 - rn) / mu_max;
@@ -745,14 +770,14 @@ template <class R, class G, class T> struct Engine {
             p.iz = locate(g.zb, g.nz, p.z, g.inv_dz);
             if (count) cnt.ncross++;
             size_t v = p.ix + (size_t)g.nx * (p.iy + (size_t)g.ny * p.iz);
-            uint8_t mm = g.med[v];
+            uint8_t mm = gld(g.med + (v));
             R r2 = rnd();
             if (mm == VACUUM) continue;
             imed = (int)mm;
             const Med<R> &md = t.med[imed];
             int lgle = pwlf_i(gle, md.ge1, md.ge0) - 1;
-            R gmfp = pwlf(t.gmfp[imed * MXGE + lgle], gle) / g.rhof[v];
-            R cohfac = pwlf(t.cohe[imed * MXGE + lgle], gle);
+            R gmfp = pwlf(t.gmfp + (imed * MXGE + lgle), gle) / gld(g.rhof + (v));
+            R cohfac = pwlf(t.cohe + (imed * MXGE + lgle), gle);
             R mu = (R)1.0 / (gmfp * cohfac);
             if (r2 * mu_max <= mu) return interact(p, q, imed, eig, gle, lgle, cohfac);
         }
@@ -812,9 +837,9 @@ template <class R, class G, class T> struct Engine {
                     if (imed != -1) {
                         const Med<R> &m = t.med[imed];
                         lgle = pwlf_i(gle, m.ge1, m.ge0) - 1;
-                        R gmfpr0 = pwlf(t.gmfp[imed * MXGE + lgle], gle);
+                        R gmfpr0 = pwlf(t.gmfp + (imed * MXGE + lgle), gle);
                         gmfp = gmfpr0 / rhof_of(p);
-                        cohfac = pwlf(t.cohe[imed * MXGE + lgle], gle);
+                        cohfac = pwlf(t.cohe + (imed * MXGE + lgle), gle);
                         gmfp *= cohfac;
                         tstep = gmfp * dpmfp;
                     } else {
@@ -858,12 +883,12 @@ template <class R, class G, class T> struct Engine {
                 }
                 r = rnd();
                 int npold = np;
-                R gbr1 = pwlf(t.gbr1[imed * MXGE + lgle], gle);
+                R gbr1 = pwlf(t.gbr1 + (imed * MXGE + lgle), gle);
                 if (r <= gbr1 && eig > (R)2.0 * RM) {
                     pair(imed, st[np], st[np + 1]);
                     np += 1;
                 } else {
-                    R gbr2 = pwlf(t.gbr2[imed * MXGE + lgle], gle);
+                    R gbr2 = pwlf(t.gbr2 + (imed * MXGE + lgle), gle);
                     if (r < gbr2) {
                         compton(st[np], st[np + 1]);
                         np += 1;
@@ -907,7 +932,7 @@ template <class R, class G, class T> struct Engine {
     struct Mscats { int i, j; R omega2; };
 
     // spin rejection (src/ompmc.c:3115-3186)
-    __device__ R spin_rejection(int imed, int qel, R elke, R beta2, R q1, R cost, bool &spin_index, bool is_single,
+    __device__ OMC_INL R spin_rejection(int imed, int qel, R elke, R beta2, R q1, R cost, bool &spin_index, bool is_single,
                                 Spinr &sr) {
         if (spin_index) {
             spin_index = false;
@@ -949,11 +974,11 @@ template <class R, class G, class T> struct Engine {
         ak -= (R)k;
         const R *row = t.spin_rej + (size_t)imed * NSPIN_MED +
                        ((qel * (MXE_SPIN1 + 1) + sr.i) * (MXQ_SPIN + 1) + sr.j) * (MXU_SPIN + 1);
-        return ((R)1.0 - ak) * row[k] + ak * row[k + 1];
+        return ((R)1.0 - ak) * gld(row + (k)) + ak * gld(row + (k + 1));
     }
 
     // single elastic scattering (src/ompmc.c:3188-3217)
-    __device__ void sscat(int imed, int qel, R chia2, R elke, R beta2, R &cost, R &sint) {
+    __device__ OMC_INL void sscat(int imed, int qel, R chia2, R elke, R beta2, R &cost, R &sint) {
         bool spin_index = true;
         Spinr sr = {0, 0};
         R xi, rejf, rn;
@@ -970,7 +995,7 @@ template <class R, class G, class T> struct Engine {
     // multiple-scattering angle (src/ompmc.c:3624-3803).  Within a bin of the
     // q^(2+) surface the reference's interpolation step is du = 0 (it
     // subtracts the bin value from itself), so u is the tabulated value.
-    __device__ void mscat(int imed, int qel, bool &spin_index, bool &find_index, R elke, R beta2, R q1, R lambda,
+    __device__ OMC_INL void mscat(int imed, int qel, bool &spin_index, bool &find_index, R elke, R beta2, R q1, R lambda,
                           R chia2, R &cost, R &sint, Mscats &ms, Spinr &sr) {
         R xi, rejf, rn;
         R explambda = M<R>::exp(-lambda);
@@ -1052,9 +1077,9 @@ template <class R, class G, class T> struct Engine {
                 R ak = xi * (R)MXU_MS;
                 int k = (int)ak;
                 ak -= (R)k;
-                if (ak > t.wms[base + k]) k = t.ims[base + k];
-                R a = t.fms[base + k];
-                R u = t.ums[base + k];
+                if (ak > gld(t.wms + (base + k))) k = gld(t.ims + (base + k));
+                R a = gld(t.fms + (base + k));
+                R u = gld(t.ums + (base + k));
                 R du = (R)0;
                 xi = rnd();
                 if (M<R>::fabs(a) < (R)0.2) {
@@ -1074,7 +1099,7 @@ template <class R, class G, class T> struct Engine {
     }
 
     // two-substep multiple scattering, PRESTA-II style (src/ompmc.c:3805-3994)
-    __device__ R msdist(const Part<R> &p, int imed, int iq, R rhof, R de, R tustep, R eke, Part<R> &out) {
+    __device__ OMC_INL R msdist(const Part<R> &p, int imed, int iq, R rhof, R de, R tustep, R eke, Part<R> &out) {
         const R RM = (R)0.5109989461;
         const Med<R> &m = t.med[imed];
         int qel = (1 + iq) / 2;
@@ -1101,15 +1126,15 @@ template <class R, class G, class T> struct Engine {
         int ix = MXEKE * imed + lelke;
         R etap, xi_corr, gamma;
         if (qel == 0) {
-            etap = pwlf(t.etae[ix], elke);
-            xi_corr = pwlf(t.q1ce[ix], elke);
-            gamma = pwlf(t.q2ce[ix], elke);
+            etap = pwlf(t.etae + (ix), elke);
+            xi_corr = pwlf(t.q1ce + (ix), elke);
+            gamma = pwlf(t.q2ce + (ix), elke);
         } else {
-            etap = pwlf(t.etap[ix], elke);
-            xi_corr = pwlf(t.q1cp[ix], elke);
-            gamma = pwlf(t.q2cp[ix], elke);
+            etap = pwlf(t.etap + (ix), elke);
+            xi_corr = pwlf(t.q1cp + (ix), elke);
+            gamma = pwlf(t.q2cp + (ix), elke);
         }
-        R ms_corr = pwlf(t.blcce[ix], elke);
+        R ms_corr = pwlf(t.blcce + (ix), elke);
         chia2 *= etap;
         lambda /= (etap * ((R)1.0 + chia2));
         lambda *= ms_corr;
@@ -1173,10 +1198,10 @@ template <class R, class G, class T> struct Engine {
     }
 
     // CSDA path length inside one PWLF interval (src/ompmc.c:3997-4032)
-    __device__ R drange(int imed, int iq, int lelke, R ekei, R ekef, R elkei, R elkef) {
+    __device__ OMC_INL R drange(int imed, int iq, int lelke, R ekei, R ekef, R elkei, R elkef) {
         R fedep = (R)1.0 - ekef / ekei;
         R elktmp = (R)0.5 * (elkei + elkef + (R)0.25 * fedep * fedep * ((R)1.0 + fedep * ((R)1.0 + (R)0.875 * fedep)));
-        const PW<R> &q = iq < 0 ? t.ededx[MXEKE * imed + lelke] : t.pdedx[MXEKE * imed + lelke];
+        const PW<R> q = gld((iq < 0 ? t.ededx : t.pdedx) + (MXEKE * imed + lelke));
         R dedxmid = (R)1.0 / pwlf(q, elktmp);
         R aux = q.c1 * dedxmid;
         aux = aux * ((R)1.0 + (R)2.0 * aux) * fedep * fedep / ((R)6.0 * ((R)2.0 - fedep) * ((R)2.0 - fedep));
@@ -1184,15 +1209,15 @@ template <class R, class G, class T> struct Engine {
     }
 
     // sub-threshold energy loss over a path tustep (src/ompmc.c:4034-4126)
-    __device__ R eloss(int imed, int iq, R rhof, R tustep, R range, R eke, R elke, int lelke) {
+    __device__ OMC_INL R eloss(int imed, int iq, R rhof, R tustep, R range, R eke, R elke, int lelke) {
         const Med<R> &m = t.med[imed];
         int qel = (1 + iq) / 2;
         const R *rep = t.range_ep + ((size_t)qel * t.nmed + imed) * MXEKE;
         const PW<R> *dx = iq < 0 ? t.ededx : t.pdedx;
         R de, fedep;
-        R tuss = range - rep[lelke] / rhof;
+        R tuss = range - gld(rep + (lelke)) / rhof;
         if (tuss >= tustep) {
-            const PW<R> &q = dx[imed * MXEKE + lelke];
+            const PW<R> q = gld(dx + (imed * MXEKE + lelke));
             R dedxmid = pwlf(q, elke);
             R aux = q.c1 / dedxmid;
             de = dedxmid * tustep * rhof;
@@ -1204,11 +1229,11 @@ template <class R, class G, class T> struct Engine {
             if (tuss <= (R)0) {
                 de = eke - m.te * (R)0.99;
             } else {
-                while (l > 0 && tuss < rep[l]) l -= 1;
+                while (l > 0 && tuss < gld(rep + (l))) l -= 1;
                 R elktmp = ((R)(l + 2) - m.eke0) / m.eke1;
-                R eketmp = t.e_array[imed * MXEKE + l + 1];
-                tuss = (rep[l + 1] - tuss) / rhof;
-                const PW<R> &q = dx[MXEKE * imed + l];
+                R eketmp = gld(t.e_array + (imed * MXEKE + l + 1));
+                tuss = (gld(rep + (l + 1)) - tuss) / rhof;
+                const PW<R> q = gld(dx + (MXEKE * imed + l));
                 R dedxmid = pwlf(q, elktmp);
                 R aux = q.c1 / dedxmid;
                 de = dedxmid * tuss * rhof;
@@ -1237,7 +1262,7 @@ template <class R, class G, class T> struct Engine {
     }
 
     // two-photon annihilation at rest (src/ompmc.c:4129-4185)
-    __device__ void rannih(Part<R> &a, Part<R> &b) {
+    __device__ OMC_INL void rannih(Part<R> &a, Part<R> &b) {
         const R RM = (R)0.5109989461;
         R rn = rnd();
         R costhe = (R)2.0 * rn - (R)1;
@@ -1255,7 +1280,7 @@ template <class R, class G, class T> struct Engine {
     }
 
     // bremsstrahlung (src/ompmc.c:4188-4374)
-    __device__ void brems(Part<R> &pe, Part<R> &pg) {
+    __device__ OMC_INL void brems(Part<R> &pe, Part<R> &pg) {
         const R RM = (R)0.5109989461;
         int imed = medium(pe);
         const Med<R> &m = t.med[imed];
@@ -1339,7 +1364,7 @@ template <class R, class G, class T> struct Engine {
     }
 
     // Moller (src/ompmc.c:4377-4453); nothing happens at or below 2*TE (quirk Q4)
-    __device__ int moller(Part<R> &p0, Part<R> &p1) {
+    __device__ OMC_INL int moller(Part<R> &p0, Part<R> &p1) {
         const R RM = (R)0.5109989461;
         const Med<R> &m = t.med[medium(p0)];
         R eie = p0.e, ekin = eie - RM;
@@ -1369,7 +1394,7 @@ template <class R, class G, class T> struct Engine {
     }
 
     // Bhabha (src/ompmc.c:4456-4543)
-    __device__ void bhabha(Part<R> &p0, Part<R> &p1) {
+    __device__ OMC_INL void bhabha(Part<R> &p0, Part<R> &p1) {
         const R RM = (R)0.5109989461;
         const Med<R> &m = t.med[medium(p0)];
         R eip = p0.e, ekin = eip - RM, t0 = ekin / RM, e0 = t0 + (R)1.0;
@@ -1408,7 +1433,7 @@ template <class R, class G, class T> struct Engine {
     }
 
     // two-photon annihilation in flight (src/ompmc.c:4546-4663)
-    __device__ void annih(Part<R> &p0, Part<R> &p1) {
+    __device__ OMC_INL void annih(Part<R> &p0, Part<R> &p1) {
         const R RM = (R)0.5109989461;
         R avip = p0.e + RM, a = avip / RM;
         R g = a - (R)1.0, tt = g - (R)1.0, pp = M<R>::sqrt(a * tt);
@@ -1470,7 +1495,7 @@ template <class R, class G, class T> struct Engine {
 
     // electron below cut-off or discarded: deposit, annihilate a positron
     // (src/ompmc.c:4683-4705, 5079-5106)
-    __device__ int electron_stop(Part<R> &p, Part<R> &q, int iq, R eie, bool discard) {
+    __device__ OMC_INL int electron_stop(Part<R> &p, Part<R> &q, int iq, R eie, bool discard) {
         const R RM = (R)0.5109989461;
         R edep = (discard && iq > 0) ? p.e + RM : p.e - RM;
         ausgab(p, edep);
@@ -1493,7 +1518,7 @@ template <class R, class G, class T> struct Engine {
     // interaction (total_tstep) restarts at zero in every ustep iteration, so
     // a step that does not reach the interaction site is followed by a
     // zero-length step that resamples the distance.
-    __device__ int electron(Part<R> &p, Part<R> &q) {
+    __device__ OMC_INL int electron(Part<R> &p, Part<R> &q) {
         EState es = {(R)0, (R)0, 0};
         return electron_run(p, q, es, 0x7fffffff);
     }
@@ -1502,7 +1527,7 @@ template <class R, class G, class T> struct Engine {
     // with es.phase != 0 and the loop state saved; called again with that state
     // it continues exactly where the uninterrupted routine would.  Every other
     // exit leaves es.phase = 0.
-    __device__ int electron_run(Part<R> &p, Part<R> &q, EState &es, int budget) {
+    __device__ __forceinline__ int electron_run(Part<R> &p, Part<R> &q, EState &es, int budget) {
         const R RM = (R)0.5109989461;
         if (count) cnt.nelectron++;
         int imed = medium(p);
@@ -1536,8 +1561,8 @@ template <class R, class G, class T> struct Engine {
                 lelke = pwlf_i(elke, m.eke1, m.eke0) - 1;
                 int ix = imed * MXEKE + lelke;
                 if (m.sig_mono[qel]) {
-                    if (iq < 0) sig0 = pwlf(t.esig[ix], elke) / pwlf(t.ededx[ix], elke);
-                    else sig0 = pwlf(t.psig[ix], elke) / pwlf(t.pdedx[ix], elke);
+                    if (iq < 0) sig0 = pwlf(t.esig + (ix), elke) / pwlf(t.ededx + (ix), elke);
+                    else sig0 = pwlf(t.psig + (ix), elke) / pwlf(t.pdedx + (ix), elke);
                 } else {
                     sig0 = iq < 0 ? m.esig_e : m.psig_e;
                 }
@@ -1570,13 +1595,13 @@ template <class R, class G, class T> struct Engine {
                                 if (lelkef == lelke) {
                                     tstep = drange(imed, iq, lelke, eke, ekef, elke, elkef);
                                 } else {
-                                    R ekei = t.e_array[imed * MXEKE + lelke];
+                                    R ekei = gld(t.e_array + (imed * MXEKE + lelke));
                                     R elkei = ((R)(lelke + 1) - m.eke0) / m.eke1;
                                     R tuss = drange(imed, iq, lelke, eke, ekei, elke, elkei);
-                                    ekei = t.e_array[imed * MXEKE + lelkef + 1];
+                                    ekei = gld(t.e_array + (imed * MXEKE + lelkef + 1));
                                     elkei = ((R)(lelkef + 2) - m.eke0) / m.eke1;
                                     tstep = drange(imed, iq, lelkef, ekei, ekef, elkei, elkef);
-                                    tstep += tuss + rep[lelke] - rep[lelkef + 1];
+                                    tstep += tuss + gld(rep + (lelke)) - gld(rep + (lelkef + 1));
                                 }
                             }
                             total_tstep = tstep;
@@ -1585,22 +1610,22 @@ template <class R, class G, class T> struct Engine {
                         tstep = total_tstep / rhof;
                     }
                     int ix = imed * MXEKE + lelke;
-                    R dedx0 = iq < 0 ? pwlf(t.ededx[ix], elke) : pwlf(t.pdedx[ix], elke);
+                    R dedx0 = iq < 0 ? pwlf(t.ededx + (ix), elke) : pwlf(t.pdedx + (ix), elke);
                     R dedx = rhof * dedx0;
-                    R tmxs = pwlf(t.tmxs[ix], elke);
+                    R tmxs = pwlf(t.tmxs + (ix), elke);
                     tmxs /= rhof;
-                    R ekei = t.e_array[imed * MXEKE + lelke];
+                    R ekei = gld(t.e_array + (imed * MXEKE + lelke));
                     R elkei = ((R)(lelke + 1) - m.eke0) / m.eke1;
                     range = drange(imed, iq, lelke, eke, ekei, elke, elkei);
-                    range += rep[lelke];
+                    range += gld(rep + (lelke));
                     range /= rhof;
                     tustep = M<R>::fmin(M<R>::fmin(tstep, tmxs), range);
                     tperp = hownear(p);
                     R blccl = rhof * m.blcc, xccl = rhof * m.xcc;
                     R p2 = eke * (eke + (R)2.0 * RM);
                     R beta2 = p2 / (p2 + RM * RM);
-                    R etap = iq < 0 ? pwlf(t.etae[ix], elke) : pwlf(t.etap[ix], elke);
-                    R ms_corr = pwlf(t.blcce[ix], elke);
+                    R etap = iq < 0 ? pwlf(t.etae + (ix), elke) : pwlf(t.etap + (ix), elke);
+                    R ms_corr = pwlf(t.blcce + (ix), elke);
                     blccl = blccl / etap / ((R)1.0 + (R)0.25 * etap * xccl / blccl / p2) * ms_corr;
                     R ssmfp = beta2 / blccl;
                     R skindepth = (R)3 * ssmfp;
@@ -1678,8 +1703,8 @@ template <class R, class G, class T> struct Engine {
                         R chia2 = m.xcc / ((R)4.0 * m.blcc * p2);
                         R elkems = M<R>::log(ekems);
                         int lelkems = pwlf_i(elkems, m.eke1, m.eke0) - 1;
-                        R etap = iq < 0 ? pwlf(t.etae[MXEKE * imed + lelkems], elkems)
-                                        : pwlf(t.etap[MXEKE * imed + lelkems], elkems);
+                        R etap = iq < 0 ? pwlf(t.etae + (MXEKE * imed + lelkems), elkems)
+                                        : pwlf(t.etap + (MXEKE * imed + lelkems), elkems);
                         chia2 *= etap;
                         sscat(imed, qel, chia2, elkems, beta2, costhe, sinthe);
                     }
@@ -1724,14 +1749,14 @@ template <class R, class G, class T> struct Engine {
             } while (demfp >= (R)1.0E-5);
             if ((imed != medold) || (ustep == (R)0.0) || (imed == -1)) continue;
             int ix = imed * MXEKE + lelke;
-            R sigf = iq < 0 ? pwlf(t.esig[ix], elke) / pwlf(t.ededx[ix], elke)
-                            : pwlf(t.psig[ix], elke) / pwlf(t.pdedx[ix], elke);
+            R sigf = iq < 0 ? pwlf(t.esig + (ix), elke) / pwlf(t.ededx + (ix), elke)
+                            : pwlf(t.psig + (ix), elke) / pwlf(t.pdedx + (ix), elke);
             sigratio = sigf / sig0;
             rfict = rnd();
         } while (rfict >= sigratio);
         int ix = imed * MXEKE + lelke;
         if (iq < 0) {
-            R ebr1 = pwlf(t.ebr1[ix], elke);
+            R ebr1 = pwlf(t.ebr1 + (ix), elke);
             R rn = rnd();
             if (rn <= ebr1) {
                 brems(p, q);
@@ -1744,13 +1769,13 @@ template <class R, class G, class T> struct Engine {
             }
             return moller(p, q);
         }
-        R pbr1 = pwlf(t.pbr1[ix], elke);
+        R pbr1 = pwlf(t.pbr1 + (ix), elke);
         R rn = rnd();
         if (rn < pbr1) {
             brems(p, q);
             return 1;
         }
-        R pbr2 = pwlf(t.pbr2[ix], elke);
+        R pbr2 = pwlf(t.pbr2 + (ix), elke);
         if (rn < pbr2) bhabha(p, q);
         else annih(p, q);
         return 1;

@@ -151,11 +151,12 @@ __device__ __forceinline__ void flush_counters(unsigned long long *dst, const Co
     }
 }
 
-// The charged-particle kernel needs about 250 VGPRs unconstrained (2 waves
-// per SIMD); capping it at 128 (4 waves) spills some state to scratch and
-// still runs faster, as measured on gfx950.
+// Occupancy hint of the charged-particle kernel (waves per SIMD).  With the
+// whole engine inlined the kernel keeps its state in registers; capping it at
+// 4 or 3 waves spills that state to scratch, and on gfx950 2 waves measured
+// fastest (C3 subset: 6.0 M histories/s at 2, 5.9 M at 3, 4.9 M at 4).
 #ifndef OMC_CHARGED_WAVES_PER_EU
-#define OMC_CHARGED_WAVES_PER_EU 4
+#define OMC_CHARGED_WAVES_PER_EU 2
 #endif
 
 // Persistent transport of one particle type.  Every lane holds one particle
