@@ -247,6 +247,7 @@ template <class R, class G, class T> struct Engine {
     Counters cnt;
     bool count;
     int nsplit = 1;                 // photon splitting factor (vrt.nsplit)
+    int aborted = 0;                // set when a particle exceeded an iteration guard (it is dropped)
 
     __device__ __forceinline__ R rnd() {
         if (count) cnt.nrn++;
@@ -754,7 +755,12 @@ template <class R, class G, class T> struct Engine {
             return -1;
         }
         const R xlo = gld(g.xb + (0)), xhi = gld(g.xb + (g.nx)), ylo = gld(g.yb + (0)), yhi = gld(g.yb + (g.ny)), zlo = gld(g.zb + (0)), zhi = gld(g.zb + (g.nz));
-        for (;;) {
+        for (int iters = 0;; iters++) {
+            if (iters > (1 << 24)) {     // no progress: dropped, reported through `aborted`
+                aborted = 1;
+                p.ix = -1;
+                return -1;
+            }
             R rn = rnd();
             R s = -M<R>::log((R)1.0 - rn) / mu_max;
             if (count) cnt.nhowfar++;
@@ -1545,7 +1551,26 @@ template <class R, class G, class T> struct Engine {
             lelke = pwlf_i(elke, t.med[imed].eke1, t.med[imed].eke0) - 1;
         }
         R sigratio = (R)0, rfict = (R)0;
+        bool first = true;
+        int iters = 0;
         do {
+            // Every pass of this loop and of the step loop below counts
+            // against the budget; the start of a later pass is the state of a
+            // fresh call (everything read below is recomputed first), so the
+            // routine may yield here with phase 0.  A call that makes no
+            // progress for 2^24 passes is abandoned: the particle is dropped
+            // and `aborted` reports it.
+            if (!first) {
+                if (--budget <= 0) {
+                    es.phase = 0;
+                    return 0;
+                }
+                if (++iters > (1 << 24)) {
+                    aborted = 1;
+                    return -1;
+                }
+            }
+            first = false;
             bool compute_tstep = resume ? resume_compute : true;
             R eke = eie - RM, demfp = (R)0, sig0 = (R)0, ustep = (R)0;
             if (resume) {
@@ -1740,6 +1765,10 @@ template <class R, class G, class T> struct Engine {
                 R tt0 = total_tstep;
                 total_tstep -= tvstep * rhof;
                 if (total_tstep < M<R>::reach_tol(tt0)) demfp = (R)0.0;
+                if (++iters > (1 << 24)) {
+                    aborted = 1;
+                    return -1;
+                }
                 if (demfp >= (R)1.0E-5 && --budget <= 0) {
                     es.demfp = demfp;
                     es.sig0 = sig0;

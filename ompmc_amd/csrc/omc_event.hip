@@ -173,6 +173,8 @@ int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int 
     js.ev_charged += done->events[1];
     js.rounds += (uint64_t)(done->done_round >= 0 ? done->done_round : done->rounds);
     js.peak = std::max<uint64_t>(js.peak, done->peak);
+    if (done->err & 2)
+        return omc_seterr(OMC_ERR_LIMIT, "a particle made no progress within its iteration guard and was dropped");
     if (done->err & 1) {
         omc_seterr(OMC_ERR_STACK, "event pool overflow (target %lld, capacity %lld)", S.target, S.cap);
         return RC_POOL_OVERFLOW;

@@ -71,7 +71,7 @@ struct EvCtl {
     unsigned long long peak;          // largest number of particles in flight
     unsigned long long claim[2][2];   // [round parity][photon, charged] queue entries claimed by lanes
     int inj_dest;                     // pool of the primaries: 0 photons, 1 charged
-    int err;                          // bit 0: pool overflow
+    int err;                          // bit 0: pool overflow; bit 1: a particle exceeded an iteration guard
 };
 
 template <class R> struct EvArgs {
@@ -181,9 +181,11 @@ k_event(EvArgs<R> a) {
     PhiloxAt rng;
     TallyAtomic tally;
     tally.init(a.endep);
+    int iters = 0;
     for (;;) {
         long long idx = wave_reserve(a.claim, !have);
         if (!have && idx >= 0 && idx < n) {
+            iters = 0;
             load(a.in, idx, p);
             stream = a.in.stream[idx];
             slot = a.in.slot[idx];
@@ -213,6 +215,12 @@ k_event(EvArgs<R> a) {
                 phase = es.phase;
             }
             qpos = rng.n;
+            // a particle that made no progress within its guards is dropped
+            // and reported (EvCtl::err bit 1), never transported forever
+            if (e.aborted || ++iters > (1 << 22)) {
+                atomicOr(a.err, 2);
+                d = -1;
+            }
             // d: -1 particle gone, 0 particle continues, +1 continues with a secondary
             done = d < 0 || ((p.iq == 0) != PHOTON);
             if (done) tally.flush();

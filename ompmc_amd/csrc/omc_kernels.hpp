@@ -235,6 +235,7 @@ __global__ void __launch_bounds__(256) k_history(HistArgs<R> a) {
             long guard = 0;
             for (;;) {
                 int d = e.step(p, q);
+                if (e.aborted) { atomicOr(a.err + 2, 1); break; }
                 if (d < 0) {
                     if (depth == 0) break;
                     p = deep[--depth];
@@ -308,6 +309,7 @@ __global__ void __launch_bounds__(256) k_history_split(HistArgs<R> a) {
                     }
                 }
                 if (err) break;
+                if (e.aborted) { atomicOr(a.err + 2, 1); break; }
                 if (++guard > 100000000L) { atomicOr(a.err + 2, 1); break; }
             }
             if (err) atomicOr(a.err, 1);

@@ -132,22 +132,24 @@ def ct():
 
 def test_fp32_dose_within_half_percent_of_fp64(ct):
     """FP32 transport against the FP64 GPU run on the synthetic CT: the dose
-    summed over voxels above 50% of the maximum (region fixed by an
+    summed over voxels above 20% of the maximum (region fixed by an
     independent pilot run) differs by less than 0.5%; eight independent seeds
-    per precision put the statistical uncertainty of the difference near 0.15%."""
+    of 500k histories per precision put the statistical uncertainty of the
+    difference near 0.15%."""
     c, ph = ct
     spec = omc.Spectrum.read(omc.os.path.join(omc.DATA, "spectra", "mohan6.spectrum"))
 
-    def run(prec, seed, n=200000):
+    def run(prec, seed, n=500000):
         return c.run_dosxyz(ph, spec, n // 4, 4, collimator=(-3, 3, -3, 3), precision=prec, form=omc.FORM_EVENT,
                             seed=seed)[0][1:]
     pilot = run(omc.FP64, 999)
-    sel = pilot > 0.5 * pilot.max()
+    sel = pilot > 0.2 * pilot.max()
     reg = {prec: np.array([run(prec, 200 + k)[sel].sum() for k in range(8)]) for prec in (omc.FP32, omc.FP64)}
     a, b = reg[omc.FP32], reg[omc.FP64]
     rel = a.mean() / b.mean() - 1.0
     sem = np.hypot(a.std(ddof=1), b.std(ddof=1)) / np.sqrt(8) / b.mean()
-    print(f"FP32 vs FP64 dose (voxels > 50% max): {rel:+.3e} +- {sem:.1e}")
+    print(f"FP32 vs FP64 dose (voxels > 20% max): {rel:+.3e} +- {sem:.1e}")
+    assert sem < 2.5e-3
     assert abs(rel) < 5e-3
 
 
