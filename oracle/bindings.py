@@ -40,6 +40,7 @@ def lib():
         L.orc_session_matrad.argtypes = [vp, C.c_int, C.c_double, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int,
                                          C.c_double, ip, C.c_int, dp, dp, ip]
         L.orc_session_matrad_finish.argtypes = [vp, dp, dp, C.c_int, dp, dp]
+        L.orc_session_write_3ddose.argtypes = [vp, C.c_char_p, dp, dp]
         L.orc_session_matrad_source.argtypes = [vp, C.c_int, C.c_double, dp, dp, C.c_int, ip, dp, dp, dp, dp,
                                                 C.c_int, C.c_double, C.c_int, C.c_int, dp]
         L.orc_session_matrad_omp.restype = C.c_double
@@ -130,6 +131,12 @@ class Session:
         dose, unc = np.zeros(nv), np.zeros(nv)
         lib().orc_session_finish(self.h, _dp(s), _dp(s2), nperbatch, nbatch, _dp(dose), _dp(unc))
         return dose, unc
+
+    def write_3ddose(self, path, dose, unc):
+        rc = lib().orc_session_write_3ddose(self.h, path.encode(), _dp(np.ascontiguousarray(dose)),
+                                            _dp(np.ascontiguousarray(unc)))
+        if rc:
+            raise RuntimeError(f"3ddose writer failed: {rc}")
 
     def ranmar(self, n):
         out = np.zeros(n, np.int32)

@@ -333,3 +333,9 @@ int orc_session_matrad_source(orc_session *s, int spectrum, double energy, const
     orc_destroy(o);
     return OMC_OK;
 }
+
+/* The .3ddose writer of the product's host layer (omc_host.c, reference
+ * omc_dosxyz.c:801-886) on the session's phantom. */
+int orc_session_write_3ddose(orc_session *s, const char *path, const double *dose, const double *unc) {
+    return omc_write_3ddose(path, &s->ph, dose, unc);
+}

@@ -253,3 +253,30 @@ def test_matrad_source_geometry(tmp_path, gaussian):
             assert abs(np.corrcoef(g12[:, 0], g12[:, 1])[0, 1]) < 4 / np.sqrt(n)
         else:
             assert np.allclose(g12, 0.0, atol=1e-7)
+
+
+# ---------------------------------------------------------------- file formats
+@pytest.mark.parametrize("name", ["w20", "spec"])
+def test_3ddose_file_equals_reference_output(tmp_path, name):
+    """The reference's own omc_dosxyz (serial build, one RANMAR stream) writes
+    <output folder><name>.3ddose; the oracle reproduces its dose bit for bit,
+    and the product's writer (omc_host.c) must then produce the same file byte
+    for byte (header, %5d / %f / %e formatting, boundaries, dose and
+    uncertainty blocks, omc_dosxyz.c:801-886)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(B.__file__), "_ref", "omc_dosxyz_serial")
+    if not os.path.exists(exe):
+        pytest.skip("reference binary not built (no reference tree)")
+    stem = write_case_inp(tmp_path, name)
+    _, nh, nb = DOSE_CASES[name]
+    r = subprocess.run([exe, "-i", stem, "-o", "ref"], cwd=str(tmp_path), stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:]
+    ref = (tmp_path / "ref.3ddose").read_bytes()
+    s = B.Session(stem)
+    s1, s2 = s.dose(nh // nb, nb)
+    d, u = s.finish(s1, s2, nh // nb, nb)
+    out = str(tmp_path / "ours.3ddose")
+    s.write_3ddose(out, d, u)
+    assert open(out, "rb").read() == ref
