@@ -293,7 +293,9 @@ def bench_c3(a):
                                           "seeds": 3}
         log(f"fp32/fp64: {d:+.3e} +- {sig:.1e}")
     if not a.no_serve:
-        out["serving"] = serve_mix(ctx, value, precision=prec)
+        loads = [float(x) for x in a.serve_loads.split(",")]
+        runs = [serve_mix(ctx, value, precision=prec, load=ld, nreq=a.serve_requests) for ld in loads]
+        out["serving"] = runs[0] if len(runs) == 1 else {"loads": runs}
         log(f"serving: {out['serving']}")
     cpu = None
     if not a.no_cpu:
@@ -381,6 +383,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--subset", type=int, default=1, help="c3: every n-th beamlet only (profiling runs)")
+    ap.add_argument("--serve-loads", default="0.8",
+                    help="offered loads of the served mix, as fractions of the batched throughput (e.g. 0.5,0.8,1.0)")
+    ap.add_argument("--serve-requests", type=int, default=12, help="requests per served mix")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device once (PMC scale of FETCH_SIZE / WRITE_SIZE) and exit")
     a = ap.parse_args()

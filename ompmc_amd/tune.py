@@ -1,0 +1,89 @@
+"""Autotuner of the event-queue form's launch parameters.
+
+The batched form has four knobs (``omc_run_opts``): the particles in flight
+per round (``event_target``), the electron steps between lane refills
+(``electron_budget``), the grid of the transport kernels (``grid``) and, for
+dose-influence runs, the beamlets per group (``group``, which bounds the
+tally memory).  ``tune_dij`` times candidate settings on a sample of the
+caller's own beamlets and returns the fastest as keyword arguments for
+``Context.run_dij``; results are cached per device and workload shape in a
+JSON file so a later process reuses them.
+
+    from ompmc_amd import tune
+    best = tune.tune_dij(ctx, phantom, spectrum, beamlets)
+    ctx.run_dij(phantom, spectrum, beamlets, **best)
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+
+import numpy as np
+
+from . import FORM_EVENT, FP32
+
+CACHE = os.environ.get("OMC_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "ompmc_amd_tune.json"))
+
+TARGETS = (262144, 524288, 1048576)
+BUDGETS = (4, 8, 16)
+
+
+def _sample(bix, n):
+    from .phantoms import Beamlets
+    sel = np.linspace(0, bix.n - 1, min(n, bix.n)).astype(int)
+    return Beamlets(bix.ibeam[sel], bix.src, bix.corner[sel], bix.side1[sel], bix.side2[sel],
+                    np.asarray(bix.nhist)[sel])
+
+
+def _key(phantom, bix, precision, sample):
+    nvox = phantom.nreg - 1
+    return f"dij/{precision}/nvox={nvox}/beamlets={min(sample, bix.n)}/hist={int(np.median(bix.nhist))}"
+
+
+def _load():
+    try:
+        with open(CACHE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(d):
+    try:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        with open(CACHE, "w") as f:
+            json.dump(d, f, indent=1)
+    except OSError:
+        pass
+
+
+def tune_dij(ctx, phantom, spectrum, bix, nbatch=10, precision=FP32, sample=200, targets=TARGETS, budgets=BUDGETS,
+             use_cache=True, verbose=False):
+    """Returns the fastest {event_target, electron_budget} (plus form and
+    precision) for ``Context.run_dij`` on a sample of ``bix``; every
+    candidate computes the same matrix (the random streams do not depend on
+    these settings), only the time differs."""
+    key = _key(phantom, bix, precision, sample)
+    cache = _load() if use_cache else {}
+    if key in cache:
+        return dict(cache[key]["opts"])
+    sub = _sample(bix, sample)
+    ctx.run_dij(phantom, spectrum, _sample(bix, 4), nbatch=nbatch, precision=precision, form=FORM_EVENT)
+    results = []
+    for tgt in targets:
+        for bud in budgets:
+            t0 = time.perf_counter()
+            _, _, _, st = ctx.run_dij(phantom, spectrum, sub, nbatch=nbatch, precision=precision, form=FORM_EVENT,
+                                      event_target=tgt, electron_budget=bud)
+            rate = st.histories / (time.perf_counter() - t0)
+            results.append((rate, tgt, bud))
+            if verbose:
+                print(f"tune: target {tgt} budget {bud}: {rate:.4g} histories/s")
+    rate, tgt, bud = max(results)
+    opts = dict(form=FORM_EVENT, precision=precision, event_target=tgt, electron_budget=bud)
+    if use_cache:
+        cache[key] = {"opts": opts, "histories_per_s": rate,
+                      "table": [dict(rate=r, event_target=t, electron_budget=b) for r, t, b in results]}
+        _store(cache)
+    return opts

@@ -131,3 +131,23 @@ def test_dij_grouping_and_forms_agree(tmp_path):
     z = np.abs(pd_h[sel] - pd_e[sel]) / np.sqrt(pv_h[sel] + pv_e[sel])
     print(f"event vs history plan dose: {sel.sum()} voxels, {np.mean(z <= 2):.3f} within 2 sigma")
     assert np.mean(z <= 2.0) >= 0.85 and np.mean(z <= 3.0) >= 0.97
+
+
+def test_autotuner_picks_a_measured_setting(tmp_path, monkeypatch):
+    """The tuner times its candidates on a sample of the caller's beamlets,
+    returns the fastest and caches it; the matrix does not depend on the
+    setting (random streams are keyed by beamlet, batch and history)."""
+    from ompmc_amd import tune
+    monkeypatch.setattr(tune, "CACHE", str(tmp_path / "tune.json"))
+    ph_h, bix, _ = _case(tmp_path, nhist=(500, 1000))
+    c = omc.Context(ph_h.media)
+    ph = c.phantom(ph_h, ecut=0.7, pcut=0.01)
+    spec = omc.Spectrum.mono(2.0)
+    best = tune.tune_dij(c, ph, spec, bix, nbatch=4, sample=8, targets=(4096, 65536), budgets=(2, 8))
+    assert best["event_target"] in (4096, 65536) and best["electron_budget"] in (2, 8)
+    assert (tmp_path / "tune.json").exists()
+    assert tune.tune_dij(c, ph, spec, bix, nbatch=4, sample=8) == best          # from the cache
+    a = c.run_dij(ph, spec, bix, nbatch=4, seed=5, **best)[0]
+    b = c.run_dij(ph, spec, bix, nbatch=4, seed=5, event_target=999, electron_budget=3)[0]
+    c.close()
+    _compare_csc(a, b, "tuned vs other setting")
