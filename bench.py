@@ -186,7 +186,8 @@ def bench_c3(a):
     gph = ctx.phantom(ph, ecut=0.7, pcut=0.01)
     log(f"c3: {bix.n} beamlets, {int(np.sum(np.maximum(bix.nhist // nbatch, 1)) * nbatch)} histories per step, "
         f"setup {time.perf_counter() - t_setup:.1f} s")
-    opts = dict(precision=prec, form=form, group=a.group, event_target=a.target, electron_budget=a.budget)
+    opts = dict(precision=prec, form=form, group=a.group, event_target=a.target, electron_budget=a.budget,
+                electron_turns=a.turns, batches_per_job=a.batches_per_job)
     # event counts of the transport kernels (algorithmic bytes per history)
     sub20 = subset(bix, 20)
     _, _, _, stc = ctx.run_dij(gph, spec, sub20, nbatch=nbatch, rel_threshold=rel, count_events=True, **opts)
@@ -209,7 +210,7 @@ def bench_c3(a):
             plan = pl
         st_last = st
         log(f"step {k}: {st.histories / walls[-1]:.4g} histories/s, {st.groups} groups of {st.group_size}, "
-            f"nnz {st.nnz}")
+            f"{st.batches_per_job} batches per job, nnz {st.nnz}")
     wall = sum(walls)
     value = hist / wall
     pd, pv = plan
@@ -241,7 +242,8 @@ def bench_c3(a):
         "config": {"workload": "c3: Dij of a 5-beam plan, 160x160x100 synthetic CT at 0.25x0.25x0.3 cm, "
                                f"{bix.n} beamlets of 0.5 cm, lognormal 1e3..5e4 histories each, 10 batches, "
                                "threshold 1e-4", "beamlets": int(bix.n), "histories_per_step": hist / a.steps,
-                   "form": a.form, "precision": a.precision, "group": int(st_last.group_size)},
+                   "form": a.form, "precision": a.precision, "group": int(st_last.group_size),
+                   "batches_per_job": int(st_last.batches_per_job)},
         "plan_dose_rel_unc_gt50": unc50,
         "nnz": int(st_last.nnz),
         "kernels": {KERNEL_NAMES[k]: {"seconds": float(ksec[k]), "launches": int(klaunch[k])}
@@ -376,6 +378,8 @@ def main():
     ap.add_argument("--form", default="event", choices=("event", "history"))
     ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64"))
     ap.add_argument("--group", type=int, default=0, help="beamlets per group (0: by HBM budget)")
+    ap.add_argument("--turns", type=int, default=0, help="electron refills per lane before requeue (0 auto, -1 never)")
+    ap.add_argument("--batches-per-job", type=int, default=0, help="batches per transport job (0: by HBM budget)")
     ap.add_argument("--target", type=int, default=0, help="event form: particles in flight (0: automatic)")
     ap.add_argument("--budget", type=int, default=0, help="event form: electron steps per event (0: automatic)")
     ap.add_argument("--no-cpu", action="store_true")

@@ -123,9 +123,16 @@ typedef struct omc_run_opts {
     int electron_budget;    /* event form: condensed-history steps per charged-particle event (0 = automatic) */
     int group;              /* Dij: beamlets transported together (0 = as many as hbm_budget allows,
                                1 = one beamlet at a time, the reference's loop order) */
-    int64_t hbm_budget;     /* device bytes a Dij run may use for its tallies (0 = automatic) */
+    int64_t hbm_budget;     /* device bytes a Dij run may use for its tallies (0 = half the free memory,
+                               at most 96 GiB); dose runs: a quarter of it bounds the batch tallies */
     int voxel_walk;         /* event form: 1 = photons fly voxel by voxel as the reference's photon()
                                does; 0 = delta tracking with the phantom's majorant (same distribution) */
+    int electron_turns;     /* event form: refills of electron_budget steps an electron may hold its lane
+                               within one round before it is requeued (0 = automatic, -1 = no limit) */
+    int batches_per_job;    /* event form: batches transported together, each into tallies of its own
+                               (0 = as many as the memory budget holds, 1 = one batch per job) */
+    int streams;            /* event-form Dij: beamlet groups transported concurrently, each on its own
+                               stream (0 = automatic: up to 3 when the run has several groups; 1..4) */
 } omc_run_opts;
 
 /* Kernel classes of omc_run_stats.kernel_seconds / kernel_launches */
@@ -151,6 +158,8 @@ typedef struct omc_run_stats {
     uint64_t events_photon, events_charged, rounds, peak_inflight;
     /* Dij: beamlet groups, beamlets per group, non-zeros */
     int64_t groups, group_size, nnz;
+    /* event form: batches transported together in one job; Dij: concurrent streams */
+    int64_t batches_per_job, streams;
 } omc_run_stats;
 
 /* Defaults: FP32, event form, Philox, seed 1802, nsplit 1, automatic launch. */
@@ -255,6 +264,7 @@ typedef struct omc_result {
     double histories;
 } omc_result;
 
+/* hbm_budget_bytes 0: 3/4 of the free device memory, at most 96 GiB */
 int omc_server_create(omc_ctx *ctx, int64_t hbm_budget_bytes, int max_inflight, omc_server **srv);
 int omc_server_submit(omc_server *srv, const omc_request *req, int64_t *req_id);
 int omc_server_poll(omc_server *srv, int64_t req_id, int *state);

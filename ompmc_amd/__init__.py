@@ -59,7 +59,8 @@ class RunOpts(C.Structure):
                 ("nsplit", C.c_int), ("replay", _u32), ("replay_offsets", _i64), ("replay_len", C.c_int64),
                 ("grid", C.c_int), ("block", C.c_int), ("count_events", C.c_int), ("event_target", C.c_int64),
                 ("electron_budget", C.c_int), ("group", C.c_int), ("hbm_budget", C.c_int64),
-                ("voxel_walk", C.c_int)]
+                ("voxel_walk", C.c_int), ("electron_turns", C.c_int), ("batches_per_job", C.c_int),
+                ("streams", C.c_int)]
 
 
 class RunStats(C.Structure):
@@ -71,7 +72,8 @@ class RunStats(C.Structure):
                 ("ev_howfar", C.c_uint64 * 2), ("ev_hownear", C.c_uint64 * 2), ("ev_cross", C.c_uint64 * 2),
                 ("ev_dep", C.c_uint64 * 2), ("events_photon", C.c_uint64), ("events_charged", C.c_uint64),
                 ("rounds", C.c_uint64), ("peak_inflight", C.c_uint64), ("groups", C.c_int64),
-                ("group_size", C.c_int64), ("nnz", C.c_int64)]
+                ("group_size", C.c_int64), ("nnz", C.c_int64), ("batches_per_job", C.c_int64),
+                ("streams", C.c_int64)]
 
     def as_dict(self):
         out = {}
@@ -180,7 +182,7 @@ def default_paths():
 
 def run_opts(precision=FP32, form=FORM_EVENT, seed=1802, rng=RNG_PHILOX, replay=None, replay_offsets=None,
              grid=0, block=0, count_events=False, nsplit=1, event_target=0, electron_budget=0, group=0,
-             hbm_budget=0, voxel_walk=False):
+             hbm_budget=0, voxel_walk=False, electron_turns=0, batches_per_job=0, streams=0):
     o = RunOpts()
     lib().omc_run_opts_default(C.byref(o))
     o.precision, o.form, o.seed, o.rng, o.nsplit = precision, form, seed, rng, nsplit
@@ -188,6 +190,7 @@ def run_opts(precision=FP32, form=FORM_EVENT, seed=1802, rng=RNG_PHILOX, replay=
     o.count_events = 2 if count_events == "time" else int(bool(count_events))
     o.event_target, o.electron_budget, o.group, o.hbm_budget = event_target, electron_budget, group, hbm_budget
     o.voxel_walk = int(bool(voxel_walk))
+    o.electron_turns, o.batches_per_job, o.streams = electron_turns, batches_per_job, streams
     keep = []
     if replay is not None:
         rp = np.ascontiguousarray(replay, np.uint32)

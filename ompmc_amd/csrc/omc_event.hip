@@ -8,7 +8,9 @@
 #include <cmath>
 #include <condition_variable>
 #include <deque>
+#include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 
 using namespace omc;
@@ -60,7 +62,12 @@ int device_cus(int dev) {
 
 // ------------------------------------------------------------------ event-form state
 constexpr int DEFAULT_BUDGET = 8;            // condensed-history steps between lane refills
-constexpr long MAX_ROUNDS = 2000000;         // a job that has not drained by then is reported as a step-guard error
+constexpr int DEFAULT_TURNS = 32;            // refills an electron may hold its lane in one round
+// Round guard of a job: the rounds its injection needs plus 2^16 -- with the
+// requeue after DEFAULT_TURNS x DEFAULT_BUDGET steps that is ~1.7e7 steps for
+// one particle, thousands of times the longest physical track.  A job that
+// has not drained by then is reported as a step-guard error.
+long max_rounds(long long nhist, long long target) { return 65536L + 8L * (long)(nhist / std::max(target, 1LL)); }
 
 template <class R> size_t pool_entry_bytes() { return sizeof(R) * 10 + sizeof(int) * 6 + 4 + 8; }
 
@@ -154,6 +161,7 @@ int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int 
         }
         js.n_source += K; js.n_photon += K; js.n_charged += K; js.n_other += K;
     };
+    const long rmax = max_rounds(nhist, S.target);
     long r = 0;
     int cur = 0;
     CK(enqueue(r, 0));
@@ -163,7 +171,7 @@ int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int 
         r += K;
         CK(hipEventSynchronize(chunk_done.ev[cur]));
         harvest(cur);
-        if (snap[cur].err || snap[cur].done_round >= 0 || r > MAX_ROUNDS) break;
+        if (snap[cur].err || snap[cur].done_round >= 0 || r > rmax) break;
         cur = 1 - cur;
     }
     CK(hipStreamSynchronize(st));
@@ -179,8 +187,23 @@ int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int 
         omc_seterr(OMC_ERR_STACK, "event pool overflow (target %lld, capacity %lld)", S.target, S.cap);
         return RC_POOL_OVERFLOW;
     }
-    if (done->done_round < 0) return omc_seterr(OMC_ERR_LIMIT, "event job did not drain within %ld rounds", MAX_ROUNDS);
+    if (done->done_round < 0) return omc_seterr(OMC_ERR_LIMIT, "event job did not drain within %ld rounds", rmax);
     return OMC_OK;
+}
+
+size_t default_budget() {
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) freeb = (size_t)8 << 30;
+    return freeb / 2;
+}
+
+// Batches of a dose run transported in one job: o->batches_per_job, else as
+// many as a quarter of the memory budget holds tallies for.
+int batches_per_job(const omc_run_opts *o, int nbatch, size_t tally_bytes) {
+    if (o->batches_per_job > 0) return std::min(o->batches_per_job, nbatch);
+    size_t budget = o->hbm_budget > 0 ? (size_t)o->hbm_budget : default_budget();
+    size_t n = budget / 4 / std::max<size_t>(tally_bytes, 1);
+    return (int)std::max<size_t>(1, std::min<size_t>(n, (size_t)nbatch));
 }
 
 template <class R> void fill_launch(omc_ctx *c, const PhantomDev *pd, EvState<R> &S, const omc_run_opts *o,
@@ -194,6 +217,7 @@ template <class R> void fill_launch(omc_ctx *c, const PhantomDev *pd, EvState<R>
     L.ctl = S.ctl;
     L.cap = S.cap;
     L.budget = o->electron_budget > 0 ? o->electron_budget : DEFAULT_BUDGET;
+    L.turns = o->electron_turns > 0 ? o->electron_turns : (o->electron_turns < 0 ? 0 : DEFAULT_TURNS);
     L.delta = o->voxel_walk ? 0 : 1;
     int cus = device_cus(c->device);
     L.grid_event = o->grid > 0 ? o->grid : cus * 8;
@@ -267,23 +291,34 @@ int omc_run_dosxyz_event(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const o
     DevArena A;
     ArenaGuard ag{A};
     EvState<R> S;
+    long nreg = pd->nreg;
+    // Batches transported together: each tallies into its own copy of the
+    // region array (slot = batch) and the batch statistics are taken in batch
+    // order afterwards, so the result is the batch-by-batch one while the
+    // job drains its longest tracks once instead of once per batch.
+    const int nbj = batches_per_job(o, nbatch, (size_t)nreg * sizeof(double));
     long long tgt = o->event_target > 0 ? o->event_target : default_target(c->device);
-    if (tgt > 4 * (long long)nper) tgt = std::max<long long>(4 * (long long)nper, 64);
+    if (tgt > 4 * (long long)nper * nbj) tgt = std::max<long long>(4 * (long long)nper * nbj, 64);
     int rc = S.alloc(tgt);
     if (rc) return rc;
-    long nreg = pd->nreg;
-    double *d_e = A.alloc_n<double>(nreg), *d_s = A.alloc_n<double>(nreg), *d_s2 = A.alloc_n<double>(nreg);
+    double *d_e = A.alloc_n<double>((size_t)nbj * nreg), *d_s = A.alloc_n<double>(nreg),
+           *d_s2 = A.alloc_n<double>(nreg);
+    long *d_soff = A.alloc_n<long>(nbj + 1);
     unsigned long long *d_cnt = A.alloc_n<unsigned long long>(16);
     if (!A.ok) return omc_seterr(OMC_ERR_NOMEM, "device allocation failed");
+    std::vector<long> soff(nbj + 1);
+    for (int k = 0; k <= nbj; k++) soff[k] = (long)k * nper;
+    CK(hipMemcpy(d_soff, soff.data(), sizeof(long) * (nbj + 1), hipMemcpyHostToDevice));
     EvLaunch<R> L;
     fill_launch(c, pd, S, o, L);
     rc = make_dosxyz_src<R>(A, pd, src, L.src.dos);
     if (rc) return rc;
     L.src.use_beamlets = 0;
+    L.src.slot_off = d_soff;
     L.endep = d_e;
-    L.slot_stride = 0;
+    L.slot_stride = nreg;
     L.counters = o->count_events ? d_cnt : nullptr;
-    CK(hipMemsetAsync(d_e, 0, sizeof(double) * nreg, stream));
+    CK(hipMemsetAsync(d_e, 0, sizeof(double) * nbj * nreg, stream));
     CK(hipMemsetAsync(d_s, 0, sizeof(double) * nreg, stream));
     CK(hipMemsetAsync(d_s2, 0, sizeof(double) * nreg, stream));
     CK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long) * 16, stream));
@@ -291,12 +326,14 @@ int omc_run_dosxyz_event(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const o
     JobStats js;
     Events ev(2);
     CK(hipEventRecord(ev.ev[0], stream));
-    for (int b = 0; b < nbatch; b++) {
-        L.src.hist_base = (long)b * nper;
-        rc = run_job_grow(stream, S, L, nper, dest, (o->count_events & 1) != 0, o->count_events != 0, js, d_e,
-                          (size_t)nreg);
+    for (int b0 = 0; b0 < nbatch; b0 += nbj) {
+        const int nb = std::min(nbj, nbatch - b0);
+        L.src.hist_base = (long)b0 * nper;       // streams stay keyed by the history's place in the run
+        L.src.nslot = nb;
+        rc = run_job_grow(stream, S, L, (long long)nper * nb, dest, (o->count_events & 1) != 0,
+                          o->count_events != 0, js, d_e, (size_t)nb * nreg);
         if (rc) return rc;
-        CK(launch_accumulate(d_e, d_s, d_s2, nreg, 1.0, stream));
+        for (int k = 0; k < nb; k++) CK(launch_accumulate(d_e + (size_t)k * nreg, d_s, d_s2, nreg, 1.0, stream));
     }
     CK(hipEventRecord(ev.ev[1], stream));
     CK(hipStreamSynchronize(stream));
@@ -307,6 +344,7 @@ int omc_run_dosxyz_event(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const o
         CK(hipMemcpy(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost));
         st->gpu_seconds = ev.ms(0, 1) * 1e-3;
         st->histories = (double)nper * nbatch;
+        st->batches_per_job = nbj;
         stats_counters(st, cnt);
         stats_from_job(st, js);
         st->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
@@ -334,7 +372,9 @@ std::vector<long> beamlet_nper(const omc_beamlets_desc *bix, int nbatch) {
     return v;
 }
 
-size_t dij_slot_bytes(long nreg) { return (size_t)3 * nreg * sizeof(double); }
+// Device bytes of one beamlet of a group whose jobs carry nbj batches: nbj
+// batch tallies plus the running sum and sum of squares.
+size_t dij_slot_bytes(long nreg, int nbj) { return (size_t)(nbj + 2) * nreg * sizeof(double) + (size_t)64 * nbj; }
 
 size_t dij_fixed_bytes(long nreg, int precision, int form, long long target) {
     size_t b = (size_t)3 * nreg * sizeof(double) + ((size_t)64 << 20);
@@ -343,24 +383,44 @@ size_t dij_fixed_bytes(long nreg, int precision, int form, long long target) {
     return b;
 }
 
-int pick_group(const omc_run_opts *o, long nreg, int nbix, int dev, long long target) {
-    if (o->group > 0) return std::min(o->group, nbix);
-    size_t budget = (size_t)o->hbm_budget;
-    if (!budget) {
-        size_t freeb = 0, totalb = 0;
-        if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) freeb = (size_t)8 << 30;
-        budget = std::min(freeb / 2, (size_t)48 << 30);
-    }
-    size_t fixed = dij_fixed_bytes(nreg, o->precision, o->form, target);
-    if (budget <= fixed + dij_slot_bytes(nreg)) return 1;
-    long g = (long)((budget - fixed) / (dij_slot_bytes(nreg) + 64));
-    (void)dev;
-    return (int)std::max(1L, std::min<long>(g, std::min(nbix, 4096)));
-}
-
-template <class R> struct DijDev {
-    SrcBeamlets<R> src;
+// Shape of a Dij run: G beamlets per group and nbj batches per transport job
+// (event form; the history form runs one batch per launch).  Every job ends
+// with the slowest track of its particles, so the plan takes the fewest jobs,
+// ceil(nbix / G) x ceil(nbatch / nbj), whose tallies fit the budget; the
+// options group and batches_per_job pin either factor.  Returns the bytes the
+// plan uses.
+struct DijPlan {
+    int G = 1, nbj = 1;
 };
+
+// Default memory budget of a Dij run's tallies: half the free device memory,
+// at most 96 GiB.  Past ~100 GiB held by the process, allocations cost
+// seconds to map and unmap (measured on the 1960-beamlet plan: 128-144 GiB
+// runs lost 2-5 s per call, 96 GiB none) while saving only a few jobs.
+size_t dij_default_budget() { return std::min(default_budget(), (size_t)96 << 30); }
+
+size_t dij_plan(const omc_run_opts *o, long nreg, int nbix, int nbatch, long long target, size_t budget, DijPlan &P) {
+    const bool event = o->form == OMC_FORM_EVENT;
+    if (!budget) budget = o->hbm_budget > 0 ? (size_t)o->hbm_budget : dij_default_budget();
+    const size_t fixed = dij_fixed_bytes(nreg, o->precision, o->form, target);
+    const size_t room = budget > fixed ? budget - fixed : 0;
+    const int gmax = std::min(nbix, 4096);
+    long best_jobs = -1;
+    int lo = 1, hi = event ? nbatch : 1;
+    if (event && o->batches_per_job > 0) lo = hi = std::min(o->batches_per_job, nbatch);
+    for (int nbj = hi; nbj >= lo; nbj--) {
+        long g = o->group > 0 ? std::min(o->group, nbix) : (long)(room / dij_slot_bytes(nreg, nbj));
+        g = std::max(1L, std::min<long>(g, gmax));
+        if (nbj > lo && (size_t)g * dij_slot_bytes(nreg, nbj) > room) continue;   // fewer batches per job
+        long jobs = (long)((nbix + g - 1) / g) * (long)((nbatch + nbj - 1) / nbj);
+        if (best_jobs < 0 || jobs < best_jobs) {
+            best_jobs = jobs;
+            P.G = (int)g;
+            P.nbj = nbj;
+        }
+    }
+    return fixed + (size_t)P.G * dij_slot_bytes(nreg, P.nbj);
+}
 
 template <class R>
 int upload_beamlets(DevArena &A, const omc_spectrum_desc *spec, const omc_beamlets_desc *b, SrcBeamlets<R> &S) {
@@ -384,240 +444,423 @@ int upload_beamlets(DevArena &A, const omc_spectrum_desc *spec, const omc_beamle
     return OMC_OK;
 }
 
-struct HostCSC {
-    std::vector<int64_t> jc, ir;
+// One lane of a Dij run: a stream with its own event pools, batch tallies,
+// finish and compaction buffers.  With two lanes, alternate beamlet groups
+// are transported concurrently, so the drain at the end of one group's job
+// (a few long electron tracks that leave most of the device idle) overlaps
+// the bulk of the next group's job on the other stream.  The finish of a
+// group reads and writes the carry of the previous beamlet's variance (a
+// behaviour of the reference kept for parity) and the plan totals, so group
+// finishes run in group order across the lanes, chained by events.
+template <class R> struct DijLane {
+    hipStream_t st = nullptr;
+    bool own_stream = false;
+    DevArena A, O;
+    EvState<R> S;
+    EvLaunch<R> L;
+    HistArgs<R> H;
+    double *d_e = nullptr, *d_s = nullptr, *d_s2 = nullptr, *d_bmax = nullptr, *d_thresh = nullptr, *d_nper = nullptr;
+    int *d_counts = nullptr, *d_slot_bix = nullptr, *d_slot_bt = nullptr;
+    int64_t *d_off = nullptr;
+    long *d_slot_off = nullptr;
+    size_t ocap = 0;
+    int64_t *d_ir = nullptr;
+    double *d_val = nullptr, *d_vval = nullptr;
+    hipEvent_t fin = nullptr;          // finish of the lane's latest group
+    JobStats js;
+    double t_hist = 0, t_other = 0, histories = 0;
+    uint64_t n_hist = 0, n_other = 0;
+    int rc = OMC_OK;
+    std::string msg;
+    ~DijLane() {
+        O.release();
+        A.release();
+        if (fin) (void)hipEventDestroy(fin);
+        if (own_stream && st) (void)hipStreamDestroy(st);
+    }
+};
+
+struct DijGroupOut {
+    std::vector<int64_t> collen, ir;
     std::vector<double> val, var;
 };
+
+template <class R> struct DijRun {
+    omc_ctx *c;
+    PhantomDev *pd;
+    const omc_run_opts *o;
+    int nbix, nbatch, G, NBJ, nblk, nchunk, ngroups;
+    long nreg, nvox;
+    bool event, count, timing, replay, want_var;
+    double rel;
+    long long tgt;
+    std::vector<long> nper, gfirst;
+    SrcBeamlets<R> bsrc;
+    double *d_carry, *d_tdose, *d_tvar;
+    unsigned long long *d_cnt, *d_next;
+    int *d_err;
+    uint32_t *d_rn;
+    long *d_rstart, *d_rend;
+    std::vector<DijLane<R> *> lanes;
+    std::vector<DijGroupOut> outs;
+    std::mutex mu;
+    std::condition_variable cv;
+    int next_finish = 0;
+    bool failed = false;
+};
+
+constexpr int RC_LANE_STOPPED = -101;     // a lane gave up because the other lane failed
+
+template <class R> int dij_lane_init(DijRun<R> &D, DijLane<R> &l) {
+    const int G = D.G;
+    const size_t nslot = (size_t)G * D.NBJ;
+    DevArena &A = l.A;
+    l.d_e = A.alloc_n<double>(nslot * D.nreg);
+    l.d_s = A.alloc_n<double>((size_t)G * D.nreg);
+    l.d_s2 = A.alloc_n<double>((size_t)G * D.nreg);
+    l.d_bmax = A.alloc_n<double>((size_t)G * D.nblk);
+    l.d_thresh = A.alloc_n<double>(G);
+    l.d_nper = A.alloc_n<double>(G);
+    l.d_counts = A.alloc_n<int>((size_t)G * D.nchunk);
+    l.d_off = A.alloc_n<int64_t>((size_t)G * D.nchunk);
+    l.d_slot_off = A.alloc_n<long>(nslot + 1);
+    l.d_slot_bix = A.alloc_n<int>(nslot);
+    l.d_slot_bt = A.alloc_n<int>(nslot);
+    if (!A.ok)
+        return omc_seterr(OMC_ERR_NOMEM, "Dij tallies for a group of %d beamlets x %d batches do not fit", G, D.NBJ);
+    CK(hipEventCreateWithFlags(&l.fin, hipEventDisableTiming));
+    CK(hipMemsetAsync(l.d_e, 0, sizeof(double) * nslot * D.nreg, l.st));
+    const omc_run_opts *o = D.o;
+    if (D.event) {
+        long long maxjob = 0;
+        for (int g0 = 0; g0 < D.nbix; g0 += G) {
+            long long t = 0;
+            for (int b = g0; b < std::min(D.nbix, g0 + G); b++) t += D.nper[b];
+            maxjob = std::max(maxjob, t * D.NBJ);
+        }
+        long long tgt = D.tgt;
+        if (tgt > 4 * maxjob) tgt = std::max<long long>(4 * maxjob, 64);
+        int rc = l.S.alloc(tgt);
+        if (rc) return rc;
+        fill_launch(D.c, D.pd, l.S, o, l.L);
+        l.L.src.bix = D.bsrc;
+        l.L.src.use_beamlets = 1;
+        l.L.src.slot_off = l.d_slot_off;
+        l.L.src.slot_beamlet = l.d_slot_bix;
+        l.L.src.slot_batch = l.d_slot_bt;
+        l.L.endep = l.d_e;
+        l.L.slot_stride = D.nreg;
+        l.L.counters = D.count ? D.d_cnt : nullptr;
+    } else {
+        HistArgs<R> &H = l.H;
+        memset(&H, 0, sizeof H);
+        H.t = tables_of<R>(D.c);
+        H.g = geom_of<R>(D.pd);
+        H.bix = D.bsrc;
+        H.use_beamlets = 1;
+        H.slot_off = l.d_slot_off;
+        H.slot_beamlet = l.d_slot_bix;
+        H.slot_stride = D.nreg;
+        H.seed = o->seed;
+        H.endep = l.d_e;
+        H.counters = D.count ? D.d_cnt : nullptr;
+        H.err = D.d_err;
+        H.next = D.d_next;
+        H.chunk = 1;
+        H.replay = D.d_rn;
+        H.replay_off = D.d_rstart;
+        H.replay_end = D.d_rend;
+        H.nsplit = o->nsplit;
+    }
+    return OMC_OK;
+}
+
+// Transport, batch statistics, finish (in group order) and compaction of
+// group gi on lane l; the group's columns land in D.outs[gi].
+template <class R> int dij_lane_group(DijRun<R> &D, DijLane<R> &l, int gi) {
+    const omc_run_opts *o = D.o;
+    const hipStream_t stream = l.st;
+    const long nreg = D.nreg, nvox = D.nvox;
+    const int nchunk = D.nchunk, g0 = gi * D.G, ng = std::min(D.G, D.nbix - g0);
+    std::vector<long> soff(ng + 1, 0);
+    std::vector<int> sbix(ng);
+    std::vector<double> snper(ng);
+    for (int s = 0; s < ng; s++) {
+        sbix[s] = g0 + s;
+        snper[s] = (double)D.nper[g0 + s];
+        soff[s + 1] = soff[s] + D.nper[g0 + s];
+    }
+    const long njob = soff[ng];
+    if (!D.event) {
+        CK(hipMemcpyAsync(l.d_slot_off, soff.data(), sizeof(long) * (ng + 1), hipMemcpyHostToDevice, stream));
+        CK(hipMemcpyAsync(l.d_slot_bix, sbix.data(), sizeof(int) * ng, hipMemcpyHostToDevice, stream));
+    }
+    CK(hipMemcpyAsync(l.d_nper, snper.data(), sizeof(double) * ng, hipMemcpyHostToDevice, stream));
+    CK(hipMemsetAsync(l.d_s, 0, sizeof(double) * ng * nreg, stream));
+    CK(hipMemsetAsync(l.d_s2, 0, sizeof(double) * ng * nreg, stream));
+    Events ev(2);
+    for (int bt = 0; bt < D.nbatch; bt += D.NBJ) {
+        const int nb = std::min(D.NBJ, D.nbatch - bt);
+        if (D.event) {
+            // one job for batches bt .. bt+nb-1 of the group: slot k * ng + s
+            // is beamlet g0 + s in batch bt + k, with that batch's stream keys
+            std::vector<long> voff((size_t)nb * ng + 1, 0);
+            std::vector<int> vbix((size_t)nb * ng), vbt((size_t)nb * ng);
+            for (int k = 0; k < nb; k++)
+                for (int sl = 0; sl < ng; sl++) {
+                    const size_t v = (size_t)k * ng + sl;
+                    vbix[v] = g0 + sl;
+                    vbt[v] = bt + k;
+                    voff[v + 1] = voff[v] + D.nper[g0 + sl];
+                }
+            CK(hipMemcpyAsync(l.d_slot_off, voff.data(), sizeof(long) * voff.size(), hipMemcpyHostToDevice, stream));
+            CK(hipMemcpyAsync(l.d_slot_bix, vbix.data(), sizeof(int) * vbix.size(), hipMemcpyHostToDevice, stream));
+            CK(hipMemcpyAsync(l.d_slot_bt, vbt.data(), sizeof(int) * vbt.size(), hipMemcpyHostToDevice, stream));
+            l.L.src.nslot = nb * ng;
+            l.L.src.batch = bt;
+            int rc = run_job_grow(stream, l.S, l.L, voff.back(), 0, D.count, D.timing, l.js, l.d_e,
+                                  (size_t)nb * ng * nreg);
+            if (rc) return rc;     // run_job synchronised the stream: the host vectors may go
+        } else {
+            HistArgs<R> &H = l.H;
+            H.nslot = ng;
+            H.batch = bt;
+            H.nhist = njob;
+            H.hist_base = 0;
+            if (D.replay) {
+                std::vector<long> rs(njob), re(njob);
+                for (int sl = 0; sl < ng; sl++)
+                    for (long i = 0; i < D.nper[g0 + sl]; i++) {
+                        long gh = D.gfirst[g0 + sl] + (long)bt * D.nper[g0 + sl] + i;
+                        rs[soff[sl] + i] = (long)o->replay_offsets[gh];
+                        re[soff[sl] + i] = (long)o->replay_offsets[gh + 1];
+                    }
+                CK(hipMemcpyAsync(D.d_rstart, rs.data(), sizeof(long) * njob, hipMemcpyHostToDevice, stream));
+                CK(hipMemcpyAsync(D.d_rend, re.data(), sizeof(long) * njob, hipMemcpyHostToDevice, stream));
+                CK(hipStreamSynchronize(stream));   // rs/re go out of scope
+            }
+            CK(hipMemsetAsync(D.d_next, 0, sizeof(unsigned long long), stream));
+            int block = o->block > 0 ? o->block : 256;
+            long want = (njob + block - 1) / block;
+            int grid = o->grid > 0 ? o->grid : (int)std::min<long>(want, 2048);
+            if (D.timing) CK(hipEventRecord(ev.ev[0], stream));
+            CK(launch_history<R>(H, D.replay ? RNG_REPLAY : RNG_PHILOX, D.count, grid, block, stream));
+            if (D.timing) {
+                CK(hipEventRecord(ev.ev[1], stream));
+                CK(hipEventSynchronize(ev.ev[1]));
+                l.t_hist += ev.ms(0, 1) * 1e-3;
+            }
+            l.n_hist++;
+        }
+        // batch statistics in batch order, as batch-by-batch jobs take them
+        for (int k = 0; k < nb; k++) {
+            CK(launch_accum_slots(l.d_e + (size_t)k * ng * nreg, l.d_s, l.d_s2, nreg, nreg, ng, l.d_nper, stream));
+            l.histories += (double)njob;
+        }
+    }
+    if (D.timing) CK(hipEventRecord(ev.ev[0], stream));
+    {
+        std::unique_lock<std::mutex> lk(D.mu);
+        D.cv.wait(lk, [&] { return D.failed || D.next_finish == gi; });
+        if (D.failed) return RC_LANE_STOPPED;
+    }
+    if (gi > 0) CK(hipStreamWaitEvent(stream, D.lanes[(gi - 1) % D.lanes.size()]->fin, 0));
+    int nb = 0;
+    CK(launch_dij_finish(l.d_s, l.d_s2, D.d_carry, ng, nreg, nvox, D.pd->dmass, D.pd->drho, D.nbatch, l.d_bmax,
+                         D.d_tdose, D.d_tvar, &nb, stream));
+    CK(hipEventRecord(l.fin, stream));
+    {
+        std::lock_guard<std::mutex> lk(D.mu);
+        D.next_finish = gi + 1;
+        D.cv.notify_all();
+    }
+    CK(launch_dij_select(l.d_s, nreg, nvox, ng, l.d_bmax, nb, D.rel, l.d_thresh, l.d_counts, stream));
+    std::vector<int> counts((size_t)ng * nchunk);
+    std::vector<int64_t> off((size_t)ng * nchunk);
+    CK(hipMemcpyAsync(counts.data(), l.d_counts, sizeof(int) * ng * nchunk, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    int64_t tot = 0;
+    for (size_t k = 0; k < counts.size(); k++) {
+        off[k] = tot;
+        tot += counts[k];
+    }
+    DijGroupOut &out = D.outs[gi];
+    out.collen.resize(ng);
+    for (int s = 0; s < ng; s++) {
+        int64_t end = off[(size_t)(s + 1) * nchunk - 1] + counts[(size_t)(s + 1) * nchunk - 1];
+        int64_t begin = s ? off[(size_t)s * nchunk - 1] + counts[(size_t)s * nchunk - 1] : 0;
+        out.collen[s] = end - begin;
+    }
+    if ((size_t)tot > l.ocap) {
+        DevArena &O = l.O;
+        O.release();
+        l.ocap = std::max<size_t>((size_t)tot, l.ocap * 2);
+        l.d_ir = O.alloc_n<int64_t>(l.ocap);
+        l.d_val = O.alloc_n<double>(l.ocap);
+        l.d_vval = D.want_var ? O.alloc_n<double>(l.ocap) : nullptr;
+        if (!O.ok) return omc_seterr(OMC_ERR_NOMEM, "Dij output buffers (%lld entries)", (long long)tot);
+    }
+    if (tot > 0) {
+        CK(hipMemcpyAsync(l.d_off, off.data(), sizeof(int64_t) * ng * nchunk, hipMemcpyHostToDevice, stream));
+        CK(launch_dij_compact(l.d_s, l.d_s2, nreg, nvox, ng, l.d_thresh, l.d_off, l.d_ir, l.d_val, l.d_vval, stream));
+        out.ir.resize(tot);
+        out.val.resize(tot);
+        CK(hipMemcpyAsync(out.ir.data(), l.d_ir, sizeof(int64_t) * tot, hipMemcpyDeviceToHost, stream));
+        CK(hipMemcpyAsync(out.val.data(), l.d_val, sizeof(double) * tot, hipMemcpyDeviceToHost, stream));
+        if (D.want_var) {
+            out.var.resize(tot);
+            CK(hipMemcpyAsync(out.var.data(), l.d_vval, sizeof(double) * tot, hipMemcpyDeviceToHost, stream));
+        }
+    }
+    if (D.timing) {
+        CK(hipEventRecord(ev.ev[1], stream));
+        CK(hipEventSynchronize(ev.ev[1]));
+        l.t_other += ev.ms(0, 1) * 1e-3;
+        l.n_other += 4;
+    }
+    CK(hipStreamSynchronize(stream));   // counts/off and the output vectors are read by the host
+    return OMC_OK;
+}
+
+// Groups gi = first, first + step, ... on lane l; on failure the error text is
+// kept with the lane (error messages are per thread) and the other lane is
+// released from its wait for the finish order.
+template <class R> void dij_lane_run(DijRun<R> &D, DijLane<R> &l, int first, int step) {
+    (void)hipSetDevice(D.c->device);      // the current device is per thread
+    for (int gi = first; gi < D.ngroups; gi += step) {
+        int rc = dij_lane_group(D, l, gi);
+        if (rc) {
+            if (rc != RC_LANE_STOPPED) {
+                l.rc = rc;
+                l.msg = omc_host_last_error();
+            }
+            std::lock_guard<std::mutex> lk(D.mu);
+            D.failed = true;
+            D.cv.notify_all();
+            return;
+        }
+    }
+}
 
 template <class R>
 int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_desc *spec, const omc_beamlets_desc *bix,
             int nbatch, double rel, const omc_run_opts *o, omc_csc *dij, omc_csc *var, double *plan_dose,
             double *plan_var, omc_run_stats *st) {
     auto wall0 = std::chrono::steady_clock::now();
-    const int nbix = bix->nbixels;
-    const long nreg = pd->nreg, nvox = nreg - 1;
-    const bool event = o->form == OMC_FORM_EVENT;
-    const bool count = (o->count_events & 1) != 0, timing = o->count_events != 0;
-    std::vector<long> nper = beamlet_nper(bix, nbatch);
-    for (long n : nper)
+    DijRun<R> D;
+    D.c = c;
+    D.pd = pd;
+    D.o = o;
+    D.nbix = bix->nbixels;
+    D.nbatch = nbatch;
+    D.nreg = pd->nreg;
+    D.nvox = D.nreg - 1;
+    D.event = o->form == OMC_FORM_EVENT;
+    D.count = (o->count_events & 1) != 0;
+    D.timing = o->count_events != 0;
+    D.replay = o->rng == OMC_RNG_REPLAY;
+    D.want_var = var != nullptr;
+    D.rel = rel;
+    const int nbix = D.nbix;
+    const long nreg = D.nreg, nvox = D.nvox;
+    D.nper = beamlet_nper(bix, nbatch);
+    for (long n : D.nper)
         if (n >= (1L << 32)) return omc_seterr(OMC_ERR_LIMIT, "more than 2^32 histories per batch of a beamlet");
-    long long tgt = o->event_target > 0 ? o->event_target : default_target(c->device);
-    const int G = pick_group(o, nreg, nbix, c->device, tgt);
+    D.tgt = o->event_target > 0 ? o->event_target : default_target(c->device);
+    // lanes: up to three for an event-form run of several groups (o->streams
+    // pins the number), each planned on its share of the memory budget.  On
+    // the 1960-beamlet plan at 96 GiB: 1 lane 7.2, 2 lanes 10.1, 3 lanes
+    // 10.4, 4 lanes 7-9 M histories/s.
+    size_t budget = o->hbm_budget > 0 ? (size_t)o->hbm_budget : dij_default_budget();
+    DijPlan plan;
+    dij_plan(o, nreg, nbix, nbatch, D.tgt, budget, plan);
+    int NL = 1;
+    if (D.event) NL = o->streams > 0 ? o->streams : std::min(3, (nbix + plan.G - 1) / plan.G);
+    if (NL > 1) dij_plan(o, nreg, nbix, nbatch, D.tgt, budget / NL, plan);
+    D.G = plan.G;
+    D.NBJ = D.event ? plan.nbj : 1;
+    D.ngroups = (nbix + D.G - 1) / D.G;
+    NL = std::min(NL, D.ngroups);
+    D.nblk = (int)((nvox + DIJ_BLOCK - 1) / DIJ_BLOCK);
+    D.nchunk = (int)((nvox + DIJ_CHUNK - 1) / DIJ_CHUNK);
     DevArena A;
     ArenaGuard ag{A};
-    SrcBeamlets<R> bsrc;
-    int rc = upload_beamlets<R>(A, spec, bix, bsrc);
+    int rc = upload_beamlets<R>(A, spec, bix, D.bsrc);
     if (rc) return rc;
-    const int nblk = (int)((nvox + DIJ_BLOCK - 1) / DIJ_BLOCK);
-    const int nchunk = (int)((nvox + DIJ_CHUNK - 1) / DIJ_CHUNK);
-    double *d_e = A.alloc_n<double>((size_t)G * nreg), *d_s = A.alloc_n<double>((size_t)G * nreg),
-           *d_s2 = A.alloc_n<double>((size_t)G * nreg);
-    double *d_carry = A.alloc_n<double>(nreg), *d_tdose = A.alloc_n<double>(nreg), *d_tvar = A.alloc_n<double>(nreg);
-    double *d_bmax = A.alloc_n<double>((size_t)G * nblk), *d_thresh = A.alloc_n<double>(G);
-    double *d_nper = A.alloc_n<double>(G);
-    int *d_counts = A.alloc_n<int>((size_t)G * nchunk);
-    int64_t *d_off = A.alloc_n<int64_t>((size_t)G * nchunk);
-    long *d_slot_off = A.alloc_n<long>(G + 1);
-    int *d_slot_bix = A.alloc_n<int>(G);
-    unsigned long long *d_cnt = A.alloc_n<unsigned long long>(16), *d_next = A.alloc_n<unsigned long long>(1);
-    int *d_err = A.alloc_n<int>(4);
-    if (!A.ok) return omc_seterr(OMC_ERR_NOMEM, "Dij tallies for a group of %d beamlets do not fit", G);
+    D.d_carry = A.alloc_n<double>(nreg);
+    D.d_tdose = A.alloc_n<double>(nreg);
+    D.d_tvar = A.alloc_n<double>(nreg);
+    D.d_cnt = A.alloc_n<unsigned long long>(16);
+    D.d_next = A.alloc_n<unsigned long long>(1);
+    D.d_err = A.alloc_n<int>(4);
+    D.d_rn = nullptr;
+    D.d_rstart = D.d_rend = nullptr;
+    if (!A.ok) return omc_seterr(OMC_ERR_NOMEM, "Dij run arrays");
     // Replayed draws (verification): histories are numbered beamlet by beamlet,
     // batch by batch, as the reference's serial loop consumes its stream.
-    const bool replay = o->rng == OMC_RNG_REPLAY;
-    std::vector<long> gfirst(nbix + 1, 0);
-    for (int b = 0; b < nbix; b++) gfirst[b + 1] = gfirst[b] + (long)nbatch * nper[b];
-    uint32_t *d_rn = nullptr;
-    long *d_rstart = nullptr, *d_rend = nullptr;
-    if (replay) {
-        if (o->replay_len < o->replay_offsets[gfirst[nbix]])
+    D.gfirst.assign(nbix + 1, 0);
+    for (int b = 0; b < nbix; b++) D.gfirst[b + 1] = D.gfirst[b] + (long)nbatch * D.nper[b];
+    if (D.replay) {
+        if (o->replay_len < o->replay_offsets[D.gfirst[nbix]])
             return omc_seterr(OMC_ERR_ARG, "replay stream shorter than its offsets");
-        d_rn = A.alloc_n<uint32_t>(o->replay_len > 0 ? o->replay_len : 1);
+        D.d_rn = A.alloc_n<uint32_t>(o->replay_len > 0 ? o->replay_len : 1);
         long maxjob = 0;
-        for (int g0 = 0; g0 < nbix; g0 += G) {
+        for (int g0 = 0; g0 < nbix; g0 += D.G) {
             long t = 0;
-            for (int b = g0; b < std::min(nbix, g0 + G); b++) t += nper[b];
+            for (int b = g0; b < std::min(nbix, g0 + D.G); b++) t += D.nper[b];
             maxjob = std::max(maxjob, t);
         }
-        d_rstart = A.alloc_n<long>(maxjob);
-        d_rend = A.alloc_n<long>(maxjob);
+        D.d_rstart = A.alloc_n<long>(maxjob);
+        D.d_rend = A.alloc_n<long>(maxjob);
         if (!A.ok) return omc_seterr(OMC_ERR_NOMEM, "replay stream");
-        if (o->replay_len > 0) CK(hipMemcpy(d_rn, o->replay, sizeof(uint32_t) * o->replay_len, hipMemcpyHostToDevice));
+        if (o->replay_len > 0)
+            CK(hipMemcpy(D.d_rn, o->replay, sizeof(uint32_t) * o->replay_len, hipMemcpyHostToDevice));
     }
-    EvState<R> S;
-    if (event) {
-        long long maxjob = 0;
-        for (int g0 = 0; g0 < nbix; g0 += G) {
-            long long t = 0;
-            for (int b = g0; b < std::min(nbix, g0 + G); b++) t += nper[b];
-            maxjob = std::max(maxjob, t);
+    CK(hipMemsetAsync(D.d_carry, 0, sizeof(double) * nreg, stream));
+    CK(hipMemsetAsync(D.d_tdose, 0, sizeof(double) * nreg, stream));
+    CK(hipMemsetAsync(D.d_tvar, 0, sizeof(double) * nreg, stream));
+    CK(hipMemsetAsync(D.d_cnt, 0, sizeof(unsigned long long) * 16, stream));
+    CK(hipMemsetAsync(D.d_err, 0, sizeof(int) * 4, stream));
+    std::vector<std::unique_ptr<DijLane<R>>> lanes;
+    for (int k = 0; k < NL; k++) {
+        lanes.emplace_back(new DijLane<R>());
+        DijLane<R> &l = *lanes.back();
+        if (k == 0) {
+            l.st = stream;
+        } else {
+            CK(hipStreamCreateWithFlags(&l.st, hipStreamNonBlocking));
+            l.own_stream = true;
         }
-        if (tgt > 4 * maxjob) tgt = std::max<long long>(4 * maxjob, 64);
-        rc = S.alloc(tgt);
-        if (rc) return rc;
+        D.lanes.push_back(&l);
     }
-    CK(hipMemsetAsync(d_e, 0, sizeof(double) * G * nreg, stream));
-    CK(hipMemsetAsync(d_carry, 0, sizeof(double) * nreg, stream));
-    CK(hipMemsetAsync(d_tdose, 0, sizeof(double) * nreg, stream));
-    CK(hipMemsetAsync(d_tvar, 0, sizeof(double) * nreg, stream));
-    CK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long) * 16, stream));
-    CK(hipMemsetAsync(d_err, 0, sizeof(int) * 4, stream));
-    EvLaunch<R> L;
-    HistArgs<R> H;
-    memset(&H, 0, sizeof H);
-    if (event) {
-        fill_launch(c, pd, S, o, L);
-        L.src.bix = bsrc;
-        L.src.use_beamlets = 1;
-        L.src.slot_off = d_slot_off;
-        L.src.slot_beamlet = d_slot_bix;
-        L.endep = d_e;
-        L.slot_stride = nreg;
-        L.counters = count ? d_cnt : nullptr;
-    } else {
-        H.t = tables_of<R>(c);
-        H.g = geom_of<R>(pd);
-        H.bix = bsrc;
-        H.use_beamlets = 1;
-        H.slot_off = d_slot_off;
-        H.slot_beamlet = d_slot_bix;
-        H.slot_stride = nreg;
-        H.seed = o->seed;
-        H.endep = d_e;
-        H.counters = count ? d_cnt : nullptr;
-        H.err = d_err;
-        H.next = d_next;
-        H.chunk = 1;
-        H.replay = d_rn;
-        H.replay_off = d_rstart;
-        H.replay_end = d_rend;
-        H.nsplit = o->nsplit;
-    }
-    HostCSC out;
-    out.jc.assign(nbix + 1, 0);
-    std::vector<int> counts;
-    std::vector<int64_t> off;
-    DevArena O;             // compaction output, grown on demand
-    ArenaGuard og{O};
-    size_t ocap = 0;
-    int64_t *d_ir = nullptr;
-    double *d_val = nullptr, *d_vval = nullptr;
-    JobStats js;
-    double t_hist = 0, t_other = 0;
-    uint64_t n_hist = 0, n_other = 0;
-    double histories = 0;
-    Events ev(4);
+    Events ev(2);
     CK(hipEventRecord(ev.ev[0], stream));
-    for (int g0 = 0; g0 < nbix; g0 += G) {
-        const int ng = std::min(G, nbix - g0);
-        std::vector<long> soff(ng + 1, 0);
-        std::vector<int> sbix(ng);
-        std::vector<double> snper(ng);
-        for (int s = 0; s < ng; s++) {
-            sbix[s] = g0 + s;
-            snper[s] = (double)nper[g0 + s];
-            soff[s + 1] = soff[s] + nper[g0 + s];
-        }
-        const long njob = soff[ng];
-        CK(hipMemcpyAsync(d_slot_off, soff.data(), sizeof(long) * (ng + 1), hipMemcpyHostToDevice, stream));
-        CK(hipMemcpyAsync(d_slot_bix, sbix.data(), sizeof(int) * ng, hipMemcpyHostToDevice, stream));
-        CK(hipMemcpyAsync(d_nper, snper.data(), sizeof(double) * ng, hipMemcpyHostToDevice, stream));
-        CK(hipMemsetAsync(d_s, 0, sizeof(double) * ng * nreg, stream));
-        CK(hipMemsetAsync(d_s2, 0, sizeof(double) * ng * nreg, stream));
-        for (int bt = 0; bt < nbatch; bt++) {
-            if (event) {
-                L.src.nslot = ng;
-                L.src.batch = bt;
-                rc = run_job_grow(stream, S, L, njob, 0, count, timing, js, d_e, (size_t)ng * nreg);
-                if (rc) return rc;
-            } else {
-                H.nslot = ng;
-                H.batch = bt;
-                H.nhist = njob;
-                H.hist_base = 0;
-                if (replay) {
-                    std::vector<long> rs(njob), re(njob);
-                    for (int sl = 0; sl < ng; sl++)
-                        for (long i = 0; i < nper[g0 + sl]; i++) {
-                            long gh = gfirst[g0 + sl] + (long)bt * nper[g0 + sl] + i;
-                            rs[soff[sl] + i] = (long)o->replay_offsets[gh];
-                            re[soff[sl] + i] = (long)o->replay_offsets[gh + 1];
-                        }
-                    CK(hipMemcpyAsync(d_rstart, rs.data(), sizeof(long) * njob, hipMemcpyHostToDevice, stream));
-                    CK(hipMemcpyAsync(d_rend, re.data(), sizeof(long) * njob, hipMemcpyHostToDevice, stream));
-                    CK(hipStreamSynchronize(stream));   // rs/re go out of scope
-                }
-                CK(hipMemsetAsync(d_next, 0, sizeof(unsigned long long), stream));
-                int block = o->block > 0 ? o->block : 256;
-                long want = (njob + block - 1) / block;
-                int grid = o->grid > 0 ? o->grid : (int)std::min<long>(want, 2048);
-                if (timing) CK(hipEventRecord(ev.ev[2], stream));
-                CK(launch_history<R>(H, replay ? RNG_REPLAY : RNG_PHILOX, count, grid, block, stream));
-                if (timing) {
-                    CK(hipEventRecord(ev.ev[3], stream));
-                    CK(hipEventSynchronize(ev.ev[3]));
-                    t_hist += ev.ms(2, 3) * 1e-3;
-                }
-                n_hist++;
-            }
-            CK(launch_accum_slots(d_e, d_s, d_s2, nreg, nreg, ng, d_nper, stream));
-            histories += (double)njob;
-        }
-        if (timing) CK(hipEventRecord(ev.ev[2], stream));
-        int nb = 0;
-        CK(launch_dij_finish(d_s, d_s2, d_carry, ng, nreg, nvox, pd->dmass, pd->drho, nbatch, d_bmax, d_tdose, d_tvar,
-                             &nb, stream));
-        CK(launch_dij_select(d_s, nreg, nvox, ng, d_bmax, nb, rel, d_thresh, d_counts, stream));
-        counts.resize((size_t)ng * nchunk);
-        off.resize((size_t)ng * nchunk);
-        CK(hipMemcpyAsync(counts.data(), d_counts, sizeof(int) * ng * nchunk, hipMemcpyDeviceToHost, stream));
-        CK(hipStreamSynchronize(stream));
-        int64_t tot = 0;
-        for (size_t k = 0; k < counts.size(); k++) {
-            off[k] = tot;
-            tot += counts[k];
-        }
-        for (int s = 0; s < ng; s++) out.jc[g0 + s + 1] = (int64_t)out.ir.size() + off[(size_t)(s + 1) * nchunk - 1] +
-                                                           counts[(size_t)(s + 1) * nchunk - 1];
-        if ((size_t)tot > ocap) {
-            O.release();
-            ocap = std::max<size_t>((size_t)tot, ocap * 2);
-            d_ir = O.alloc_n<int64_t>(ocap);
-            d_val = O.alloc_n<double>(ocap);
-            d_vval = var ? O.alloc_n<double>(ocap) : nullptr;
-            if (!O.ok) return omc_seterr(OMC_ERR_NOMEM, "Dij output buffers (%lld entries)", (long long)tot);
-        }
-        if (tot > 0) {
-            CK(hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * ng * nchunk, hipMemcpyHostToDevice, stream));
-            CK(launch_dij_compact(d_s, d_s2, nreg, nvox, ng, d_thresh, d_off, d_ir, d_val, d_vval, stream));
-            size_t base = out.ir.size();
-            out.ir.resize(base + tot);
-            out.val.resize(base + tot);
-            if (var) out.var.resize(base + tot);
-            CK(hipMemcpyAsync(out.ir.data() + base, d_ir, sizeof(int64_t) * tot, hipMemcpyDeviceToHost, stream));
-            CK(hipMemcpyAsync(out.val.data() + base, d_val, sizeof(double) * tot, hipMemcpyDeviceToHost, stream));
-            if (var)
-                CK(hipMemcpyAsync(out.var.data() + base, d_vval, sizeof(double) * tot, hipMemcpyDeviceToHost, stream));
-        }
-        if (timing) {
-            CK(hipEventRecord(ev.ev[3], stream));
-            CK(hipEventSynchronize(ev.ev[3]));
-            t_other += ev.ms(2, 3) * 1e-3;
-            n_other += 4;
-        }
-        CK(hipStreamSynchronize(stream));
+    for (int k = 1; k < NL; k++) CK(hipStreamWaitEvent(lanes[k]->st, ev.ev[0], 0));   // after the set-up above
+    for (auto &l : lanes)
+        if ((rc = dij_lane_init(D, *l))) return rc;
+    D.outs.resize(D.ngroups);
+    {
+        std::vector<std::thread> th;
+        for (int k = 1; k < NL; k++) th.emplace_back(dij_lane_run<R>, std::ref(D), std::ref(*lanes[k]), k, NL);
+        dij_lane_run<R>(D, *lanes[0], 0, NL);
+        for (auto &t : th) t.join();
     }
+    for (auto &l : lanes)
+        if (l->rc) return omc_seterr(l->rc, "%s", l->msg.c_str());
+    for (auto &l : lanes) CK(hipStreamSynchronize(l->st));
     CK(hipEventRecord(ev.ev[1], stream));
     CK(hipStreamSynchronize(stream));
     int err[4];
-    CK(hipMemcpy(err, d_err, sizeof err, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(err, D.d_err, sizeof err, hipMemcpyDeviceToHost));
     if (err[0]) return omc_seterr(OMC_ERR_STACK, "particle stack overflow on the device");
     if (err[1]) return omc_seterr(OMC_ERR_ARG, "replay stream underrun");
     if (err[2]) return omc_seterr(OMC_ERR_LIMIT, "history step guard reached");
-    if (plan_dose) CK(hipMemcpy(plan_dose, d_tdose + 1, sizeof(double) * nvox, hipMemcpyDeviceToHost));
-    if (plan_var) CK(hipMemcpy(plan_var, d_tvar + 1, sizeof(double) * nvox, hipMemcpyDeviceToHost));
-    const int64_t nnz = (int64_t)out.ir.size();
-    auto fill_csc = [&](omc_csc *m, const std::vector<double> &v) -> int {
+    if (plan_dose) CK(hipMemcpy(plan_dose, D.d_tdose + 1, sizeof(double) * nvox, hipMemcpyDeviceToHost));
+    if (plan_var) CK(hipMemcpy(plan_var, D.d_tvar + 1, sizeof(double) * nvox, hipMemcpyDeviceToHost));
+    // columns in beamlet order, group after group
+    int64_t nnz = 0;
+    for (auto &g : D.outs) nnz += (int64_t)g.ir.size();
+    auto fill_csc = [&](omc_csc *m, bool variance) -> int {
         m->nrows = nvox;
         m->ncols = nbix;
         m->nnz = nnz;
@@ -628,34 +871,46 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
             omc_csc_free(m);
             return omc_seterr(OMC_ERR_NOMEM, "host CSC of %lld entries", (long long)nnz);
         }
-        memcpy(m->jc, out.jc.data(), sizeof(int64_t) * (nbix + 1));
-        if (nnz) {
-            memcpy(m->ir, out.ir.data(), sizeof(int64_t) * nnz);
-            memcpy(m->val, v.data(), sizeof(double) * nnz);
+        int64_t e = 0;
+        int col = 0;
+        m->jc[0] = 0;
+        for (auto &g : D.outs) {
+            for (int64_t len : g.collen) {
+                m->jc[col + 1] = m->jc[col] + len;
+                col++;
+            }
+            if (!g.ir.empty()) {
+                memcpy(m->ir + e, g.ir.data(), sizeof(int64_t) * g.ir.size());
+                memcpy(m->val + e, (variance ? g.var : g.val).data(), sizeof(double) * g.ir.size());
+                e += (int64_t)g.ir.size();
+            }
         }
         return OMC_OK;
     };
-    if (dij && (rc = fill_csc(dij, out.val))) return rc;
-    if (var && (rc = fill_csc(var, out.var))) {
+    if (dij && (rc = fill_csc(dij, false))) return rc;
+    if (var && (rc = fill_csc(var, true))) {
         if (dij) omc_csc_free(dij);
         return rc;
     }
     if (st) {
         unsigned long long cnt[16];
-        CK(hipMemcpy(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost));
+        CK(hipMemcpy(cnt, D.d_cnt, sizeof cnt, hipMemcpyDeviceToHost));
         st->gpu_seconds = ev.ms(0, 1) * 1e-3;
-        st->histories = histories;
         stats_counters(st, cnt);
-        if (event) {
-            stats_from_job(st, js);
-        } else {
-            st->kernel_seconds[OMC_K_HISTORY] = t_hist;
-            st->kernel_launches[OMC_K_HISTORY] = n_hist;
+        double histories = 0;
+        for (auto &l : lanes) {
+            histories += l->histories;
+            if (D.event) stats_from_job(st, l->js);
+            st->kernel_seconds[OMC_K_HISTORY] += l->t_hist;
+            st->kernel_launches[OMC_K_HISTORY] += l->n_hist;
+            st->kernel_seconds[OMC_K_OTHER] += l->t_other;
+            st->kernel_launches[OMC_K_OTHER] += l->n_other;
         }
-        st->kernel_seconds[OMC_K_OTHER] += t_other;
-        st->kernel_launches[OMC_K_OTHER] += n_other;
-        st->groups = (nbix + G - 1) / G;
-        st->group_size = G;
+        st->histories = histories;
+        st->groups = D.ngroups;
+        st->group_size = D.G;
+        st->batches_per_job = D.NBJ;
+        st->streams = NL;
         st->nnz = nnz;
         st->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
     }
@@ -758,14 +1013,16 @@ size_t request_bytes(omc_server *s, const omc_request &r) {
     size_t b = omc_phantom_bytes(&p);
     long long tgt = default_target(s->ctx->device);
     size_t ev = r.precision == OMC_FP64 ? event_bytes<double>(tgt) : event_bytes<float>(tgt);
-    if (r.kind == OMC_REQ_DOSE) return b + (size_t)3 * nreg * 8 + ev;
-    // Dij: as many beamlets per group as the request's share of the budget
-    // holds (at least one), so that each transport job is large
-    size_t fixed = b + dij_fixed_bytes(nreg, r.precision, OMC_FORM_EVENT, tgt);
+    // dose: every batch in one job, each with its own tallies
+    if (r.kind == OMC_REQ_DOSE) return b + (size_t)(r.nbatch + 2) * nreg * 8 + ev;
+    // Dij: the plan with the fewest transport jobs that the request's share
+    // of the budget holds (at least one beamlet and one batch per job)
+    omc_run_opts o;
+    omc_run_opts_default(&o);
+    o.precision = r.precision == OMC_FP64 ? OMC_FP64 : OMC_FP32;
+    DijPlan plan;
     size_t share = (size_t)s->budget / (size_t)s->max_inflight;
-    size_t slots = share > fixed ? (share - fixed) / (dij_slot_bytes(nreg) + 64) : 1;
-    slots = std::max<size_t>(1, std::min<size_t>(slots, (size_t)r.bix.nbixels));
-    return fixed + slots * (dij_slot_bytes(nreg) + 64);
+    return b + dij_plan(&o, nreg, r.bix.nbixels, r.nbatch, tgt, share > b ? share - b : 1, plan);
 }
 
 void run_request(omc_server *s, omc_server::Req *q) {
@@ -785,6 +1042,7 @@ void run_request(omc_server *s, omc_server::Req *q) {
     o.precision = q->r.precision == OMC_FP64 ? OMC_FP64 : OMC_FP32;
     o.form = OMC_FORM_EVENT;
     o.seed = q->r.seed;
+    o.batches_per_job = q->r.kind == OMC_REQ_DOSE ? q->r.nbatch : 0;
     omc_run_stats stats;
     memset(&stats, 0, sizeof stats);
     if (!rc && q->r.kind == OMC_REQ_DOSE) {
@@ -854,10 +1112,12 @@ extern "C" int omc_server_create(omc_ctx *ctx, int64_t hbm_budget_bytes, int max
     omc_server *s = new omc_server();
     s->ctx = ctx;
     if (hbm_budget_bytes <= 0) {
+        // 3/4 of the free memory, at most 96 GiB: past ~100 GiB held by the
+        // process, allocations take seconds to map and unmap
         size_t freeb = 0, totalb = 0;
         (void)hipSetDevice(ctx->device);
         if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) freeb = (size_t)8 << 30;
-        hbm_budget_bytes = (int64_t)(freeb * 3 / 4);
+        hbm_budget_bytes = (int64_t)std::min(freeb * 3 / 4, (size_t)96 << 30);
     }
     s->budget = hbm_budget_bytes;
     s->max_inflight = max_inflight;

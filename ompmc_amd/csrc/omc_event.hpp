@@ -89,6 +89,7 @@ template <class R> struct EvArgs {
     unsigned long long *counters;     // [8] event counters (optional)
     int *err;                         // EvCtl::err
     int budget;                       // electron ustep iterations between refills of a wave
+    int turns;                        // electrons: refills a particle may hold its lane before it is requeued (0: no limit)
     int delta;                        // photons: 1 delta tracking, 0 voxel-by-voxel flight
 };
 
@@ -168,6 +169,17 @@ __device__ __forceinline__ void flush_counters(unsigned long long *dst, const Co
 // it is gone, so the queues only carry new secondaries and type changes, and
 // no lane idles while the queue still holds work.  The wave exits when its
 // lanes are empty and the queue is exhausted.
+//
+// Two things keep a launch short.  A wave reads the claim counter before it
+// bumps it, so once the queue is exhausted idle waves leave without
+// serialising on that one address (8192 waves queueing on it cost ~100 us
+// per launch).  And an electron that has held its lane for a.turns refills
+// is handed back to the queue with its resumable state (position in its
+// random stream, distance to the next discrete interaction, step phase): an
+// electron in single-scattering mode along a voxel face can take thousands
+// of steps, and without the hand-back it alone would hold the round -- and
+// every particle queued behind the round -- open for tens of milliseconds.
+// Where a particle is requeued does not change its track.
 template <class R, bool PHOTON, bool COUNT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PHOTON ? 1 : OMC_CHARGED_WAVES_PER_EU)))
 k_event(EvArgs<R> a) {
@@ -181,11 +193,16 @@ k_event(EvArgs<R> a) {
     PhiloxAt rng;
     TallyAtomic tally;
     tally.init(a.endep);
-    int iters = 0;
+    int iters = 0, turns = 0;
     for (;;) {
-        long long idx = wave_reserve(a.claim, !have);
+        long long idx = -1;
+        if (__any(!have)) {
+            const unsigned long long seen = __shfl(*(volatile const unsigned long long *)a.claim, 0);
+            if ((long long)seen < n) idx = wave_reserve(a.claim, !have);
+        }
         if (!have && idx >= 0 && idx < n) {
             iters = 0;
+            turns = 0;
             load(a.in, idx, p);
             stream = a.in.stream[idx];
             slot = a.in.slot[idx];
@@ -223,6 +240,7 @@ k_event(EvArgs<R> a) {
             }
             // d: -1 particle gone, 0 particle continues, +1 continues with a secondary
             done = d < 0 || ((p.iq == 0) != PHOTON);
+            if (!PHOTON && !done && a.turns > 0 && ++turns >= a.turns) done = true;   // requeue, see above
             if (done) tally.flush();
             if (COUNT) {
                 tot.nhowfar += e.cnt.nhowfar; tot.nhownear += e.cnt.nhownear; tot.ncross += e.cnt.ncross;
@@ -245,8 +263,9 @@ template <class R> struct SrcArgs {
     SrcDosxyz<R> dos;
     SrcBeamlets<R> bix;
     int use_beamlets;
-    const long *slot_off;        // beamlet source: [nslot+1] first history of each slot in the job
-    const int *slot_beamlet;     // beamlet of each slot
+    const long *slot_off;        // [nslot+1] first history of each tally slot in the job (nslot 0: one slot)
+    const int *slot_beamlet;     // beamlet source: beamlet of each slot
+    const int *slot_batch;       // beamlet source: batch of each slot (null: every slot is of `batch`)
     int nslot;
     int batch;
     long hist_base;              // dosxyz source: stream of history 0 of the job
@@ -262,10 +281,10 @@ __global__ void __launch_bounds__(256) k_source(SrcArgs<R> a) {
         long h = h0 + k;
         int slot = 0, beamlet = 0;
         uint64_t stream = (uint64_t)(a.hist_base + h);
+        if (a.nslot > 0) slot = find_slot(a.slot_off, a.nslot, h);
         if (a.use_beamlets) {
-            slot = find_slot(a.slot_off, a.nslot, h);
             beamlet = a.slot_beamlet[slot];
-            stream = beamlet_stream(beamlet, a.batch, h - a.slot_off[slot]);
+            stream = beamlet_stream(beamlet, a.slot_batch ? a.slot_batch[slot] : a.batch, h - a.slot_off[slot]);
         }
         PhiloxAt rng;
         rng.at(a.seed, stream, 0);

@@ -25,6 +25,7 @@ template <class R> struct EvLaunch {
     long slot_stride;
     unsigned long long *counters;
     int budget;
+    int turns;
     int delta;
     SrcArgs<R> src;              // source of the job (ctl and out are set per round)
     int grid_event, grid_source;

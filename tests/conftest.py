@@ -36,6 +36,7 @@ def write_case_inp(tmp_path, name, data_folder=None, **over):
     from ompmc_amd import phantoms as P
     kw, nh, nb = DOSE_CASES[name]
     kw = dict(kw)
+    nh, nb = over.pop("ncase", nh), over.pop("nbatch", nb)
     kw.update(over)
     spec = kw.pop("spectrum", None)
     if spec == "mohan6":

@@ -151,3 +151,29 @@ def test_autotuner_picks_a_measured_setting(tmp_path, monkeypatch):
     b = c.run_dij(ph, spec, bix, nbatch=4, seed=5, event_target=999, electron_budget=3)[0]
     c.close()
     _compare_csc(a, b, "tuned vs other setting")
+
+
+def test_dij_batches_per_job_same_matrix(tmp_path):
+    """Event-form Dij with every batch of a group in one job (a tally slot per
+    beamlet and batch), two batches per job, and one batch per job: the same
+    histories with the same stream keys, so the matrix and its variance agree
+    to FP64 summation order."""
+    ph_h, bix, _ = _case(tmp_path, nhist=(400, 800))
+    c = omc.Context(ph_h.media)
+    ph = c.phantom(ph_h, ecut=0.7, pcut=0.01)
+    spec = omc.Spectrum.mono(2.0)
+    kw = dict(nbatch=5, rel_threshold=1e-4, want_var=True, precision=omc.FP64, form=omc.FORM_EVENT, seed=11)
+    one = c.run_dij(ph, spec, bix, batches_per_job=1, group=3, **kw)
+    two = c.run_dij(ph, spec, bix, batches_per_job=2, group=4, **kw)
+    all_ = c.run_dij(ph, spec, bix, **kw)
+    lanes = c.run_dij(ph, spec, bix, group=2, streams=2, want_plan=True, **kw)
+    lane1 = c.run_dij(ph, spec, bix, group=2, streams=1, want_plan=True, **kw)
+    c.close()
+    assert one[3].batches_per_job == 1 and two[3].batches_per_job == 2 and all_[3].batches_per_job == 5
+    assert lanes[3].streams == 2 and lanes[3].groups == (bix.n + 1) // 2 and lane1[3].streams == 1
+    # concurrent groups: the finish still runs in group order (variance carry, plan totals)
+    assert np.allclose(lanes[2][0], lane1[2][0], rtol=1e-12, atol=0)
+    assert np.allclose(lanes[2][1], lane1[2][1], rtol=1e-12, atol=0)
+    for other, what in ((two, "2 batches per job"), (all_, "all batches per job"), (lanes, "two streams")):
+        _compare_csc(other[0], one[0], what)
+        _compare_csc(other[1], one[1], what + " (variance)")

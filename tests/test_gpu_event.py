@@ -87,9 +87,22 @@ def test_event_result_independent_of_schedule(slab):
     c, ph = slab
     ref = _run(c, ph, "slab", 4000, 4, precision=omc.FP64, form=omc.FORM_EVENT, seed=3)[0]
     for kw in (dict(event_target=100, electron_budget=1), dict(event_target=777, electron_budget=3, grid=7),
-               dict(event_target=1 << 16, electron_budget=1000)):
+               dict(event_target=1 << 16, electron_budget=1000), dict(electron_turns=1, electron_budget=2),
+               dict(electron_turns=-1), dict(batches_per_job=1), dict(batches_per_job=3, electron_turns=2)):
         g = _run(c, ph, "slab", 4000, 4, precision=omc.FP64, form=omc.FORM_EVENT, seed=3, **kw)[0]
         assert np.allclose(g, ref, rtol=1e-10, atol=1e-12 * ref.max()), kw
+
+
+def test_event_batches_per_job_keep_batch_statistics(slab):
+    """All batches in one job (a tally copy per batch) and one job per batch
+    give the same per-region sums and sums of squares: the batch statistics
+    are taken batch by batch in both."""
+    c, ph = slab
+    a, a2, sa = _run(c, ph, "slab", 4000, 5, precision=omc.FP64, form=omc.FORM_EVENT, seed=4, batches_per_job=1)
+    b, b2, sb = _run(c, ph, "slab", 4000, 5, precision=omc.FP64, form=omc.FORM_EVENT, seed=4)
+    assert sa.batches_per_job == 1 and sb.batches_per_job == 5
+    assert np.allclose(a, b, rtol=1e-10, atol=1e-12 * b.max())
+    assert np.allclose(a2, b2, rtol=1e-10, atol=1e-12 * b2.max())
 
 
 def test_event_pool_growth_reruns_job():
