@@ -128,7 +128,7 @@ typedef struct omc_run_opts {
     int voxel_walk;         /* event form: 1 = photons fly voxel by voxel as the reference's photon()
                                does; 0 = delta tracking with the phantom's majorant (same distribution) */
     int electron_turns;     /* event form: refills of electron_budget steps an electron may hold its lane
-                               within one round before it is requeued (0 = automatic, -1 = no limit) */
+                               within one round before it is requeued (0 = automatic: 24, -1 = no limit) */
     int batches_per_job;    /* event form: batches transported together, each into tallies of its own
                                (0 = as many as the memory budget holds, 1 = one batch per job) */
     int streams;            /* event-form Dij: beamlet groups transported concurrently, each on its own

@@ -29,16 +29,137 @@ int omc_seterr(int code, const char *fmt, ...) {
 }
 
 // ------------------------------------------------------------------ device memory
+namespace {
+
+// Released run blocks of one device, by size.  A request takes the smallest
+// cached block of at least its size and at most 5/4 of it; blocks are
+// rounded to 2 MiB.  The cache holds at most CACHE_MAX bytes; a block that
+// would exceed it goes back to the runtime.
+struct DevCache {
+    std::mutex mu;
+    std::multimap<size_t, void *> free;
+    size_t held = 0;
+};
+constexpr size_t CACHE_GRAIN = (size_t)2 << 20;
+constexpr size_t CACHE_MAX = (size_t)128 << 30;
+constexpr int CACHE_DEVICES = 64;
+
+DevCache &cache_here() {
+    static DevCache caches[CACHE_DEVICES];
+    int d = 0;
+    (void)hipGetDevice(&d);
+    return caches[(unsigned)d % CACHE_DEVICES];
+}
+
+std::mutex pinned_mu;
+std::vector<void *> pinned_free;
+
+}  // namespace
+
+void *devcache_alloc(size_t n, size_t *got) {
+    n = (n + CACHE_GRAIN - 1) / CACHE_GRAIN * CACHE_GRAIN;
+    DevCache &C = cache_here();
+    {
+        std::lock_guard<std::mutex> lk(C.mu);
+        auto it = C.free.lower_bound(n);
+        if (it != C.free.end() && it->first <= n + n / 4) {
+            void *p = it->second;
+            *got = it->first;
+            C.held -= it->first;
+            C.free.erase(it);
+            return p;
+        }
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, n) != hipSuccess) {
+        (void)hipGetLastError();          // recovered below: keep it from surfacing at the next launch check
+        devcache_trim();
+        if (hipMalloc(&p, n) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    *got = n;
+    return p;
+}
+
+void devcache_free(void *p, size_t n) {
+    DevCache &C = cache_here();
+    {
+        std::lock_guard<std::mutex> lk(C.mu);
+        if (C.held + n <= CACHE_MAX) {
+            C.free.emplace(n, p);
+            C.held += n;
+            return;
+        }
+    }
+    (void)hipFree(p);
+}
+
+size_t devcache_held() {
+    DevCache &C = cache_here();
+    std::lock_guard<std::mutex> lk(C.mu);
+    return C.held;
+}
+
+void devcache_trim() {
+    DevCache &C = cache_here();
+    std::multimap<size_t, void *> out;
+    {
+        std::lock_guard<std::mutex> lk(C.mu);
+        out.swap(C.free);
+        C.held = 0;
+    }
+    for (auto &kv : out) (void)hipFree(kv.second);
+}
+
+void *pinned_small_alloc() {
+    {
+        std::lock_guard<std::mutex> lk(pinned_mu);
+        if (!pinned_free.empty()) {
+            void *p = pinned_free.back();
+            pinned_free.pop_back();
+            return p;
+        }
+    }
+    void *p = nullptr;
+    if (hipHostMalloc(&p, PINNED_SMALL_BYTES, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return p;
+}
+
+void pinned_small_free(void *p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(pinned_mu);
+    pinned_free.push_back(p);
+}
+
 void DevArena::release() {
-    for (void *p : blocks) (void)hipFree(p);
+    if (cached && !blocks.empty()) {
+        // a block goes back to the cache only once the work using it is done
+        if (stream) (void)hipStreamSynchronize(stream);
+        else (void)hipDeviceSynchronize();
+        for (auto &b : blocks) devcache_free(b.first, b.second);
+    } else {
+        for (auto &b : blocks) (void)hipFree(b.first);
+    }
     blocks.clear();
     bytes = 0;
 }
 
 void *DevArena::alloc(size_t n) {
     void *p = nullptr;
-    if (hipMalloc(&p, n ? n : 1) != hipSuccess) return nullptr;
-    blocks.push_back(p);
+    size_t got = n ? n : 1;
+    if (cached) {
+        p = devcache_alloc(got, &got);
+        if (!p) return nullptr;
+    } else if (hipMalloc(&p, got) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    blocks.emplace_back(p, got);
     bytes += n;
     return p;
 }
@@ -162,6 +283,7 @@ extern "C" void omc_ctx_destroy(omc_ctx *c) {
     c->phantoms.clear();
     c->tabmem.release();
     c->scratch.release();
+    devcache_trim();          // blocks cached by this context's runs (none of them in use)
     if (c->stream) (void)hipStreamDestroy(c->stream);
     omc_tables_free(&c->host);
     delete c;
@@ -264,7 +386,7 @@ size_t omc_phantom_bytes(const omc_phantom_desc *d) {
     return nvox * (1 + 4 + 8 + 16) + 16 * (size_t)(d->nx + d->ny + d->nz + 3);
 }
 
-int omc_phantom_make(omc_ctx *c, const omc_phantom_desc *d, PhantomDev **out) {
+int omc_phantom_make(omc_ctx *c, const omc_phantom_desc *d, PhantomDev **out, hipStream_t cache_stream) {
     if (!c || !d || !d->xb || !d->yb || !d->zb || !d->med || !d->rho || d->nx < 1 || d->ny < 1 || d->nz < 1)
         return omc_seterr(OMC_ERR_ARG, "omc_phantom_upload: bad descriptor");
     long nvox = (long)d->nx * d->ny * d->nz;
@@ -272,6 +394,7 @@ int omc_phantom_make(omc_ctx *c, const omc_phantom_desc *d, PhantomDev **out) {
         if (d->med[v] < 0 || d->med[v] > c->host.nmed)
             return omc_seterr(OMC_ERR_FORMAT, "voxel %ld has medium %d (context has %d)", v, d->med[v], c->host.nmed);
     PhantomDev *pd = new PhantomDev();
+    if (cache_stream) pd->mem.use_cache(cache_stream);
     omc_phantom &ph = pd->host;
     ph.nmed = c->host.nmed;
     ph.nx = d->nx; ph.ny = d->ny; ph.nz = d->nz;
@@ -396,6 +519,7 @@ static int run_dosxyz_history(omc_ctx *c, hipStream_t stream, PhantomDev *pd, co
                               omc_run_stats *st) {
     int rc = OMC_OK;
     DevArena A;
+    A.use_cache(stream);
     HistArgs<R> a;
     memset(&a, 0, sizeof a);
     a.t = tables_of<R>(c);

@@ -12,11 +12,22 @@
 #include "omc_host.h"
 #include "omc_launch.hpp"
 
-// Device allocations owned together and released together.
+// Device allocations owned together and released together.  An arena set to
+// use_cache(stream) takes its blocks from a per-device cache and hands them
+// back on release, after synchronising that stream.  hipFree synchronises the
+// whole device, which would stall every other stream of a run or of a
+// server.  And mapping ~100 GiB anew costs seconds, so run-scoped arenas
+// reuse blocks across calls.
 struct DevArena {
-    std::vector<void *> blocks;
+    std::vector<std::pair<void *, size_t>> blocks;
     size_t bytes = 0;
     bool ok = true;
+    bool cached = false;
+    hipStream_t stream = nullptr;     // cached arenas: the stream their blocks are used on
+    void use_cache(hipStream_t s) {
+        cached = true;
+        stream = s;
+    }
     void *alloc(size_t n);
     template <class T> T *alloc_n(size_t n) {
         T *p = (T *)alloc(sizeof(T) * n);
@@ -31,6 +42,18 @@ struct DevArena {
     }
     void release();
 };
+
+// Block cache of the current device (see DevArena).  devcache_held() is the
+// memory it holds for reuse; devcache_trim() returns all of it to the runtime.
+void *devcache_alloc(size_t n, size_t *got);
+void devcache_free(void *p, size_t n);
+size_t devcache_held();
+void devcache_trim();
+// Small pinned host buffers (control-block snapshots), kept for reuse: hipHostFree
+// also synchronises the device.
+void *pinned_small_alloc();
+void pinned_small_free(void *p);
+constexpr size_t PINNED_SMALL_BYTES = 4096;
 
 struct PhantomDev {
     omc_phantom host;            // owned copy (dose conversion, source set-up)
@@ -71,7 +94,9 @@ template <> inline const omc::Geom<double> *geom_of<double>(const PhantomDev *p)
 int omc_seterr(int code, const char *fmt, ...);
 int omc_check_opts(const omc_run_opts *o);
 PhantomDev *omc_get_phantom(omc_ctx *c, int id);
-int omc_phantom_make(omc_ctx *c, const omc_phantom_desc *d, PhantomDev **out);
+// cache_stream: the phantom's device memory comes from the block cache and is
+// used on that stream only (server requests); null: plain allocations
+int omc_phantom_make(omc_ctx *c, const omc_phantom_desc *d, PhantomDev **out, hipStream_t cache_stream = nullptr);
 void omc_phantom_destroy(PhantomDev *pd);
 size_t omc_phantom_bytes(const omc_phantom_desc *d);
 

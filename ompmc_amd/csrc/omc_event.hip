@@ -44,14 +44,10 @@ struct Events {
     }
 };
 
-struct Pinned {
-    void *p = nullptr;
-    explicit Pinned(size_t n) {
-        if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) p = nullptr;
-    }
-    ~Pinned() {
-        if (p) (void)hipHostFree(p);
-    }
+// A pinned host buffer of PINNED_SMALL_BYTES from the reuse pool.
+struct PinnedSmall {
+    void *p = pinned_small_alloc();
+    ~PinnedSmall() { pinned_small_free(p); }
 };
 
 int device_cus(int dev) {
@@ -62,7 +58,10 @@ int device_cus(int dev) {
 
 // ------------------------------------------------------------------ event-form state
 constexpr int DEFAULT_BUDGET = 8;            // condensed-history steps between lane refills
-constexpr int DEFAULT_TURNS = 32;            // refills an electron may hold its lane in one round
+// Refills an electron may hold its lane in one round before it is requeued.
+// Measured on the 1960-beamlet plan (1 M in flight): 12 -> 12.0, 16 -> 12.3,
+// 24 -> 12.5, 64 -> 11.2, no limit -> 9.7 M histories/s.
+constexpr int DEFAULT_TURNS = 24;
 // Round guard of a job: the rounds its injection needs plus 2^16 -- with the
 // requeue after DEFAULT_TURNS x DEFAULT_BUDGET steps that is ~1.7e7 steps for
 // one particle, thousands of times the longest physical track.  A job that
@@ -75,8 +74,10 @@ template <class R> size_t event_bytes(long long target) {
     return (size_t)4 * (size_t)(8 * target) * pool_entry_bytes<R>() + sizeof(EvCtl);
 }
 
-// particles in flight: 2048 per compute unit, 8 per lane at 4 waves per SIMD
-long long default_target(int dev) { return (long long)device_cus(dev) * 2048; }
+// Particles in flight: 4096 per compute unit.  On the 1960-beamlet plan the
+// rate is flat from 1 M to 2 M in flight (12.3-12.6 M histories/s) and falls
+// below (10.1 M at 256 k, 12.0 M at 512 k) and above (11.6 M at 4 M).
+long long default_target(int dev) { return (long long)device_cus(dev) * 4096; }
 
 // run_job's code for a pool overflow: the caller zeroes the job's tallies,
 // doubles the pools (EvState::grow) and runs the job again.  Every particle's
@@ -131,7 +132,8 @@ int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int 
             JobStats &js) {
     constexpr int K = 4;
     CK(launch_job_start(S.ctl, nhist, S.target, S.cap, dest, st));
-    Pinned snapbuf(2 * sizeof(EvCtl));
+    static_assert(2 * sizeof(EvCtl) <= PINNED_SMALL_BYTES, "control-block snapshots exceed a pinned buffer");
+    PinnedSmall snapbuf;
     if (!snapbuf.p) return omc_seterr(OMC_ERR_NOMEM, "pinned host buffer");
     EvCtl *snap = (EvCtl *)snapbuf.p;
     Events chunk_done(2);
@@ -191,10 +193,11 @@ int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int 
     return OMC_OK;
 }
 
+// Half of the device memory that is free or held for reuse by the block cache.
 size_t default_budget() {
     size_t freeb = 0, totalb = 0;
     if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) freeb = (size_t)8 << 30;
-    return freeb / 2;
+    return (freeb + devcache_held()) / 2;
 }
 
 // Batches of a dose run transported in one job: o->batches_per_job, else as
@@ -290,7 +293,9 @@ int omc_run_dosxyz_event(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const o
     auto wall0 = std::chrono::steady_clock::now();
     DevArena A;
     ArenaGuard ag{A};
+    A.use_cache(stream);
     EvState<R> S;
+    S.A.use_cache(stream);
     long nreg = pd->nreg;
     // Batches transported together: each tallies into its own copy of the
     // region array (slot = batch) and the batch statistics are taken in batch
@@ -473,8 +478,9 @@ template <class R> struct DijLane {
     int rc = OMC_OK;
     std::string msg;
     ~DijLane() {
-        O.release();
+        O.release();      // cached blocks: each release synchronises st, so all go before st does
         A.release();
+        S.A.release();
         if (fin) (void)hipEventDestroy(fin);
         if (own_stream && st) (void)hipStreamDestroy(st);
     }
@@ -785,6 +791,7 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
     D.nchunk = (int)((nvox + DIJ_CHUNK - 1) / DIJ_CHUNK);
     DevArena A;
     ArenaGuard ag{A};
+    A.use_cache(stream);
     int rc = upload_beamlets<R>(A, spec, bix, D.bsrc);
     if (rc) return rc;
     D.d_carry = A.alloc_n<double>(nreg);
@@ -831,6 +838,9 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
             CK(hipStreamCreateWithFlags(&l.st, hipStreamNonBlocking));
             l.own_stream = true;
         }
+        l.A.use_cache(l.st);
+        l.O.use_cache(l.st);
+        l.S.A.use_cache(l.st);
         D.lanes.push_back(&l);
     }
     Events ev(2);
@@ -1036,7 +1046,7 @@ void run_request(omc_server *s, omc_server::Req *q) {
     }
     PhantomDev *pd = nullptr;
     omc_phantom_desc pdsc = q->r.phantom;
-    if (!rc) rc = omc_phantom_make(c, &pdsc, &pd);
+    if (!rc) rc = omc_phantom_make(c, &pdsc, &pd, st);
     omc_run_opts o;
     omc_run_opts_default(&o);
     o.precision = q->r.precision == OMC_FP64 ? OMC_FP64 : OMC_FP32;
@@ -1117,6 +1127,7 @@ extern "C" int omc_server_create(omc_ctx *ctx, int64_t hbm_budget_bytes, int max
         size_t freeb = 0, totalb = 0;
         (void)hipSetDevice(ctx->device);
         if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) freeb = (size_t)8 << 30;
+        freeb += devcache_held();
         hbm_budget_bytes = (int64_t)std::min(freeb * 3 / 4, (size_t)96 << 30);
     }
     s->budget = hbm_budget_bytes;

@@ -1,13 +1,14 @@
 """Autotuner of the event-queue form's launch parameters.
 
-The batched form has four knobs (``omc_run_opts``): the particles in flight
-per round (``event_target``), the electron steps between lane refills
-(``electron_budget``), the grid of the transport kernels (``grid``) and, for
-dose-influence runs, the beamlets per group (``group``, which bounds the
-tally memory).  ``tune_dij`` times candidate settings on a sample of the
-caller's own beamlets and returns the fastest as keyword arguments for
-``Context.run_dij``; results are cached per device and workload shape in a
-JSON file so a later process reuses them.
+The batched form's queue knobs (``omc_run_opts``) are the particles in
+flight per round (``event_target``), the electron steps between lane refills
+(``electron_budget``) and the refills an electron may hold its lane before it
+is requeued (``electron_turns``).  ``tune_dij`` times candidate settings on a
+sample of the caller's own beamlets and returns the fastest as keyword
+arguments for ``Context.run_dij``.  Results are cached per workload shape in
+a JSON file, so a later process reuses them.  None of the knobs changes a
+result beyond FP64 summation order: every history keeps its random-stream
+key.
 
     from ompmc_amd import tune
     best = tune.tune_dij(ctx, phantom, spectrum, beamlets)
@@ -25,8 +26,9 @@ from . import FORM_EVENT, FP32
 
 CACHE = os.environ.get("OMC_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "ompmc_amd_tune.json"))
 
-TARGETS = (262144, 524288, 1048576)
-BUDGETS = (4, 8, 16)
+TARGETS = (524288, 1048576, 2097152)
+BUDGETS = (8, 12)
+TURNS = (12, 24, 48)
 
 
 def _sample(bix, n):
@@ -59,11 +61,11 @@ def _store(d):
 
 
 def tune_dij(ctx, phantom, spectrum, bix, nbatch=10, precision=FP32, sample=200, targets=TARGETS, budgets=BUDGETS,
-             use_cache=True, verbose=False):
-    """Returns the fastest {event_target, electron_budget} (plus form and
-    precision) for ``Context.run_dij`` on a sample of ``bix``; every
-    candidate computes the same matrix (the random streams do not depend on
-    these settings), only the time differs."""
+             turns=TURNS, use_cache=True, verbose=False):
+    """Returns the fastest {event_target, electron_budget, electron_turns}
+    (plus form and precision) for ``Context.run_dij`` on a sample of
+    ``bix``.  Every candidate computes the same matrix (the random streams do
+    not depend on these settings); only the time differs."""
     key = _key(phantom, bix, precision, sample)
     cache = _load() if use_cache else {}
     if key in cache:
@@ -73,17 +75,19 @@ def tune_dij(ctx, phantom, spectrum, bix, nbatch=10, precision=FP32, sample=200,
     results = []
     for tgt in targets:
         for bud in budgets:
-            t0 = time.perf_counter()
-            _, _, _, st = ctx.run_dij(phantom, spectrum, sub, nbatch=nbatch, precision=precision, form=FORM_EVENT,
-                                      event_target=tgt, electron_budget=bud)
-            rate = st.histories / (time.perf_counter() - t0)
-            results.append((rate, tgt, bud))
-            if verbose:
-                print(f"tune: target {tgt} budget {bud}: {rate:.4g} histories/s")
-    rate, tgt, bud = max(results)
-    opts = dict(form=FORM_EVENT, precision=precision, event_target=tgt, electron_budget=bud)
+            for tu in turns:
+                t0 = time.perf_counter()
+                _, _, _, st = ctx.run_dij(phantom, spectrum, sub, nbatch=nbatch, precision=precision,
+                                          form=FORM_EVENT, event_target=tgt, electron_budget=bud, electron_turns=tu)
+                rate = st.histories / (time.perf_counter() - t0)
+                results.append((rate, tgt, bud, tu))
+                if verbose:
+                    print(f"tune: target {tgt} budget {bud} turns {tu}: {rate:.4g} histories/s")
+    rate, tgt, bud, tu = max(results)
+    opts = dict(form=FORM_EVENT, precision=precision, event_target=tgt, electron_budget=bud, electron_turns=tu)
     if use_cache:
         cache[key] = {"opts": opts, "histories_per_s": rate,
-                      "table": [dict(rate=r, event_target=t, electron_budget=b) for r, t, b in results]}
+                      "table": [dict(rate=r, event_target=t, electron_budget=b, electron_turns=u)
+                                for r, t, b, u in results]}
         _store(cache)
     return opts

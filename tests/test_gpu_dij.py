@@ -143,12 +143,13 @@ def test_autotuner_picks_a_measured_setting(tmp_path, monkeypatch):
     c = omc.Context(ph_h.media)
     ph = c.phantom(ph_h, ecut=0.7, pcut=0.01)
     spec = omc.Spectrum.mono(2.0)
-    best = tune.tune_dij(c, ph, spec, bix, nbatch=4, sample=8, targets=(4096, 65536), budgets=(2, 8))
+    best = tune.tune_dij(c, ph, spec, bix, nbatch=4, sample=8, targets=(4096, 65536), budgets=(2, 8), turns=(3, -1))
     assert best["event_target"] in (4096, 65536) and best["electron_budget"] in (2, 8)
+    assert best["electron_turns"] in (3, -1)
     assert (tmp_path / "tune.json").exists()
     assert tune.tune_dij(c, ph, spec, bix, nbatch=4, sample=8) == best          # from the cache
     a = c.run_dij(ph, spec, bix, nbatch=4, seed=5, **best)[0]
-    b = c.run_dij(ph, spec, bix, nbatch=4, seed=5, event_target=999, electron_budget=3)[0]
+    b = c.run_dij(ph, spec, bix, nbatch=4, seed=5, event_target=999, electron_budget=3, electron_turns=2)[0]
     c.close()
     _compare_csc(a, b, "tuned vs other setting")
 
@@ -177,3 +178,27 @@ def test_dij_batches_per_job_same_matrix(tmp_path):
     for other, what in ((two, "2 batches per job"), (all_, "all batches per job"), (lanes, "two streams")):
         _compare_csc(other[0], one[0], what)
         _compare_csc(other[1], one[1], what + " (variance)")
+
+
+def test_repeated_runs_reuse_cached_memory(tmp_path):
+    """Run-scoped device blocks go back to a per-device cache and are handed
+    out again with whatever the last run left in them: a Dij run, a dose run,
+    and both again must repeat their results (to FP64 summation order), so
+    every buffer that has to start at zero is cleared by its run."""
+    ph_h, bix, _ = _case(tmp_path, nhist=(400, 800))
+    c = omc.Context(ph_h.media)
+    ph = c.phantom(ph_h, ecut=0.7, pcut=0.01)
+    spec = omc.Spectrum.mono(2.0)
+    kw = dict(nbatch=5, rel_threshold=1e-4, want_var=True, want_plan=True, precision=omc.FP64,
+              form=omc.FORM_EVENT, seed=3, group=3)
+    dkw = dict(collimator=(-1, 1, -1, 1), precision=omc.FP64, form=omc.FORM_EVENT, seed=9)
+    a = c.run_dij(ph, spec, bix, **kw)
+    s1, q1, _ = c.run_dosxyz(ph, spec, 1000, 5, **dkw)
+    b = c.run_dij(ph, spec, bix, **kw)
+    s2, q2, _ = c.run_dosxyz(ph, spec, 1000, 5, **dkw)
+    c.close()
+    _compare_csc(b[0], a[0], "repeated Dij")
+    _compare_csc(b[1], a[1], "repeated Dij (variance)")
+    assert np.allclose(b[2][0], a[2][0], rtol=1e-10, atol=0)
+    assert np.allclose(s2, s1, rtol=1e-10, atol=1e-12 * s1.max())
+    assert np.allclose(q2, q1, rtol=1e-10, atol=1e-12 * q1.max())
