@@ -55,6 +55,16 @@ def cpu_cores():
     return max(1, min(16, n))
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def kernel_bytes(st, k):
     """Algorithmic bytes of event kernel class k (0 photon, 1 charged):
     72 per transport event (howfar/hownear call: read + write of the 36-byte
@@ -89,30 +99,46 @@ def c3_config():
 
 
 # ------------------------------------------------------------------ serving mix
-def serve_mix(ctx, rate_hist_per_s, nreq=12, load=0.8, seed=20261206, precision=32):
-    """A short Poisson-arrival request mix (phantoms 64^3 .. 192x192x128 at
-    0.2-0.5 cm, media water/lung/bone/air, half full-dose requests with
-    1e5..1e7 histories, half Dij requests with 1-9 beams); offered load is
-    `load` times the measured batched throughput."""
+_PHANTOM_POOL = {}
+
+
+def serve_mix(ctx, rate_hist_per_s, nreq=12, load=0.8, seed=20261206, precision=32, max_hist=1e8):
+    """A Poisson-arrival request mix with the ranges of the serving workload:
+    phantoms 64^3 .. 256x256x160 at 0.2-0.5 cm (a pool of at most 8, reused),
+    media water/lung/bone/air, 1e5..max_hist histories per request
+    (log-uniform), half full-dose requests (10 batches) and half Dij requests
+    over 1-9 beams.  The offered load is `load` times the measured batched
+    throughput.  200 requests per load reproduce the full workload; the
+    default takes 12."""
     import ompmc_amd as omc
     from ompmc_amd import phantoms as P
     rng = np.random.default_rng(seed)
     e, w = P.synthetic_6mv()
     spec = omc.Spectrum.from_histogram(e, w)
+    key = (seed, min(nreq, 8))
+    if key not in _PHANTOM_POOL:          # the same phantoms at every offered load
+        pool = []
+        for _ in range(key[1]):
+            nx = int(rng.integers(64, 257)); nz = int(rng.integers(64, 161))
+            vox = float(rng.uniform(0.2, 0.5))
+            pool.append((P.synthetic_ct(nx=nx, ny=nx, nz=nz, vx=vox, vy=vox, vz=vox,
+                                        seed=int(rng.integers(1 << 30)), target_radius=min(2.0, 0.2 * nx * vox)),
+                         nx, vox))
+        _PHANTOM_POOL[key] = pool
+    pool = _PHANTOM_POOL[key]
+    rng = np.random.default_rng(seed + 1)
     reqs = []
     for i in range(nreq):
-        nx = int(rng.integers(64, 193)); nz = int(rng.integers(64, 129))
-        vox = float(rng.uniform(0.2, 0.5))
-        ph = P.synthetic_ct(nx=nx, ny=nx, nz=nz, vx=vox, vy=vox, vz=vox, seed=int(rng.integers(1 << 30)),
-                            target_radius=min(2.0, 0.2 * nx * vox))
+        ph, nx, vox = pool[i % len(pool)]
+        nh = int(10 ** rng.uniform(5, math.log10(max_hist)))
         if i % 2 == 0:
-            nh = int(10 ** rng.uniform(5, 7))
             half = min(5.0, 0.3 * nx * vox)
-            reqs.append(("dose", ph, nh, dict(collimator=(-half, half, -half, half))))
+            reqs.append(("dose", ph, nh // 10 * 10, dict(collimator=(-half, half, -half, half))))
         else:
             nb = int(rng.integers(1, 10))
             bix = P.beamlets(nbeams=nb, radius=min(2.0, 0.25 * nx * vox), hist_median=3000, hist_sigma_log10=0.3,
                              hist_range=(1000, 20000), seed=int(rng.integers(1 << 30)))
+            bix.nhist = np.full(bix.n, max(10, nh // bix.n), np.int64)
             reqs.append(("dij", ph, int(bix.nhist.sum()), dict(bix=bix)))
     mean_h = np.mean([r[2] for r in reqs])
     rate = load * rate_hist_per_s / mean_h               # requests per second
@@ -136,10 +162,14 @@ def serve_mix(ctx, rate_hist_per_s, nreq=12, load=0.8, seed=20261206, precision=
     bad = [r for r in res if r["error"]]
     lat = np.array([r["end_t"] - r["submit_t"] for r in res])
     hist = sum(r["histories"] for r in res)
+    kinds = np.array([r[0] for r in reqs])
+    per_kind = {k: {"p50_s": float(np.percentile(lat[kinds == k], 50)), "p99_s": float(np.percentile(lat[kinds == k], 99))}
+                for k in ("dose", "dij")}
     return dict(requests=nreq, failed=len(bad), offered_load=load, p50_s=float(np.percentile(lat, 50)),
-                p99_s=float(np.percentile(lat, 99)), histories_per_s=hist / wall, wall_s=wall,
-                mix="Poisson arrivals; phantoms 64^3..192x192x128 at 0.2-0.5 cm; half dose 1e5..1e7 histories, "
-                    "half Dij 1-9 beams")
+                p99_s=float(np.percentile(lat, 99)), histories_per_s=hist / wall, wall_s=wall, by_kind=per_kind,
+                histories=float(hist),
+                mix=f"Poisson arrivals; phantoms 64^3..256x256x160 at 0.2-0.5 cm; 1e5..{max_hist:.0e} histories "
+                    "per request (log-uniform); half dose, half Dij over 1-9 beams")
 
 
 # ------------------------------------------------------------------ CPU baseline
@@ -159,17 +189,23 @@ def cpu_baseline_c3(ph, bix, spec, nbatch, seconds_target=12.0):
                   side1=bix.side1.reshape(-1), side2=bix.side2.reshape(-1), gaussian=0, sigma=0.0)
         nper = np.maximum(np.asarray(bix.nhist) // nbatch, 1).astype(np.int32)
 
-        def run(nb):
+        def run(nb, threads, div=1):
             sub = {k: (v[:nb] if k == "ibeam" else v[:3 * nb] if k in ("corner", "side1", "side2") else v)
                    for k, v in bd.items()}
-            _, _, secs = s.matrad_omp(spec, sub, nper[:nb], nbatch, cores)
-            return secs, int(nper[:nb].sum()) * nbatch
-        secs, nh = run(2)
+            np_ = np.maximum(nper[:nb] // div, 1).astype(np.int32)
+            _, _, secs = s.matrad_omp(spec, sub, np_, nbatch, threads)
+            return secs, int(np_.sum()) * nbatch
+        secs, nh = run(2, cores)
         nb = int(min(bix.n, max(2, 2 * seconds_target / max(secs, 1e-3))))
-        secs, nh = run(nb)
+        secs, nh = run(nb, cores)
+        # one thread on the same beamlets with 1/cores of their histories
+        secs1, nh1 = run(nb, 1, cores)
         return dict(value=nh / secs, unit="histories/s", cores=cores, kind="port",
                     sample=f"first {nb} beamlets of the plan ({nh} histories, {nbatch} batches), oracle "
-                           f"restatement of the reference's matRad loop, OpenMP {cores} threads, {secs:.1f} s")
+                           f"restatement of the reference's matRad loop, OpenMP {cores} threads, {secs:.1f} s",
+                    one_core={"value": nh1 / secs1, "sample": f"the same {nb} beamlets with 1/{cores} of their "
+                                                               f"histories ({nh1}), 1 thread, {secs1:.1f} s"},
+                    host={"cpu_model": cpu_model(), "nproc": os.cpu_count(), "threads_allowed": cores})
 
 
 # ------------------------------------------------------------------ c3
@@ -254,6 +290,16 @@ def bench_c3(a):
                      "per_launch_bytes": bph * hist / max(klaunch[dom], 1),
                      "per_launch_seconds": ksec[dom] / max(klaunch[dom], 1)},
     }
+    # secondary roofline: FP64 tally atomics.  Every deposit call counts as one
+    # atomic (an upper bound: a lane folds repeated deposits to one region into
+    # one add), against the device's measured rate of scattered FP64 atomic adds
+    ndep = stc.n_dep / stc.histories
+    peak_small = omc.atomic_probe(1 << 18, 1 << 26)
+    peak_large = omc.atomic_probe(1 << 27, 1 << 27)
+    out["atomic_roofline"] = {"bound": "fp64 atomics", "deposits_per_history": ndep, "achieved": ndep * value,
+                              "peak": peak_large, "unit": "atomics/s", "frac": ndep * value / peak_large,
+                              "probe": {"table_2MiB": peak_small, "table_1GiB": peak_large}}
+    log(f"atomics: {ndep:.1f} deposits/history -> {ndep * value:.3g}/s against {peak_large:.3g}/s measured")
     if not a.no_plain:
         sub = subset(bix, 20)
         _, _, _, sp = ctx.run_dij(gph, spec, sub, nbatch=nbatch, rel_threshold=rel, precision=prec,
@@ -296,7 +342,8 @@ def bench_c3(a):
         log(f"fp32/fp64: {d:+.3e} +- {sig:.1e}")
     if not a.no_serve:
         loads = [float(x) for x in a.serve_loads.split(",")]
-        runs = [serve_mix(ctx, value, precision=prec, load=ld, nreq=a.serve_requests) for ld in loads]
+        runs = [serve_mix(ctx, value, precision=prec, load=ld, nreq=a.serve_requests, max_hist=a.serve_max_hist)
+                for ld in loads]
         out["serving"] = runs[0] if len(runs) == 1 else {"loads": runs}
         log(f"serving: {out['serving']}")
     cpu = None
@@ -387,9 +434,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--subset", type=int, default=1, help="c3: every n-th beamlet only (profiling runs)")
-    ap.add_argument("--serve-loads", default="0.8",
+    ap.add_argument("--serve-loads", default="0.5,0.8,1.0",
                     help="offered loads of the served mix, as fractions of the batched throughput (e.g. 0.5,0.8,1.0)")
-    ap.add_argument("--serve-requests", type=int, default=12, help="requests per served mix")
+    ap.add_argument("--serve-requests", type=int, default=12, help="requests per served mix (200: the full workload)")
+    ap.add_argument("--serve-max-hist", type=float, default=1e8, help="largest request, in histories")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device once (PMC scale of FETCH_SIZE / WRITE_SIZE) and exit")
     a = ap.parse_args()

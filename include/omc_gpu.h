@@ -228,6 +228,14 @@ int omc_sampler_eval(omc_ctx *ctx, int phantom_id, int precision, int ncase, con
                      int in_stride, const uint32_t *rn, const int64_t *rn_off, double *out, int out_stride,
                      int64_t *rn_used);
 
+/* ----------------------------------------------------------------- measurement */
+/* No counterpart in the reference: the FP64 atomic throughput of the device,
+ * the secondary roofline of the tallies.  Adds 1.0 at nops addresses drawn
+ * uniformly from a table of nslot doubles (a power of two, 2^10..2^30) with
+ * the hardware FP64 atomic add the tallies use; *ops_per_second is the rate
+ * of one timed launch after a warm-up launch. */
+int omc_atomic_probe(int device, int64_t nslot, int64_t nops, double *ops_per_second);
+
 /* ----------------------------------------------------------------- serving */
 /* A resident request loop on the context's device: requests (dose or Dij
  * over their own phantom) are admitted while their device memory fits the

@@ -133,6 +133,7 @@ def lib():
     L = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     L.omc_last_error.restype = C.c_char_p
+    L.omc_atomic_probe.argtypes = [C.c_int, C.c_int64, C.c_int64, C.POINTER(C.c_double)]
     L.omc_api_version.restype = C.c_int
     L.omc_ctx_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(C.c_char_p), C.c_char_p, C.c_char_p,
                                  C.c_char_p, C.c_int]
@@ -401,6 +402,14 @@ class Context:
                                       rn.ctypes.data_as(_u32), off.ctypes.data_as(_i64), _dp(out), out_stride,
                                       used.ctypes.data_as(_i64)))
         return out, used
+
+
+def atomic_probe(nslot, nops, device=0):
+    """FP64 atomic adds per second at uniformly scattered addresses of a
+    table of nslot doubles (omc_atomic_probe)."""
+    r = C.c_double(0)
+    _check(lib().omc_atomic_probe(int(device), int(nslot), int(nops), C.byref(r)))
+    return r.value
 
 
 class Server:

@@ -227,10 +227,9 @@ template <class R> static int upload_tables(omc_ctx *c, DevArena &A, Tables<R> &
     T.range_ep = vec(h.range_ep, 2 * nek);
     T.e_array = vec(h.e_array, nek);
     T.spin_rej = vec(h.spin_rej, (size_t)nm * OMC_NSPIN_MED + 1);   // + the zero entry of quirk Q17
-    T.ums = vec(h.ums, OMC_NMS);
-    T.fms = vec(h.fms, OMC_NMS);
-    T.wms = vec(h.wms, OMC_NMS);
-    T.ims = A.upload(std::vector<int>(h.ims, h.ims + OMC_NMS));
+    std::vector<MsBin<R>> msb(OMC_NMS);
+    for (size_t n = 0; n < (size_t)OMC_NMS; n++) msb[n] = MsBin<R>{(R)h.wms[n], h.ims[n], (R)h.fms[n], (R)h.ums[n]};
+    T.ms = A.upload(msb);
     T.b2spin_min = (R)h.b2spin_min; T.dbeta2i = (R)h.dbeta2i; T.espml = (R)h.espml; T.dleneri = (R)h.dleneri;
     T.dqq1i = (R)h.dqq1i; T.dllambi = (R)h.dllambi; T.dqmsi = (R)h.dqmsi;
     if (!A.ok) return omc_seterr(OMC_ERR_DEVICE, "table upload failed: %s", hipGetErrorString(hipGetLastError()));
@@ -698,4 +697,37 @@ extern "C" int omc_sampler_eval(omc_ctx *c, int phantom_id, int precision, int n
     (void)hipSetDevice(c->device);
     return precision == OMC_FP64 ? sampler_eval<double>(c, pd, ncase, kind, in, in_stride, rn, rn_off, out, out_stride, rn_used)
                                  : sampler_eval<float>(c, pd, ncase, kind, in, in_stride, rn, rn_off, out, out_stride, rn_used);
+}
+
+// ------------------------------------------------------------------ measurement
+extern "C" int omc_atomic_probe(int device, int64_t nslot, int64_t nops, double *ops_per_second) {
+    if (!ops_per_second || nslot < 1024 || nslot > (1LL << 30) || (nslot & (nslot - 1)) || nops < 1)
+        return omc_seterr(OMC_ERR_ARG, "omc_atomic_probe: nslot a power of two in 2^10..2^30, nops > 0");
+    int rc = OMC_OK;
+    if (hipSetDevice(device) != hipSuccess) return omc_seterr(OMC_ERR_DEVICE, "device %d", device);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
+    const int grid = cus * 16;
+    const long lanes = (long)grid * 256;
+    const int nper = (int)std::max<long>(1, (long)((nops + lanes - 1) / lanes));
+    DevArena A;
+    double *t = A.alloc_n<double>((size_t)nslot);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.f;
+    if (!A.ok) { rc = omc_seterr(OMC_ERR_NOMEM, "probe table of %lld doubles", (long long)nslot); goto done; }
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipMemset(t, 0, sizeof(double) * nslot));
+    HIPCHK(launch_atomic_probe(t, (unsigned long long)(nslot - 1), grid, nper, 1, nullptr));
+    HIPCHK(hipEventRecord(e0, nullptr));
+    HIPCHK(launch_atomic_probe(t, (unsigned long long)(nslot - 1), grid, nper, 2, nullptr));
+    HIPCHK(hipEventRecord(e1, nullptr));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *ops_per_second = (double)lanes * nper / (ms * 1e-3);
+done:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    A.release();
+    return rc;
 }

@@ -24,6 +24,7 @@
 //    new top, p lies below it), which is exactly the reference's LIFO order.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace omc {
@@ -93,11 +94,34 @@ template <> struct M<float> {
 // interleaved pair per (medium, interval) so one 8/16-byte load serves a lookup.
 template <class R> struct PW { R c1, c0; };
 
+// One bin of a row of the q^(2+) surface (msnew tables): alias probability,
+// alias bin within the row, and the bin's (a, u) pair, interleaved so that a
+// sampling trial reads one 16-byte entry (a second one when it takes the
+// alias) instead of a chain of loads from four arrays.
+template <class R> struct alignas(16) MsBin {
+    R w;
+    int i;
+    R f, u;
+};
+static_assert(sizeof(MsBin<float>) == 16 && sizeof(MsBin<double>) == 32 && offsetof(MsBin<double>, f) == 16,
+              "the bin loads below assume this layout");
+
 // Loads through pointers into device tables and phantom arrays, which live
 // in global memory: addressed as such they compile to global loads instead
 // of flat accesses.
 template <class T> __device__ __forceinline__ T gld(const T *p) {
     return *(const __attribute__((address_space(1))) T *)p;
+}
+typedef float omc_f4 __attribute__((ext_vector_type(4)));
+typedef double omc_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ MsBin<float> gld(const MsBin<float> *p) {
+    const omc_f4 v = *(const __attribute__((address_space(1))) omc_f4 *)p;      // one 16-byte load
+    return MsBin<float>{v.x, __float_as_int(v.y), v.z, v.w};
+}
+__device__ __forceinline__ MsBin<double> gld(const MsBin<double> *p) {
+    const __attribute__((address_space(1))) omc_d2 *q = (const __attribute__((address_space(1))) omc_d2 *)p;
+    const omc_d2 a = q[0], b = q[1];                                             // two 16-byte loads
+    return MsBin<double>{a.x, (int)(__double_as_longlong(a.y) & 0xffffffffLL), b.x, b.y};
 }
 template <class R> __device__ __forceinline__ PW<R> gld(const PW<R> *p) {
     const __attribute__((address_space(1))) R *q = (const __attribute__((address_space(1))) R *)p;
@@ -129,8 +153,7 @@ template <class R> struct Tables {
     const R *range_ep;                                          // [2][nmed][MXEKE]
     const R *e_array;                                           // [nmed][MXEKE]
     const R *spin_rej;                                          // [nmed][NSPIN_MED]
-    const R *ums, *fms, *wms;
-    const int *ims;
+    const MsBin<R> *ms;                                         // [NMS] q^(2+) surface bins
     R b2spin_min, dbeta2i, espml, dleneri, dqq1i, dllambi, dqmsi;
 };
 
@@ -1077,15 +1100,16 @@ template <class R, class G, class T> struct Engine {
                                 ((R)-2.77164 + llmbda * ((R)2.94874 - llmbda * ((R)0.1535754 - llmbda * (R)0.00552888)));
                 find_index = false;
             }
-            int base = (ms.i * (MXQ_MS + 1) + ms.j) * (MXU_MS + 1);
+            const MsBin<R> *row = t.ms + (ms.i * (MXQ_MS + 1) + ms.j) * (MXU_MS + 1);
             do {
                 xi = rnd();
                 R ak = xi * (R)MXU_MS;
                 int k = (int)ak;
                 ak -= (R)k;
-                if (ak > gld(t.wms + (base + k))) k = gld(t.ims + (base + k));
-                R a = gld(t.fms + (base + k));
-                R u = gld(t.ums + (base + k));
+                MsBin<R> bin = gld(row + k);
+                if (ak > bin.w) bin = gld(row + bin.i);
+                R a = bin.f;
+                R u = bin.u;
                 R du = (R)0;
                 xi = rnd();
                 if (M<R>::fabs(a) < (R)0.2) {

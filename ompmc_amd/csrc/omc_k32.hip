@@ -15,6 +15,28 @@ __global__ void k_accumulate(double *e, double *sum, double *sum2, long n, doubl
     }
 }
 
+// FP64 atomic throughput probe: every lane adds 1.0 at `nper` addresses of a
+// splitmix64 sequence over mask + 1 doubles, with the hardware FP64 atomic
+// add the tallies use (global_atomic_add_f64).
+__global__ void __launch_bounds__(256) k_atomic_probe(double *t, unsigned long long mask, int nper,
+                                                      unsigned long long seed) {
+    unsigned long long x = seed + (unsigned long long)(blockIdx.x * blockDim.x + threadIdx.x) * 0xD1B54A32D192ED03ull;
+    for (int k = 0; k < nper; k++) {
+        x += 0x9E3779B97F4A7C15ull;
+        unsigned long long z = x;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        atomicAdd(t + (z & mask), 1.0);
+    }
+}
+
+hipError_t launch_atomic_probe(double *t, unsigned long long mask, int grid, int nper, unsigned long long seed,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(k_atomic_probe, dim3(grid), dim3(256), 0, st, t, mask, nper, seed);
+    return hipGetLastError();
+}
+
 hipError_t launch_accumulate(double *e, double *sum, double *sum2, long n, double scale, hipStream_t st) {
     int block = 256;
     long g = (n + block - 1) / block;

@@ -12,6 +12,9 @@ template <class R> hipError_t launch_history(const HistArgs<R> &a, int rng, bool
                                              hipStream_t st);
 template <class R> hipError_t launch_sampler(const SampArgs<R> &a, hipStream_t st);
 hipError_t launch_accumulate(double *e, double *sum, double *sum2, long n, double scale, hipStream_t st);
+// FP64 atomic throughput probe (grid x 256 lanes, nper scattered adds each)
+hipError_t launch_atomic_probe(double *t, unsigned long long mask, int grid, int nper, unsigned long long seed,
+                               hipStream_t st);
 
 // Everything the rounds of an event-queue job need.
 template <class R> struct EvLaunch {

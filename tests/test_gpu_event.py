@@ -199,3 +199,14 @@ def test_event_counts_and_kernel_times(slab):
     assert st.events_photon > 0 and st.events_charged > 0 and st.rounds > 0
     assert st.kernel_seconds[2] > 0 and st.kernel_seconds[3] > 0
     assert st.peak_inflight >= 1
+
+
+def test_atomic_probe_measures_a_rate():
+    """The FP64 atomic probe (secondary roofline of the tallies) runs and gives
+    a plausible rate: scattered adds over a 512 MiB table are no faster than
+    over a 512 KiB one, and both are between 1e8 and 1e13 per second."""
+    small = omc.atomic_probe(1 << 16, 1 << 24)
+    large = omc.atomic_probe(1 << 26, 1 << 26)
+    print(f"FP64 atomics: {small:.3g}/s over 512 KiB, {large:.3g}/s over 512 MiB")
+    assert 1e8 < large < 1e13 and 1e8 < small < 1e13
+    assert large <= 1.2 * small
