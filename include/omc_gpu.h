@@ -133,6 +133,8 @@ typedef struct omc_run_opts {
                                (0 = as many as the memory budget holds, 1 = one batch per job) */
     int streams;            /* event-form Dij: beamlet groups transported concurrently, each on its own
                                stream (0 = automatic: up to 3 when the run has several groups; 1..4) */
+    int64_t max_particle_steps; /* event form: condensed-history steps after which an electron counts as
+                               stuck, is dropped and the run fails with OMC_ERR_LIMIT (0 = 2^20) */
 } omc_run_opts;
 
 /* Kernel classes of omc_run_stats.kernel_seconds / kernel_launches */

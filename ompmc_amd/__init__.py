@@ -60,7 +60,7 @@ class RunOpts(C.Structure):
                 ("grid", C.c_int), ("block", C.c_int), ("count_events", C.c_int), ("event_target", C.c_int64),
                 ("electron_budget", C.c_int), ("group", C.c_int), ("hbm_budget", C.c_int64),
                 ("voxel_walk", C.c_int), ("electron_turns", C.c_int), ("batches_per_job", C.c_int),
-                ("streams", C.c_int)]
+                ("streams", C.c_int), ("max_particle_steps", C.c_int64)]
 
 
 class RunStats(C.Structure):
@@ -183,7 +183,7 @@ def default_paths():
 
 def run_opts(precision=FP32, form=FORM_EVENT, seed=1802, rng=RNG_PHILOX, replay=None, replay_offsets=None,
              grid=0, block=0, count_events=False, nsplit=1, event_target=0, electron_budget=0, group=0,
-             hbm_budget=0, voxel_walk=False, electron_turns=0, batches_per_job=0, streams=0):
+             hbm_budget=0, voxel_walk=False, electron_turns=0, batches_per_job=0, streams=0, max_particle_steps=0):
     o = RunOpts()
     lib().omc_run_opts_default(C.byref(o))
     o.precision, o.form, o.seed, o.rng, o.nsplit = precision, form, seed, rng, nsplit
@@ -192,6 +192,7 @@ def run_opts(precision=FP32, form=FORM_EVENT, seed=1802, rng=RNG_PHILOX, replay=
     o.event_target, o.electron_budget, o.group, o.hbm_budget = event_target, electron_budget, group, hbm_budget
     o.voxel_walk = int(bool(voxel_walk))
     o.electron_turns, o.batches_per_job, o.streams = electron_turns, batches_per_job, streams
+    o.max_particle_steps = max_particle_steps
     keep = []
     if replay is not None:
         rp = np.ascontiguousarray(replay, np.uint32)

@@ -505,7 +505,7 @@ int omc_check_opts(const omc_run_opts *o) {
     if (o->rng == OMC_RNG_REPLAY && o->form != OMC_FORM_HISTORY)
         return omc_seterr(OMC_ERR_ARG, "replayed random numbers need the history form (one stream per history)");
     if (o->event_target < 0 || o->electron_budget < 0 || o->group < 0 || o->hbm_budget < 0 || o->grid < 0 ||
-        o->block < 0 || o->batches_per_job < 0)
+        o->block < 0 || o->batches_per_job < 0 || o->max_particle_steps < 0)
         return omc_seterr(OMC_ERR_ARG, "negative launch option");
     if (o->electron_turns < -1) return omc_seterr(OMC_ERR_ARG, "electron_turns %d (-1 no limit, 0 automatic)", o->electron_turns);
     if (o->streams < 0 || o->streams > 4) return omc_seterr(OMC_ERR_ARG, "streams %d (0 automatic, 1..4)", o->streams);

@@ -221,6 +221,10 @@ template <class R> void fill_launch(omc_ctx *c, const PhantomDev *pd, EvState<R>
     L.cap = S.cap;
     L.budget = o->electron_budget > 0 ? o->electron_budget : DEFAULT_BUDGET;
     L.turns = o->electron_turns > 0 ? o->electron_turns : (o->electron_turns < 0 ? 0 : DEFAULT_TURNS);
+    // a particle still going after 2^20 condensed-history steps (a hundred
+    // times the longest tracks seen) is taken as stuck
+    const long steps = o->max_particle_steps > 0 ? (long)o->max_particle_steps : (1L << 20);
+    L.max_age = L.turns > 0 ? (int)std::min<long>(steps / ((long)L.turns * L.budget) + 1, 1L << 22) : 0;
     L.delta = o->voxel_walk ? 0 : 1;
     int cus = device_cus(c->device);
     L.grid_event = o->grid > 0 ? o->grid : cus * 8;

@@ -27,7 +27,7 @@ template <class R> struct Pool {
     R *e, *x, *y, *z, *u, *v, *w, *wt;
     R *demfp, *sig0;             // resumable electron state (charged pool)
     int *ix, *iy, *iz, *iq;
-    int *phase;
+    int *phase;                  // electron_run phase (bits 0-7) and the particle's requeue count (bits 8-31)
     int *slot;                   // tally slot (beamlet of a Dij group); 0 for dose runs
     uint32_t *ctr;               // next draw of the particle's stream
     uint64_t *stream;
@@ -90,6 +90,7 @@ template <class R> struct EvArgs {
     int *err;                         // EvCtl::err
     int budget;                       // electron ustep iterations between refills of a wave
     int turns;                        // electrons: refills a particle may hold its lane before it is requeued (0: no limit)
+    int max_age;                      // electrons: requeues after which a particle counts as stuck (dropped, err bit 1)
     int delta;                        // photons: 1 delta tracking, 0 voxel-by-voxel flight
 };
 
@@ -179,7 +180,10 @@ __device__ __forceinline__ void flush_counters(unsigned long long *dst, const Co
 // electron in single-scattering mode along a voxel face can take thousands
 // of steps, and without the hand-back it alone would hold the round -- and
 // every particle queued behind the round -- open for tens of milliseconds.
-// Where a particle is requeued does not change its track.
+// Where a particle is requeued does not change its track.  The requeue count
+// travels with the particle (high bits of its phase word): a particle that
+// would make no progress is dropped and reported after a.max_age requeues,
+// instead of cycling through rounds until the job's round guard ends it.
 template <class R, bool PHOTON, bool COUNT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PHOTON ? 1 : OMC_CHARGED_WAVES_PER_EU)))
 k_event(EvArgs<R> a) {
@@ -193,7 +197,7 @@ k_event(EvArgs<R> a) {
     PhiloxAt rng;
     TallyAtomic tally;
     tally.init(a.endep);
-    int iters = 0, turns = 0;
+    int iters = 0, turns = 0, age = 0;
     for (;;) {
         long long idx = -1;
         if (__any(!have)) {
@@ -209,6 +213,8 @@ k_event(EvArgs<R> a) {
             demfp = a.in.demfp[idx];
             sig0 = a.in.sig0[idx];
             phase = a.in.phase[idx];
+            age = phase >> 8;
+            phase &= 0xff;
             rng.at(a.seed, stream, a.in.ctr[idx]);
             tally.init(a.endep + (size_t)slot * a.slot_stride);
             have = true;
@@ -240,7 +246,13 @@ k_event(EvArgs<R> a) {
             }
             // d: -1 particle gone, 0 particle continues, +1 continues with a secondary
             done = d < 0 || ((p.iq == 0) != PHOTON);
-            if (!PHOTON && !done && a.turns > 0 && ++turns >= a.turns) done = true;   // requeue, see above
+            if (!PHOTON && !done && a.turns > 0 && ++turns >= a.turns) {            // requeue, see above
+                done = true;
+                if (++age > a.max_age) {
+                    atomicOr(a.err, 2);
+                    d = -1;
+                }
+            }
             if (done) tally.flush();
             if (COUNT) {
                 tot.nhowfar += e.cnt.nhowfar; tot.nhownear += e.cnt.nhownear; tot.ncross += e.cnt.ncross;
@@ -248,7 +260,7 @@ k_event(EvArgs<R> a) {
                 tot.nelectron += e.cnt.nelectron;
             }
         }
-        emit(a, have && done && d >= 0, p, demfp, sig0, phase, stream, qpos, slot);
+        emit(a, have && done && d >= 0, p, demfp, sig0, phase | (age << 8), stream, qpos, slot);
         emit(a, have && d == 1, q, (R)0, (R)0, 0, child_stream(stream, qpos), 0u, slot);
         if (done) have = false;
     }

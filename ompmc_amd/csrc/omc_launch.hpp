@@ -29,6 +29,7 @@ template <class R> struct EvLaunch {
     unsigned long long *counters;
     int budget;
     int turns;
+    int max_age;
     int delta;
     SrcArgs<R> src;              // source of the job (ctl and out are set per round)
     int grid_event, grid_source;

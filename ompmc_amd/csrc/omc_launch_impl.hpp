@@ -56,6 +56,7 @@ template <class R> hipError_t launch_round_part(const EvLaunch<R> &L, long r, in
         a.err = &L.ctl->err;
         a.budget = L.budget;
         a.turns = L.turns;
+        a.max_age = L.max_age;
         a.delta = L.delta;
         a.in = L.pool[par][part - 1];
         a.n_in = &L.ctl->cnt[par][part - 1];

@@ -210,3 +210,21 @@ def test_atomic_probe_measures_a_rate():
     print(f"FP64 atomics: {small:.3g}/s over 512 KiB, {large:.3g}/s over 512 MiB")
     assert 1e8 < large < 1e13 and 1e8 < small < 1e13
     assert large <= 1.2 * small
+
+
+def test_stuck_particle_guard_reports_limit():
+    """An electron still going after max_particle_steps condensed-history
+    steps is dropped and the run fails with OMC_ERR_LIMIT (4), the kernels
+    exiting normally; forced here from the host with a bound of 16 steps,
+    which 6 MeV electrons exceed, and checked not to trip at the default."""
+    ph_h = P.read_egsphant(golden("electron.egsphant"))
+    c = omc.Context(ph_h.media)
+    ph = c.phantom(ph_h, ecut=0.7, pcut=0.01)
+    with pytest.raises(omc.EngineError) as ei:
+        _run(c, ph, "electron", 400, 2, precision=omc.FP32, form=omc.FORM_EVENT, seed=4, electron_turns=1,
+             electron_budget=2, max_particle_steps=16)
+    assert ei.value.code == 4 and "progress" in str(ei.value)
+    _, _, st = _run(c, ph, "electron", 400, 2, precision=omc.FP32, form=omc.FORM_EVENT, seed=4, electron_turns=1,
+                    electron_budget=2)
+    c.close()
+    assert st.errors == 0
