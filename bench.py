@@ -222,8 +222,10 @@ def bench_c3(a):
     gph = ctx.phantom(ph, ecut=0.7, pcut=0.01)
     log(f"c3: {bix.n} beamlets, {int(np.sum(np.maximum(bix.nhist // nbatch, 1)) * nbatch)} histories per step, "
         f"setup {time.perf_counter() - t_setup:.1f} s")
+    # the matrices stay in device memory (inputs and outputs resident); the
+    # host-output rate, with the device-to-host copy, is reported beside it
     opts = dict(precision=prec, form=form, group=a.group, event_target=a.target, electron_budget=a.budget,
-                electron_turns=a.turns, batches_per_job=a.batches_per_job)
+                electron_turns=a.turns, batches_per_job=a.batches_per_job, csc_on_device=True, download=False)
     # event counts of the transport kernels (algorithmic bytes per history)
     sub20 = subset(bix, 20)
     _, _, _, stc = ctx.run_dij(gph, spec, sub20, nbatch=nbatch, rel_threshold=rel, count_events=True, **opts)
@@ -300,10 +302,19 @@ def bench_c3(a):
                               "peak": peak_large, "unit": "atomics/s", "frac": ndep * value / peak_large,
                               "probe": {"table_2MiB": peak_small, "table_1GiB": peak_large}}
     log(f"atomics: {ndep:.1f} deposits/history -> {ndep * value:.3g}/s against {peak_large:.3g}/s measured")
+    # the same step with the matrix returned in host memory, as the mex
+    # interface receives it (device-to-host copy and column stitching)
+    ho = dict(opts, csc_on_device=False, download=True)
+    t0 = time.perf_counter()
+    _, _, _, sth = ctx.run_dij(gph, spec, bix, nbatch=nbatch, rel_threshold=rel, **ho)
+    wh = time.perf_counter() - t0
+    out["host_output"] = {"histories_per_s": sth.histories / wh, "nnz": int(sth.nnz),
+                          "what": "one step returning the CSC in host memory (device-to-host copy included)"}
+    log(f"host output: {sth.histories / wh:.4g} histories/s")
     if not a.no_plain:
         sub = subset(bix, 20)
         _, _, _, sp = ctx.run_dij(gph, spec, sub, nbatch=nbatch, rel_threshold=rel, precision=prec,
-                                  form=omc.FORM_HISTORY, group=1)
+                                  form=omc.FORM_HISTORY, group=1, csc_on_device=True, download=False)
         out["plain_form"] = {"histories_per_s": sp.histories / sp.wall_seconds, "beamlets": int(sub.n),
                              "what": "history kernel, one beamlet at a time (reference loop order), every 20th beamlet"}
         out["speedup_vs_plain"] = value / out["plain_form"]["histories_per_s"]
@@ -317,7 +328,7 @@ def bench_c3(a):
         sp_ = scaled(subset(bix, 400, offset=7), k)
         _, _, _, stb = ctx.run_dij(gph, spec, sb, nbatch=nbatch, rel_threshold=rel, **opts)
         _, _, _, stp = ctx.run_dij(gph, spec, sp_, nbatch=nbatch, rel_threshold=rel, precision=prec,
-                                   form=omc.FORM_HISTORY, group=1)
+                                   form=omc.FORM_HISTORY, group=1, csc_on_device=True, download=False)
         rb, rp = stb.histories / stb.wall_seconds, stp.histories / stp.wall_seconds
         out["at_1pct_unc"] = {"history_scale": k, "histories_for_plan": float(hist / a.steps * k),
                               "batched_histories_per_s": rb, "batched_beamlets": int(sb.n),
@@ -330,7 +341,8 @@ def bench_c3(a):
         # precision; region (voxels above 20% of the plan maximum) fixed by an
         # FP64 pilot run with its own seed
         sub = subset(bix, 10)
-        kw = dict(nbatch=nbatch, rel_threshold=rel, want_plan=True, form=form, group=a.group)
+        kw = dict(nbatch=nbatch, rel_threshold=rel, want_plan=True, form=form, group=a.group, csc_on_device=True,
+                  download=False)
         pilot = ctx.run_dij(gph, spec, sub, precision=omc.FP64, seed=70, **kw)[2][0]
         reg = pilot > 0.2 * pilot.max()
         sums = {p: np.array([ctx.run_dij(gph, spec, sub, precision=p, seed=71 + 3 * i + (p == omc.FP64), **kw)[2][0][reg].sum()

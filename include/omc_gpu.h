@@ -135,6 +135,8 @@ typedef struct omc_run_opts {
                                stream (0 = automatic: up to 3 when the run has several groups; 1..4) */
     int64_t max_particle_steps; /* event form: condensed-history steps after which an electron counts as
                                stuck, is dropped and the run fails with OMC_ERR_LIMIT (0 = 2^20) */
+    int csc_on_device;      /* Dij: 1 = leave the matrices in device memory (omc_csc.device = 1) for a
+                               consumer on the GPU; 0 = host arrays, as the mex interface needs */
 } omc_run_opts;
 
 /* Kernel classes of omc_run_stats.kernel_seconds / kernel_launches */
@@ -194,13 +196,18 @@ typedef struct omc_beamlets_desc {
 
 /* Compressed sparse column matrix (MATLAB layout): column j holds rows
  * ir[jc[j]..jc[j+1]-1] with values val[]; allocated by the library, released
- * with omc_csc_free. */
+ * with omc_csc_free.  device = 1 (runs with csc_on_device): ir and val are
+ * device addresses on the run's device, jc is on the host. */
 typedef struct omc_csc {
     int64_t nrows, ncols, nnz;
     int64_t *jc, *ir;
     double *val;
+    int device;
 } omc_csc;
 void omc_csc_free(omc_csc *m);
+/* No counterpart in the reference: copies the rows and values of a matrix
+ * kept on the device (device = 1) into host arrays of nnz entries. */
+int omc_csc_download(const omc_csc *m, int64_t *ir, double *val);
 
 /* Dose-influence matrix with omc_matrad semantics (mexFunction,
  * omc_matrad.c:1576-1763): per bixel nbatch batches, dose per history in Gy,

@@ -60,7 +60,7 @@ class RunOpts(C.Structure):
                 ("grid", C.c_int), ("block", C.c_int), ("count_events", C.c_int), ("event_target", C.c_int64),
                 ("electron_budget", C.c_int), ("group", C.c_int), ("hbm_budget", C.c_int64),
                 ("voxel_walk", C.c_int), ("electron_turns", C.c_int), ("batches_per_job", C.c_int),
-                ("streams", C.c_int), ("max_particle_steps", C.c_int64)]
+                ("streams", C.c_int), ("max_particle_steps", C.c_int64), ("csc_on_device", C.c_int)]
 
 
 class RunStats(C.Structure):
@@ -91,12 +91,18 @@ class BeamletsDesc(C.Structure):
 
 class CSC(C.Structure):
     _fields_ = [("nrows", C.c_int64), ("ncols", C.c_int64), ("nnz", C.c_int64), ("jc", _i64), ("ir", _i64),
-                ("val", _d)]
+                ("val", _d), ("device", C.c_int)]
 
     def to_numpy(self):
-        """(jc, ir, val) copies; the library arrays stay owned by the struct."""
+        """(jc, ir, val) copies on the host (downloaded when the matrix is in
+        device memory); the library arrays stay owned by the struct."""
         jc = np.ctypeslib.as_array(self.jc, shape=(self.ncols + 1,)).copy()
         n = int(self.nnz)
+        if self.device:
+            ir, val = np.zeros(n, np.int64), np.zeros(n)
+            if n:
+                _check(lib().omc_csc_download(C.byref(self), ir.ctypes.data_as(_i64), _dp(val)))
+            return jc, ir, val
         ir = np.ctypeslib.as_array(self.ir, shape=(n,)).copy() if n else np.zeros(0, np.int64)
         val = np.ctypeslib.as_array(self.val, shape=(n,)).copy() if n else np.zeros(0)
         return jc, ir, val
@@ -134,6 +140,7 @@ def lib():
     vp = C.c_void_p
     L.omc_last_error.restype = C.c_char_p
     L.omc_atomic_probe.argtypes = [C.c_int, C.c_int64, C.c_int64, C.POINTER(C.c_double)]
+    L.omc_csc_download.argtypes = [C.POINTER(CSC), _i64, _d]
     L.omc_api_version.restype = C.c_int
     L.omc_ctx_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(C.c_char_p), C.c_char_p, C.c_char_p,
                                  C.c_char_p, C.c_int]
@@ -183,7 +190,8 @@ def default_paths():
 
 def run_opts(precision=FP32, form=FORM_EVENT, seed=1802, rng=RNG_PHILOX, replay=None, replay_offsets=None,
              grid=0, block=0, count_events=False, nsplit=1, event_target=0, electron_budget=0, group=0,
-             hbm_budget=0, voxel_walk=False, electron_turns=0, batches_per_job=0, streams=0, max_particle_steps=0):
+             hbm_budget=0, voxel_walk=False, electron_turns=0, batches_per_job=0, streams=0, max_particle_steps=0,
+             csc_on_device=False):
     o = RunOpts()
     lib().omc_run_opts_default(C.byref(o))
     o.precision, o.form, o.seed, o.rng, o.nsplit = precision, form, seed, rng, nsplit
@@ -193,6 +201,7 @@ def run_opts(precision=FP32, form=FORM_EVENT, seed=1802, rng=RNG_PHILOX, replay=
     o.voxel_walk = int(bool(voxel_walk))
     o.electron_turns, o.batches_per_job, o.streams = electron_turns, batches_per_job, streams
     o.max_particle_steps = max_particle_steps
+    o.csc_on_device = int(bool(csc_on_device))
     keep = []
     if replay is not None:
         rp = np.ascontiguousarray(replay, np.uint32)
@@ -340,10 +349,12 @@ class Context:
         return dose, unc
 
     def run_dij(self, phantom: Phantom, spectrum: Spectrum, bix: "phantoms.Beamlets", nbatch=10, rel_threshold=1e-4,
-                nhist_each=0, want_var=False, want_plan=False, gaussian=False, sigma=0.0, **opts):
+                nhist_each=0, want_var=False, want_plan=False, gaussian=False, sigma=0.0, download=True, **opts):
         """omc_matrad dose-influence matrix: returns (dij, var, plan, stats)
         with dij/var as (jc, ir, val) (var None unless asked; 0-based voxel
-        rows) and plan = (beamlet-summed dose, its variance) when asked."""
+        rows) and plan = (beamlet-summed dose, its variance) when asked.
+        With csc_on_device=True the matrices stay in device memory; they are
+        copied back unless download=False, which returns (jc, None, None)."""
         keep = []
         b = BeamletsDesc()
         b.nbixels, b.nbeams = bix.n, len(bix.src)
@@ -373,12 +384,15 @@ class Context:
         _check(lib().omc_run_dij_ex(self.h, phantom.id, C.byref(sp), C.byref(b), nbatch, rel_threshold, C.byref(o),
                                     C.byref(dij), C.byref(var) if want_var else None,
                                     _dp(pd) if want_plan else None, _dp(pv) if want_plan else None, C.byref(st)))
-        out = dij.to_numpy()
-        lib().omc_csc_free(C.byref(dij))
-        vout = None
-        if want_var:
-            vout = var.to_numpy()
-            lib().omc_csc_free(C.byref(var))
+        def take(m):
+            if m.device and not download:
+                r = (np.ctypeslib.as_array(m.jc, shape=(m.ncols + 1,)).copy(), None, None)
+            else:
+                r = m.to_numpy()
+            lib().omc_csc_free(C.byref(m))
+            return r
+        out = take(dij)
+        vout = take(var) if want_var else None
         del keep
         return out, vout, ((pd, pv) if want_plan else None), st
 

@@ -476,6 +476,7 @@ template <class R> struct DijLane {
     int64_t *d_ir = nullptr;
     double *d_val = nullptr, *d_vval = nullptr;
     hipEvent_t fin = nullptr;          // finish of the lane's latest group
+    std::vector<DevArena> gouts;       // csc_on_device: the compacted columns of this lane's groups
     JobStats js;
     double t_hist = 0, t_other = 0, histories = 0;
     uint64_t n_hist = 0, n_other = 0;
@@ -485,6 +486,7 @@ template <class R> struct DijLane {
         O.release();      // cached blocks: each release synchronises st, so all go before st does
         A.release();
         S.A.release();
+        for (auto &g : gouts) g.release();
         if (fin) (void)hipEventDestroy(fin);
         if (own_stream && st) (void)hipStreamDestroy(st);
     }
@@ -493,6 +495,9 @@ template <class R> struct DijLane {
 struct DijGroupOut {
     std::vector<int64_t> collen, ir;
     std::vector<double> val, var;
+    int64_t nnz = 0;
+    int64_t *d_ir = nullptr;           // csc_on_device: the columns in device memory
+    double *d_val = nullptr, *d_var = nullptr;
 };
 
 template <class R> struct DijRun {
@@ -501,7 +506,7 @@ template <class R> struct DijRun {
     const omc_run_opts *o;
     int nbix, nbatch, G, NBJ, nblk, nchunk, ngroups;
     long nreg, nvox;
-    bool event, count, timing, replay, want_var;
+    bool event, count, timing, replay, want_var, on_device;
     double rel;
     long long tgt;
     std::vector<long> nper, gfirst;
@@ -701,7 +706,22 @@ template <class R> int dij_lane_group(DijRun<R> &D, DijLane<R> &l, int gi) {
         int64_t begin = s ? off[(size_t)s * nchunk - 1] + counts[(size_t)s * nchunk - 1] : 0;
         out.collen[s] = end - begin;
     }
-    if ((size_t)tot > l.ocap) {
+    out.nnz = tot;
+    if (D.on_device) {
+        // compact straight into device buffers kept until the columns are stitched
+        l.gouts.emplace_back();
+        DevArena &GA = l.gouts.back();
+        GA.use_cache(stream);
+        out.d_ir = GA.alloc_n<int64_t>((size_t)std::max<int64_t>(tot, 1));
+        out.d_val = GA.alloc_n<double>((size_t)std::max<int64_t>(tot, 1));
+        out.d_var = D.want_var ? GA.alloc_n<double>((size_t)std::max<int64_t>(tot, 1)) : nullptr;
+        if (!GA.ok) return omc_seterr(OMC_ERR_NOMEM, "Dij device output (%lld entries)", (long long)tot);
+        if (tot > 0) {
+            CK(hipMemcpyAsync(l.d_off, off.data(), sizeof(int64_t) * ng * nchunk, hipMemcpyHostToDevice, stream));
+            CK(launch_dij_compact(l.d_s, l.d_s2, nreg, nvox, ng, l.d_thresh, l.d_off, out.d_ir, out.d_val, out.d_var,
+                                  stream));
+        }
+    } else if ((size_t)tot > l.ocap) {
         DevArena &O = l.O;
         O.release();
         l.ocap = std::max<size_t>((size_t)tot, l.ocap * 2);
@@ -710,7 +730,7 @@ template <class R> int dij_lane_group(DijRun<R> &D, DijLane<R> &l, int gi) {
         l.d_vval = D.want_var ? O.alloc_n<double>(l.ocap) : nullptr;
         if (!O.ok) return omc_seterr(OMC_ERR_NOMEM, "Dij output buffers (%lld entries)", (long long)tot);
     }
-    if (tot > 0) {
+    if (tot > 0 && !D.on_device) {
         CK(hipMemcpyAsync(l.d_off, off.data(), sizeof(int64_t) * ng * nchunk, hipMemcpyHostToDevice, stream));
         CK(launch_dij_compact(l.d_s, l.d_s2, nreg, nvox, ng, l.d_thresh, l.d_off, l.d_ir, l.d_val, l.d_vval, stream));
         out.ir.resize(tot);
@@ -770,6 +790,7 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
     D.timing = o->count_events != 0;
     D.replay = o->rng == OMC_RNG_REPLAY;
     D.want_var = var != nullptr;
+    D.on_device = o->csc_on_device != 0;
     D.rel = rel;
     const int nbix = D.nbix;
     const long nreg = D.nreg, nvox = D.nvox;
@@ -873,17 +894,26 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
     if (plan_var) CK(hipMemcpy(plan_var, D.d_tvar + 1, sizeof(double) * nvox, hipMemcpyDeviceToHost));
     // columns in beamlet order, group after group
     int64_t nnz = 0;
-    for (auto &g : D.outs) nnz += (int64_t)g.ir.size();
+    for (auto &g : D.outs) nnz += g.nnz;
     auto fill_csc = [&](omc_csc *m, bool variance) -> int {
+        memset(m, 0, sizeof *m);
         m->nrows = nvox;
         m->ncols = nbix;
         m->nnz = nnz;
+        m->device = D.on_device ? 1 : 0;
         m->jc = (int64_t *)malloc(sizeof(int64_t) * (nbix + 1));
-        m->ir = (int64_t *)malloc(sizeof(int64_t) * (nnz ? nnz : 1));
-        m->val = (double *)malloc(sizeof(double) * (nnz ? nnz : 1));
+        if (D.on_device) {
+            if (hipMalloc(&m->ir, sizeof(int64_t) * (nnz ? nnz : 1)) != hipSuccess) m->ir = nullptr;
+            if (hipMalloc(&m->val, sizeof(double) * (nnz ? nnz : 1)) != hipSuccess) m->val = nullptr;
+            (void)hipGetLastError();
+        } else {
+            m->ir = (int64_t *)malloc(sizeof(int64_t) * (nnz ? nnz : 1));
+            m->val = (double *)malloc(sizeof(double) * (nnz ? nnz : 1));
+        }
         if (!m->jc || !m->ir || !m->val) {
             omc_csc_free(m);
-            return omc_seterr(OMC_ERR_NOMEM, "host CSC of %lld entries", (long long)nnz);
+            return omc_seterr(OMC_ERR_NOMEM, "%s CSC of %lld entries", D.on_device ? "device" : "host",
+                              (long long)nnz);
         }
         int64_t e = 0;
         int col = 0;
@@ -893,12 +923,18 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
                 m->jc[col + 1] = m->jc[col] + len;
                 col++;
             }
-            if (!g.ir.empty()) {
-                memcpy(m->ir + e, g.ir.data(), sizeof(int64_t) * g.ir.size());
-                memcpy(m->val + e, (variance ? g.var : g.val).data(), sizeof(double) * g.ir.size());
-                e += (int64_t)g.ir.size();
+            if (g.nnz == 0) continue;
+            if (D.on_device) {
+                CK(hipMemcpyAsync(m->ir + e, g.d_ir, sizeof(int64_t) * g.nnz, hipMemcpyDeviceToDevice, stream));
+                CK(hipMemcpyAsync(m->val + e, variance ? g.d_var : g.d_val, sizeof(double) * g.nnz,
+                                  hipMemcpyDeviceToDevice, stream));
+            } else {
+                memcpy(m->ir + e, g.ir.data(), sizeof(int64_t) * g.nnz);
+                memcpy(m->val + e, (variance ? g.var : g.val).data(), sizeof(double) * g.nnz);
             }
+            e += g.nnz;
         }
+        if (D.on_device) CK(hipStreamSynchronize(stream));
         return OMC_OK;
     };
     if (dij && (rc = fill_csc(dij, false))) return rc;
@@ -973,11 +1009,26 @@ extern "C" int omc_run_dij(omc_ctx *c, int phantom_id, const omc_spectrum_desc *
 extern "C" void omc_csc_free(omc_csc *m) {
     if (!m) return;
     free(m->jc);
-    free(m->ir);
-    free(m->val);
+    if (m->device) {
+        if (m->ir) (void)hipFree(m->ir);
+        if (m->val) (void)hipFree(m->val);
+    } else {
+        free(m->ir);
+        free(m->val);
+    }
     m->jc = m->ir = nullptr;
     m->val = nullptr;
     m->nnz = 0;
+    m->device = 0;
+}
+
+extern "C" int omc_csc_download(const omc_csc *m, int64_t *ir, double *val) {
+    if (!m || !m->device || (m->nnz > 0 && (!ir || !val)))
+        return omc_seterr(OMC_ERR_ARG, "omc_csc_download: a device matrix and host arrays of nnz entries");
+    if (m->nnz == 0) return OMC_OK;
+    CK(hipMemcpy(ir, m->ir, sizeof(int64_t) * m->nnz, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(val, m->val, sizeof(double) * m->nnz, hipMemcpyDeviceToHost));
+    return OMC_OK;
 }
 
 // ------------------------------------------------------------------ request server

@@ -202,3 +202,21 @@ def test_repeated_runs_reuse_cached_memory(tmp_path):
     assert np.allclose(b[2][0], a[2][0], rtol=1e-10, atol=0)
     assert np.allclose(s2, s1, rtol=1e-10, atol=1e-12 * s1.max())
     assert np.allclose(q2, q1, rtol=1e-10, atol=1e-12 * q1.max())
+
+
+def test_dij_output_kept_on_device_matches_host_output(tmp_path):
+    """csc_on_device leaves the matrices in device memory (omc_csc.device = 1);
+    downloaded, they equal the host-output run's to FP64 summation order."""
+    ph_h, bix, _ = _case(tmp_path, nhist=(400, 800))
+    c = omc.Context(ph_h.media)
+    ph = c.phantom(ph_h, ecut=0.7, pcut=0.01)
+    spec = omc.Spectrum.mono(2.0)
+    kw = dict(nbatch=5, rel_threshold=1e-4, want_var=True, precision=omc.FP64, form=omc.FORM_EVENT, seed=13, group=3)
+    host = c.run_dij(ph, spec, bix, **kw)
+    dev = c.run_dij(ph, spec, bix, csc_on_device=True, **kw)
+    jc_only = c.run_dij(ph, spec, bix, csc_on_device=True, download=False, **kw)
+    c.close()
+    _compare_csc(dev[0], host[0], "device output")
+    _compare_csc(dev[1], host[1], "device output (variance)")
+    assert np.array_equal(jc_only[0][0], dev[0][0]) and jc_only[0][1] is None
+    assert dev[3].nnz == len(dev[0][1]) and abs(dev[3].nnz - host[3].nnz) <= max(1, host[3].nnz // 1000)
