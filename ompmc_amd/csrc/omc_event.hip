@@ -464,7 +464,7 @@ int upload_beamlets(DevArena &A, const omc_spectrum_desc *spec, const omc_beamle
 template <class R> struct DijLane {
     hipStream_t st = nullptr;
     bool own_stream = false;
-    DevArena A, O;
+    DevArena A;
     EvState<R> S;
     EvLaunch<R> L;
     HistArgs<R> H;
@@ -472,9 +472,6 @@ template <class R> struct DijLane {
     int *d_counts = nullptr, *d_slot_bix = nullptr, *d_slot_bt = nullptr;
     int64_t *d_off = nullptr;
     long *d_slot_off = nullptr;
-    size_t ocap = 0;
-    int64_t *d_ir = nullptr;
-    double *d_val = nullptr, *d_vval = nullptr;
     hipEvent_t fin = nullptr;          // finish of the lane's latest group
     std::vector<DevArena> gouts;       // csc_on_device: the compacted columns of this lane's groups
     JobStats js;
@@ -483,8 +480,7 @@ template <class R> struct DijLane {
     int rc = OMC_OK;
     std::string msg;
     ~DijLane() {
-        O.release();      // cached blocks: each release synchronises st, so all go before st does
-        A.release();
+        A.release();      // cached blocks: each release synchronises st, so all go before st does
         S.A.release();
         for (auto &g : gouts) g.release();
         if (fin) (void)hipEventDestroy(fin);
@@ -493,10 +489,9 @@ template <class R> struct DijLane {
 };
 
 struct DijGroupOut {
-    std::vector<int64_t> collen, ir;
-    std::vector<double> val, var;
+    std::vector<int64_t> collen;       // entries of each column of the group
     int64_t nnz = 0;
-    int64_t *d_ir = nullptr;           // csc_on_device: the columns in device memory
+    int64_t *d_ir = nullptr;           // the group's columns in device memory
     double *d_val = nullptr, *d_var = nullptr;
 };
 
@@ -707,8 +702,9 @@ template <class R> int dij_lane_group(DijRun<R> &D, DijLane<R> &l, int gi) {
         out.collen[s] = end - begin;
     }
     out.nnz = tot;
-    if (D.on_device) {
-        // compact straight into device buffers kept until the columns are stitched
+    {
+        // compact straight into device buffers kept until the columns are
+        // stitched (on the device, then downloaded once for host output)
         l.gouts.emplace_back();
         DevArena &GA = l.gouts.back();
         GA.use_cache(stream);
@@ -721,26 +717,6 @@ template <class R> int dij_lane_group(DijRun<R> &D, DijLane<R> &l, int gi) {
             CK(launch_dij_compact(l.d_s, l.d_s2, nreg, nvox, ng, l.d_thresh, l.d_off, out.d_ir, out.d_val, out.d_var,
                                   stream));
         }
-    } else if ((size_t)tot > l.ocap) {
-        DevArena &O = l.O;
-        O.release();
-        l.ocap = std::max<size_t>((size_t)tot, l.ocap * 2);
-        l.d_ir = O.alloc_n<int64_t>(l.ocap);
-        l.d_val = O.alloc_n<double>(l.ocap);
-        l.d_vval = D.want_var ? O.alloc_n<double>(l.ocap) : nullptr;
-        if (!O.ok) return omc_seterr(OMC_ERR_NOMEM, "Dij output buffers (%lld entries)", (long long)tot);
-    }
-    if (tot > 0 && !D.on_device) {
-        CK(hipMemcpyAsync(l.d_off, off.data(), sizeof(int64_t) * ng * nchunk, hipMemcpyHostToDevice, stream));
-        CK(launch_dij_compact(l.d_s, l.d_s2, nreg, nvox, ng, l.d_thresh, l.d_off, l.d_ir, l.d_val, l.d_vval, stream));
-        out.ir.resize(tot);
-        out.val.resize(tot);
-        CK(hipMemcpyAsync(out.ir.data(), l.d_ir, sizeof(int64_t) * tot, hipMemcpyDeviceToHost, stream));
-        CK(hipMemcpyAsync(out.val.data(), l.d_val, sizeof(double) * tot, hipMemcpyDeviceToHost, stream));
-        if (D.want_var) {
-            out.var.resize(tot);
-            CK(hipMemcpyAsync(out.var.data(), l.d_vval, sizeof(double) * tot, hipMemcpyDeviceToHost, stream));
-        }
     }
     if (D.timing) {
         CK(hipEventRecord(ev.ev[1], stream));
@@ -748,7 +724,7 @@ template <class R> int dij_lane_group(DijRun<R> &D, DijLane<R> &l, int gi) {
         l.t_other += ev.ms(0, 1) * 1e-3;
         l.n_other += 4;
     }
-    CK(hipStreamSynchronize(stream));   // counts/off and the output vectors are read by the host
+    CK(hipStreamSynchronize(stream));   // the host vectors counts/off go out of scope
     return OMC_OK;
 }
 
@@ -864,7 +840,6 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
             l.own_stream = true;
         }
         l.A.use_cache(l.st);
-        l.O.use_cache(l.st);
         l.S.A.use_cache(l.st);
         D.lanes.push_back(&l);
     }
@@ -902,18 +877,28 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
         m->nnz = nnz;
         m->device = D.on_device ? 1 : 0;
         m->jc = (int64_t *)malloc(sizeof(int64_t) * (nbix + 1));
+        // columns stitched on the device; host output downloads each array
+        // once from stitch buffers of the block cache (hipFree would
+        // synchronise the whole device), device output keeps its own
+        DevArena T;
+        ArenaGuard tg{T};
+        T.use_cache(stream);
+        int64_t *d_ir = nullptr;
+        double *d_val = nullptr;
         if (D.on_device) {
-            if (hipMalloc(&m->ir, sizeof(int64_t) * (nnz ? nnz : 1)) != hipSuccess) m->ir = nullptr;
-            if (hipMalloc(&m->val, sizeof(double) * (nnz ? nnz : 1)) != hipSuccess) m->val = nullptr;
+            if (hipMalloc(&d_ir, sizeof(int64_t) * (nnz ? nnz : 1)) != hipSuccess) d_ir = nullptr;
+            if (hipMalloc(&d_val, sizeof(double) * (nnz ? nnz : 1)) != hipSuccess) d_val = nullptr;
             (void)hipGetLastError();
         } else {
-            m->ir = (int64_t *)malloc(sizeof(int64_t) * (nnz ? nnz : 1));
-            m->val = (double *)malloc(sizeof(double) * (nnz ? nnz : 1));
+            d_ir = T.alloc_n<int64_t>((size_t)(nnz ? nnz : 1));
+            d_val = T.alloc_n<double>((size_t)(nnz ? nnz : 1));
         }
-        if (!m->jc || !m->ir || !m->val) {
+        m->ir = D.on_device ? d_ir : nullptr;
+        m->val = D.on_device ? d_val : nullptr;
+        m->device = D.on_device ? 1 : 0;
+        if (!m->jc || !d_ir || !d_val) {
             omc_csc_free(m);
-            return omc_seterr(OMC_ERR_NOMEM, "%s CSC of %lld entries", D.on_device ? "device" : "host",
-                              (long long)nnz);
+            return omc_seterr(OMC_ERR_NOMEM, "device CSC of %lld entries", (long long)nnz);
         }
         int64_t e = 0;
         int col = 0;
@@ -924,18 +909,24 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
                 col++;
             }
             if (g.nnz == 0) continue;
-            if (D.on_device) {
-                CK(hipMemcpyAsync(m->ir + e, g.d_ir, sizeof(int64_t) * g.nnz, hipMemcpyDeviceToDevice, stream));
-                CK(hipMemcpyAsync(m->val + e, variance ? g.d_var : g.d_val, sizeof(double) * g.nnz,
-                                  hipMemcpyDeviceToDevice, stream));
-            } else {
-                memcpy(m->ir + e, g.ir.data(), sizeof(int64_t) * g.nnz);
-                memcpy(m->val + e, (variance ? g.var : g.val).data(), sizeof(double) * g.nnz);
-            }
+            CK(hipMemcpyAsync(d_ir + e, g.d_ir, sizeof(int64_t) * g.nnz, hipMemcpyDeviceToDevice, stream));
+            CK(hipMemcpyAsync(d_val + e, variance ? g.d_var : g.d_val, sizeof(double) * g.nnz,
+                              hipMemcpyDeviceToDevice, stream));
             e += g.nnz;
         }
-        if (D.on_device) CK(hipStreamSynchronize(stream));
-        return OMC_OK;
+        CK(hipStreamSynchronize(stream));
+        if (D.on_device) return OMC_OK;
+        m->ir = (int64_t *)malloc(sizeof(int64_t) * (nnz ? nnz : 1));
+        m->val = (double *)malloc(sizeof(double) * (nnz ? nnz : 1));
+        int rc2 = OMC_OK;
+        if (!m->ir || !m->val) {
+            rc2 = omc_seterr(OMC_ERR_NOMEM, "host CSC of %lld entries", (long long)nnz);
+        } else if (nnz > 0 && (hipMemcpy(m->ir, d_ir, sizeof(int64_t) * nnz, hipMemcpyDeviceToHost) != hipSuccess ||
+                               hipMemcpy(m->val, d_val, sizeof(double) * nnz, hipMemcpyDeviceToHost) != hipSuccess)) {
+            rc2 = omc_seterr(OMC_ERR_DEVICE, "download of the CSC: %s", hipGetErrorString(hipGetLastError()));
+        }
+        if (rc2) omc_csc_free(m);
+        return rc2;
     };
     if (dij && (rc = fill_csc(dij, false))) return rc;
     if (var && (rc = fill_csc(var, true))) {
