@@ -275,7 +275,8 @@ def bench_c3(a):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 transport, f64 tallies" if prec == omc.FP32 else "f64",
+        "dtype": "f32" if prec == omc.FP32 else "f64",
+        "tally_dtype": "f64",
         "data": "synthetic CT, beamlets and 6 MV spectrum (seeded)",
         "config": {"workload": "c3: Dij of a 5-beam plan, 160x160x100 synthetic CT at 0.25x0.25x0.3 cm, "
                                f"{bix.n} beamlets of 0.5 cm, lognormal 1e3..5e4 histories each, 10 batches, "
@@ -417,7 +418,8 @@ def bench_dose(a):
     out = {"metric": "histories/s (dose run)", "value": st.histories / wall, "unit": "histories/s", "n_gpus": 1,
            "steps": max(2, a.steps), "warmup": a.warmup, "ms_per_step": 1e3 * wall / max(2, a.steps),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f32 transport, f64 tallies" if prec == omc.FP32 else "f64", "data": "synthetic (seeded)",
+           "dtype": "f32" if prec == omc.FP32 else "f64",
+        "tally_dtype": "f64", "data": "synthetic (seeded)",
            "config": {"workload": wl, "histories_per_step": nper, "form": a.form, "precision": a.precision},
            "mean_rel_unc_gt50": float(np.mean(unc[m])),
            "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -463,7 +465,7 @@ def main():
         print(json.dumps({"copy_bytes_read": a.calibrate_copy << 20, "copy_bytes_written": a.calibrate_copy << 20}))
         return
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+        print(f"bench.py: the engine runs on one GPU; --gpus {a.gpus} is not supported "
               "(beamlets and batches would shard across GPUs with no reduction; see the design notes)",
               file=sys.stderr)
         sys.exit(2)

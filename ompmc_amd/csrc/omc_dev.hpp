@@ -754,7 +754,13 @@ template <class R, class G, class T> struct Engine {
     // with probability mu(site) / mu_max; the interaction then follows the
     // reference exactly.  Same distribution of sites as the voxel-by-voxel
     // flight of photon() (not the same draws), without a dependent load per
-    // voxel crossed.
+    // voxel crossed.  The attenuation per unit density ratio of every medium,
+    // 1 / (gmfp cohfac), is evaluated once per flight (the energy does not
+    // change until the photon interacts) and kept in registers: a tentative
+    // site then costs one round of loads (medium and density of its voxel),
+    // and the tables are read again only for a real interaction.  The
+    // majorant is formed from the same products, so mu(site) <= mu_max holds
+    // exactly in floating point.
     __device__ OMC_INL int photon_delta(Part<R> &p, Part<R> &q) {
         if (count) cnt.nphoton++;
         int imed = medium(p);
@@ -765,13 +771,16 @@ template <class R, class G, class T> struct Engine {
         }
         R gle = M<R>::log(eig);
         R mu_max = (R)0;
-        for (int m = 0; m < t.nmed; m++) {
+        R mu_rho[MXMED];
+#pragma unroll
+        for (int m = 0; m < MXMED; m++) {
+            mu_rho[m] = (R)0;
             R rm = g.rhof_max[m];
-            if (rm <= (R)0) continue;
+            if (m >= t.nmed || rm <= (R)0) continue;
             const Med<R> &md = t.med[m];
             int lg = pwlf_i(gle, md.ge1, md.ge0) - 1;
-            R mu = rm / (pwlf(t.gmfp + (m * MXGE + lg), gle) * pwlf(t.cohe + (m * MXGE + lg), gle));
-            mu_max = M<R>::fmax(mu_max, mu);
+            mu_rho[m] = (R)1.0 / (pwlf(t.gmfp + (m * MXGE + lg), gle) * pwlf(t.cohe + (m * MXGE + lg), gle));
+            mu_max = M<R>::fmax(mu_max, mu_rho[m] * rm);
         }
         if (!(mu_max > (R)0)) {          // nothing to interact with: leaves
             p.ix = -1;
@@ -799,16 +808,20 @@ template <class R, class G, class T> struct Engine {
             p.iz = locate(g.zb, g.nz, p.z, g.inv_dz);
             if (count) cnt.ncross++;
             size_t v = p.ix + (size_t)g.nx * (p.iy + (size_t)g.ny * p.iz);
-            uint8_t mm = gld(g.med + (v));
+            const uint8_t mm = gld(g.med + (v));
+            const R rho = gld(g.rhof + (v));
             R r2 = rnd();
             if (mm == VACUUM) continue;
-            imed = (int)mm;
-            const Med<R> &md = t.med[imed];
-            int lgle = pwlf_i(gle, md.ge1, md.ge0) - 1;
-            R gmfp = pwlf(t.gmfp + (imed * MXGE + lgle), gle) / gld(g.rhof + (v));
-            R cohfac = pwlf(t.cohe + (imed * MXGE + lgle), gle);
-            R mu = (R)1.0 / (gmfp * cohfac);
-            if (r2 * mu_max <= mu) return interact(p, q, imed, eig, gle, lgle, cohfac);
+            R mur = mu_rho[0];
+#pragma unroll
+            for (int m = 1; m < MXMED; m++) mur = (int)mm == m ? mu_rho[m] : mur;   // no dynamic register index
+            if (r2 * mu_max <= mur * rho) {
+                imed = (int)mm;
+                const Med<R> &md = t.med[imed];
+                int lgle = pwlf_i(gle, md.ge1, md.ge0) - 1;
+                R cohfac = pwlf(t.cohe + (imed * MXGE + lgle), gle);
+                return interact(p, q, imed, eig, gle, lgle, cohfac);
+            }
         }
     }
 
