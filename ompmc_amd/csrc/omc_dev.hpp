@@ -772,15 +772,19 @@ template <class R, class G, class T> struct Engine {
         R gle = M<R>::log(eig);
         R mu_max = (R)0;
         R mu_rho[MXMED];
+        // Unconditional loads (a medium beyond nmed reads medium 0's tables and
+        // is then ignored): with no branch between them, every medium's loads
+        // are in flight at once instead of one medium after another.
 #pragma unroll
         for (int m = 0; m < MXMED; m++) {
-            mu_rho[m] = (R)0;
-            R rm = g.rhof_max[m];
-            if (m >= t.nmed || rm <= (R)0) continue;
-            const Med<R> &md = t.med[m];
-            int lg = pwlf_i(gle, md.ge1, md.ge0) - 1;
-            mu_rho[m] = (R)1.0 / (pwlf(t.gmfp + (m * MXGE + lg), gle) * pwlf(t.cohe + (m * MXGE + lg), gle));
-            mu_max = M<R>::fmax(mu_max, mu_rho[m] * rm);
+            const int mc = m < t.nmed ? m : 0;
+            const Med<R> &md = t.med[mc];
+            const int lg = pwlf_i(gle, md.ge1, md.ge0) - 1;
+            const R mr = (R)1.0 / (pwlf(t.gmfp + (mc * MXGE + lg), gle) * pwlf(t.cohe + (mc * MXGE + lg), gle));
+            const R rm = g.rhof_max[m];
+            const bool use = m < t.nmed && rm > (R)0;
+            mu_rho[m] = use ? mr : (R)0;
+            mu_max = use ? M<R>::fmax(mu_max, mr * rm) : mu_max;
         }
         if (!(mu_max > (R)0)) {          // nothing to interact with: leaves
             p.ix = -1;
@@ -817,12 +821,19 @@ template <class R, class G, class T> struct Engine {
             for (int m = 1; m < MXMED; m++) mur = (int)mm == m ? mu_rho[m] : mur;   // no dynamic register index
             if (r2 * mu_max <= mur * rho) {
                 imed = (int)mm;
-                const Med<R> &md = t.med[imed];
-                int lgle = pwlf_i(gle, md.ge1, md.ge0) - 1;
-                R cohfac = pwlf(t.cohe + (imed * MXGE + lgle), gle);
-                return interact(p, q, imed, eig, gle, lgle, cohfac);
+                break;
             }
         }
+        // The interaction follows the loop rather than sitting in it: lanes
+        // reach their real sites after different numbers of tentative
+        // collisions, and out here every lane of the wave whose photon
+        // interacts does so together, once per flight (inside the loop the
+        // wave ran the interaction code once per trial in which any lane
+        // accepted).  Each lane's operations and draws are the same.
+        const Med<R> &md = t.med[imed];
+        int lgle = pwlf_i(gle, md.ge1, md.ge0) - 1;
+        R cohfac = pwlf(t.cohe + (imed * MXGE + lgle), gle);
+        return interact(p, q, imed, eig, gle, lgle, cohfac);
     }
 
     // photon() with splitting (src/ompmc.c:1867-2139) on a stack array: the
