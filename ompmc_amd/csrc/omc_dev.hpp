@@ -1253,9 +1253,12 @@ template <class R, class G, class T> struct Engine {
 
     // CSDA path length inside one PWLF interval (src/ompmc.c:3997-4032)
     __device__ OMC_INL R drange(int imed, int iq, int lelke, R ekei, R ekef, R elkei, R elkef) {
+        return drange(gld((iq < 0 ? t.ededx : t.pdedx) + (MXEKE * imed + lelke)), ekei, ekef, elkei, elkef);
+    }
+    // the same with the interval's stopping-power entry already loaded
+    __device__ OMC_INL R drange(const PW<R> &q, R ekei, R ekef, R elkei, R elkef) {
         R fedep = (R)1.0 - ekef / ekei;
         R elktmp = (R)0.5 * (elkei + elkef + (R)0.25 * fedep * fedep * ((R)1.0 + fedep * ((R)1.0 + (R)0.875 * fedep)));
-        const PW<R> q = gld((iq < 0 ? t.ededx : t.pdedx) + (MXEKE * imed + lelke));
         R dedxmid = (R)1.0 / pwlf(q, elktmp);
         R aux = q.c1 * dedxmid;
         aux = aux * ((R)1.0 + (R)2.0 * aux) * fedep * fedep / ((R)6.0 * ((R)2.0 - fedep) * ((R)2.0 - fedep));
@@ -1264,14 +1267,23 @@ template <class R, class G, class T> struct Engine {
 
     // sub-threshold energy loss over a path tustep (src/ompmc.c:4034-4126)
     __device__ OMC_INL R eloss(int imed, int iq, R rhof, R tustep, R range, R eke, R elke, int lelke) {
+        const int qel = (1 + iq) / 2;
+        const R *rep = t.range_ep + ((size_t)qel * t.nmed + imed) * MXEKE;
+        return eloss(imed, iq, rhof, tustep, range, eke, elke, lelke,
+                     gld((iq < 0 ? t.ededx : t.pdedx) + (imed * MXEKE + lelke)), gld(rep + (lelke)));
+    }
+    // the same with the stopping-power entry and the range at the lower edge
+    // of the energy bin already loaded
+    __device__ OMC_INL R eloss(int imed, int iq, R rhof, R tustep, R range, R eke, R elke, int lelke,
+                               const PW<R> &q_lo, R rep_lo) {
         const Med<R> &m = t.med[imed];
         int qel = (1 + iq) / 2;
         const R *rep = t.range_ep + ((size_t)qel * t.nmed + imed) * MXEKE;
         const PW<R> *dx = iq < 0 ? t.ededx : t.pdedx;
         R de, fedep;
-        R tuss = range - gld(rep + (lelke)) / rhof;
+        R tuss = range - rep_lo / rhof;
         if (tuss >= tustep) {
-            const PW<R> q = gld(dx + (imed * MXEKE + lelke));
+            const PW<R> q = q_lo;
             R dedxmid = pwlf(q, elke);
             R aux = q.c1 / dedxmid;
             de = dedxmid * tustep * rhof;
@@ -1643,6 +1655,8 @@ template <class R, class G, class T> struct Engine {
             do {
                 bool call_howfar = false, do_single = false, called_msdist = false;
                 R tstep, tustep, total_tstep = (R)0, range = (R)0, tvstep = (R)0, de = (R)0, tperp = (R)0;
+                PW<R> q_dedx = {(R)0, (R)0};     // stopping power and range at this step's energy bin
+                R rep_lo = (R)0;                 // (loaded below for a step in a medium)
                 Part<R> fin;
                 if (imed == -1) {
                     tstep = (R)10.0E8;
@@ -1652,6 +1666,16 @@ template <class R, class G, class T> struct Engine {
                 } else {
                     const Med<R> &m = t.med[imed];
                     const R *rep = t.range_ep + ((size_t)qel * t.nmed + imed) * MXEKE;
+                    // this step's table entries at (medium, energy bin), loaded
+                    // together ahead of the branches below, whose outcome they
+                    // do not depend on (same values as loading them where used)
+                    const int ix = imed * MXEKE + lelke;
+                    q_dedx = gld((iq < 0 ? t.ededx : t.pdedx) + ix);
+                    const PW<R> q_tmxs = gld(t.tmxs + ix);
+                    const PW<R> q_eta = gld((iq < 0 ? t.etae : t.etap) + ix);
+                    const PW<R> q_msc = gld(t.blcce + ix);
+                    const R e_lo = gld(t.e_array + ix);
+                    rep_lo = gld(rep + (lelke));
                     rhof = rhof_of(p);
                     if (sig0 <= (R)0.0) {
                         tstep = (R)10.0E8;
@@ -1666,15 +1690,15 @@ template <class R, class G, class T> struct Engine {
                                 R elkef = M<R>::log(ekef);
                                 int lelkef = pwlf_i(elkef, m.eke1, m.eke0) - 1;
                                 if (lelkef == lelke) {
-                                    tstep = drange(imed, iq, lelke, eke, ekef, elke, elkef);
+                                    tstep = drange(q_dedx, eke, ekef, elke, elkef);
                                 } else {
-                                    R ekei = gld(t.e_array + (imed * MXEKE + lelke));
+                                    R ekei = e_lo;
                                     R elkei = ((R)(lelke + 1) - m.eke0) / m.eke1;
-                                    R tuss = drange(imed, iq, lelke, eke, ekei, elke, elkei);
+                                    R tuss = drange(q_dedx, eke, ekei, elke, elkei);
                                     ekei = gld(t.e_array + (imed * MXEKE + lelkef + 1));
                                     elkei = ((R)(lelkef + 2) - m.eke0) / m.eke1;
                                     tstep = drange(imed, iq, lelkef, ekei, ekef, elkei, elkef);
-                                    tstep += tuss + gld(rep + (lelke)) - gld(rep + (lelkef + 1));
+                                    tstep += tuss + rep_lo - gld(rep + (lelkef + 1));
                                 }
                             }
                             total_tstep = tstep;
@@ -1682,30 +1706,29 @@ template <class R, class G, class T> struct Engine {
                         }
                         tstep = total_tstep / rhof;
                     }
-                    int ix = imed * MXEKE + lelke;
-                    R dedx0 = iq < 0 ? pwlf(t.ededx + (ix), elke) : pwlf(t.pdedx + (ix), elke);
+                    R dedx0 = pwlf(q_dedx, elke);
                     R dedx = rhof * dedx0;
-                    R tmxs = pwlf(t.tmxs + (ix), elke);
+                    R tmxs = pwlf(q_tmxs, elke);
                     tmxs /= rhof;
-                    R ekei = gld(t.e_array + (imed * MXEKE + lelke));
+                    R ekei = e_lo;
                     R elkei = ((R)(lelke + 1) - m.eke0) / m.eke1;
-                    range = drange(imed, iq, lelke, eke, ekei, elke, elkei);
-                    range += gld(rep + (lelke));
+                    range = drange(q_dedx, eke, ekei, elke, elkei);
+                    range += rep_lo;
                     range /= rhof;
                     tustep = M<R>::fmin(M<R>::fmin(tstep, tmxs), range);
                     tperp = hownear(p);
                     R blccl = rhof * m.blcc, xccl = rhof * m.xcc;
                     R p2 = eke * (eke + (R)2.0 * RM);
                     R beta2 = p2 / (p2 + RM * RM);
-                    R etap = iq < 0 ? pwlf(t.etae + (ix), elke) : pwlf(t.etap + (ix), elke);
-                    R ms_corr = pwlf(t.blcce + (ix), elke);
+                    R etap = pwlf(q_eta, elke);
+                    R ms_corr = pwlf(q_msc, elke);
                     blccl = blccl / etap / ((R)1.0 + (R)0.25 * etap * xccl / blccl / p2) * ms_corr;
                     R ssmfp = beta2 / blccl;
                     R skindepth = (R)3 * ssmfp;
                     tustep = M<R>::fmin(tustep, M<R>::fmax(tperp, skindepth));
                     if ((tustep <= tperp) && (tustep > skindepth)) {
                         called_msdist = true;
-                        de = eloss(imed, iq, rhof, tustep, range, eke, elke, lelke);
+                        de = eloss(imed, iq, rhof, tustep, range, eke, elke, lelke, q_dedx, rep_lo);
                         tvstep = tustep;
                         ustep = msdist(p, imed, iq, rhof, de, tustep, eke, fin);
                     } else {
@@ -1759,10 +1782,10 @@ template <class R, class G, class T> struct Engine {
                 if (call_howfar) {
                     tvstep = vstep;
                     if (tvstep != tustep) do_single = false;
-                    de = eloss(imed, iq, rhof, tvstep, range, eke, elke, lelke);
+                    de = eloss(imed, iq, rhof, tvstep, range, eke, elke, lelke, q_dedx, rep_lo);
                 } else {
                     tvstep = tustep;
-                    if (!called_msdist) de = eloss(imed, iq, rhof, tvstep, range, eke, elke, lelke);
+                    if (!called_msdist) de = eloss(imed, iq, rhof, tvstep, range, eke, elke, lelke, q_dedx, rep_lo);
                 }
                 R edep = de;
                 R ekef = eke - de;
