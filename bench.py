@@ -102,7 +102,7 @@ def c3_config():
 _PHANTOM_POOL = {}
 
 
-def serve_mix(ctx, rate_hist_per_s, nreq=12, load=0.8, seed=20261206, precision=32, max_hist=1e8):
+def serve_mix(ctx, rate_hist_per_s, nreq=12, load=0.8, seed=20261206, precision=32, max_hist=1e8, max_inflight=4):
     """A Poisson-arrival request mix with the ranges of the serving workload:
     phantoms 64^3 .. 256x256x160 at 0.2-0.5 cm (a pool of at most 8, reused),
     media water/lung/bone/air, 1e5..max_hist histories per request
@@ -143,7 +143,7 @@ def serve_mix(ctx, rate_hist_per_s, nreq=12, load=0.8, seed=20261206, precision=
     mean_h = np.mean([r[2] for r in reqs])
     rate = load * rate_hist_per_s / mean_h               # requests per second
     gaps = rng.exponential(1.0 / rate, nreq)
-    srv = omc.Server(ctx, max_inflight=4)
+    srv = omc.Server(ctx, max_inflight=max_inflight)
     ids = []
     t0 = time.perf_counter()
     t_next = t0
