@@ -306,12 +306,15 @@ def bench_c3(a):
     # the same step with the matrix returned in host memory, as the mex
     # interface receives it (device-to-host copy and column stitching)
     ho = dict(opts, csc_on_device=False, download=True)
-    t0 = time.perf_counter()
-    _, _, _, sth = ctx.run_dij(gph, spec, bix, nbatch=nbatch, rel_threshold=rel, **ho)
-    wh = time.perf_counter() - t0
-    out["host_output"] = {"histories_per_s": sth.histories / wh, "nnz": int(sth.nnz),
-                          "what": "one step returning the CSC in host memory (device-to-host copy included)"}
-    log(f"host output: {sth.histories / wh:.4g} histories/s")
+    rates = []
+    for _ in range(2):          # the first call also pins the pooled host blocks it returns
+        t0 = time.perf_counter()
+        _, _, _, sth = ctx.run_dij(gph, spec, bix, nbatch=nbatch, rel_threshold=rel, **ho)
+        rates.append(sth.histories / (time.perf_counter() - t0))
+    out["host_output"] = {"histories_per_s": rates[1], "first_call_histories_per_s": rates[0], "nnz": int(sth.nnz),
+                          "what": "steps returning the CSC in host memory (device-to-host copy and the copy into "
+                                  "numpy arrays included); the first call also pins the pooled host blocks"}
+    log(f"host output: {rates[1]:.4g} histories/s (first call {rates[0]:.4g})")
     if not a.no_plain:
         sub = subset(bix, 20)
         _, _, _, sp = ctx.run_dij(gph, spec, sub, nbatch=nbatch, rel_threshold=rel, precision=prec,

@@ -197,7 +197,8 @@ typedef struct omc_beamlets_desc {
 /* Compressed sparse column matrix (MATLAB layout): column j holds rows
  * ir[jc[j]..jc[j+1]-1] with values val[]; allocated by the library, released
  * with omc_csc_free.  device = 1 (runs with csc_on_device): ir and val are
- * device addresses on the run's device, jc is on the host. */
+ * device addresses on the run's device; 0 or 2: host memory (2: pinned
+ * blocks the library reuses once released).  jc is always on the host. */
 typedef struct omc_csc {
     int64_t nrows, ncols, nnz;
     int64_t *jc, *ir;

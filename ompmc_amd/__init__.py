@@ -98,7 +98,7 @@ class CSC(C.Structure):
         device memory); the library arrays stay owned by the struct."""
         jc = np.ctypeslib.as_array(self.jc, shape=(self.ncols + 1,)).copy()
         n = int(self.nnz)
-        if self.device:
+        if self.device == 1:
             ir, val = np.zeros(n, np.int64), np.zeros(n)
             if n:
                 _check(lib().omc_csc_download(C.byref(self), ir.ctypes.data_as(_i64), _dp(val)))
@@ -385,7 +385,7 @@ class Context:
                                     C.byref(dij), C.byref(var) if want_var else None,
                                     _dp(pd) if want_plan else None, _dp(pv) if want_plan else None, C.byref(st)))
         def take(m):
-            if m.device and not download:
+            if m.device == 1 and not download:
                 r = (np.ctypeslib.as_array(m.jc, shape=(m.ncols + 1,)).copy(), None, None)
             else:
                 r = m.to_numpy()

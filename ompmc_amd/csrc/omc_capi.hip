@@ -130,6 +130,50 @@ void *pinned_small_alloc() {
     return p;
 }
 
+namespace {
+std::mutex pinned_big_mu;
+std::multimap<size_t, void *> pinned_big_free;       // released blocks by size
+std::map<void *, size_t> pinned_big_live;            // blocks handed out, with their sizes
+size_t pinned_big_held = 0;
+}  // namespace
+
+void *pinned_block_alloc(size_t n) {
+    n = (std::max<size_t>(n, 1) + CACHE_GRAIN - 1) / CACHE_GRAIN * CACHE_GRAIN;
+    std::lock_guard<std::mutex> lk(pinned_big_mu);
+    auto it = pinned_big_free.lower_bound(n);
+    void *p = nullptr;
+    size_t got = n;
+    if (it != pinned_big_free.end() && it->first <= n + n / 4) {
+        p = it->second;
+        got = it->first;
+        pinned_big_held -= got;
+        pinned_big_free.erase(it);
+    } else if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    pinned_big_live[p] = got;
+    return p;
+}
+
+void pinned_block_free(void *p) {
+    if (!p) return;
+    size_t n = 0;
+    {
+        std::lock_guard<std::mutex> lk(pinned_big_mu);
+        auto it = pinned_big_live.find(p);
+        if (it == pinned_big_live.end()) return;       // not a block of this pool
+        n = it->second;
+        pinned_big_live.erase(it);
+        if (pinned_big_held + n <= PINNED_CACHE_MAX) {
+            pinned_big_free.emplace(n, p);
+            pinned_big_held += n;
+            return;
+        }
+    }
+    (void)hipHostFree(p);
+}
+
 void pinned_small_free(void *p) {
     if (!p) return;
     std::lock_guard<std::mutex> lk(pinned_mu);

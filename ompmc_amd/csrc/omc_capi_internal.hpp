@@ -54,6 +54,12 @@ void devcache_trim();
 void *pinned_small_alloc();
 void pinned_small_free(void *p);
 constexpr size_t PINNED_SMALL_BYTES = 4096;
+// Large pinned host blocks for results handed to the caller (downloaded at
+// DMA speed into pages that are already mapped), kept for reuse when
+// released: at most PINNED_CACHE_MAX bytes are held.
+void *pinned_block_alloc(size_t n);
+void pinned_block_free(void *p);
+constexpr size_t PINNED_CACHE_MAX = (size_t)4 << 30;
 
 struct PhantomDev {
     omc_phantom host;            // owned copy (dose conversion, source set-up)

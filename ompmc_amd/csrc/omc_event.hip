@@ -916,8 +916,18 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
         }
         CK(hipStreamSynchronize(stream));
         if (D.on_device) return OMC_OK;
-        m->ir = (int64_t *)malloc(sizeof(int64_t) * (nnz ? nnz : 1));
-        m->val = (double *)malloc(sizeof(double) * (nnz ? nnz : 1));
+        // host arrays: pinned blocks from the reuse pool (a DMA download into
+        // mapped pages), else ordinary memory
+        m->ir = (int64_t *)pinned_block_alloc(sizeof(int64_t) * (nnz ? nnz : 1));
+        m->val = (double *)pinned_block_alloc(sizeof(double) * (nnz ? nnz : 1));
+        m->device = 2;
+        if (!m->ir || !m->val) {
+            pinned_block_free(m->ir);
+            pinned_block_free(m->val);
+            m->ir = (int64_t *)malloc(sizeof(int64_t) * (nnz ? nnz : 1));
+            m->val = (double *)malloc(sizeof(double) * (nnz ? nnz : 1));
+            m->device = 0;
+        }
         int rc2 = OMC_OK;
         if (!m->ir || !m->val) {
             rc2 = omc_seterr(OMC_ERR_NOMEM, "host CSC of %lld entries", (long long)nnz);
@@ -1000,9 +1010,12 @@ extern "C" int omc_run_dij(omc_ctx *c, int phantom_id, const omc_spectrum_desc *
 extern "C" void omc_csc_free(omc_csc *m) {
     if (!m) return;
     free(m->jc);
-    if (m->device) {
+    if (m->device == 1) {
         if (m->ir) (void)hipFree(m->ir);
         if (m->val) (void)hipFree(m->val);
+    } else if (m->device == 2) {
+        pinned_block_free(m->ir);
+        pinned_block_free(m->val);
     } else {
         free(m->ir);
         free(m->val);
@@ -1014,7 +1027,7 @@ extern "C" void omc_csc_free(omc_csc *m) {
 }
 
 extern "C" int omc_csc_download(const omc_csc *m, int64_t *ir, double *val) {
-    if (!m || !m->device || (m->nnz > 0 && (!ir || !val)))
+    if (!m || m->device != 1 || (m->nnz > 0 && (!ir || !val)))
         return omc_seterr(OMC_ERR_ARG, "omc_csc_download: a device matrix and host arrays of nnz entries");
     if (m->nnz == 0) return OMC_OK;
     CK(hipMemcpy(ir, m->ir, sizeof(int64_t) * m->nnz, hipMemcpyDeviceToHost));
