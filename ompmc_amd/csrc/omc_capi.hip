@@ -156,22 +156,23 @@ void *pinned_block_alloc(size_t n) {
     return p;
 }
 
-void pinned_block_free(void *p) {
-    if (!p) return;
+bool pinned_block_free(void *p) {
+    if (!p) return false;
     size_t n = 0;
     {
         std::lock_guard<std::mutex> lk(pinned_big_mu);
         auto it = pinned_big_live.find(p);
-        if (it == pinned_big_live.end()) return;       // not a block of this pool
+        if (it == pinned_big_live.end()) return false;  // not a block of this pool
         n = it->second;
         pinned_big_live.erase(it);
         if (pinned_big_held + n <= PINNED_CACHE_MAX) {
             pinned_big_free.emplace(n, p);
             pinned_big_held += n;
-            return;
+            return true;
         }
     }
     (void)hipHostFree(p);
+    return true;
 }
 
 void pinned_small_free(void *p) {

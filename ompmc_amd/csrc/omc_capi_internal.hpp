@@ -58,7 +58,7 @@ constexpr size_t PINNED_SMALL_BYTES = 4096;
 // DMA speed into pages that are already mapped), kept for reuse when
 // released: at most PINNED_CACHE_MAX bytes are held.
 void *pinned_block_alloc(size_t n);
-void pinned_block_free(void *p);
+bool pinned_block_free(void *p);     // false: p is not a block of the pool (left untouched)
 constexpr size_t PINNED_CACHE_MAX = (size_t)4 << 30;
 
 struct PhantomDev {

@@ -1121,8 +1121,11 @@ void run_request(omc_server *s, omc_server::Req *q) {
     memset(&stats, 0, sizeof stats);
     if (!rc && q->r.kind == OMC_REQ_DOSE) {
         res.nreg = pd->nreg;
-        res.sum = (double *)calloc(pd->nreg, sizeof(double));
-        res.sum2 = (double *)calloc(pd->nreg, sizeof(double));
+        // pooled pinned blocks (every entry is written by the run), else ordinary memory
+        res.sum = (double *)pinned_block_alloc(sizeof(double) * pd->nreg);
+        res.sum2 = (double *)pinned_block_alloc(sizeof(double) * pd->nreg);
+        if (!res.sum) res.sum = (double *)calloc(pd->nreg, sizeof(double));
+        if (!res.sum2) res.sum2 = (double *)calloc(pd->nreg, sizeof(double));
         if (!res.sum || !res.sum2) rc = omc_seterr(OMC_ERR_NOMEM, "result arrays");
         if (!rc)
             rc = omc_dosxyz_run(c, st, pd, &q->r.dose_src, q->r.nperbatch, q->r.nbatch, &o, res.sum, res.sum2, &stats);
@@ -1296,8 +1299,8 @@ extern "C" int omc_server_wait(omc_server *s, int64_t id, omc_result *res) {
 
 extern "C" void omc_result_free(omc_result *res) {
     if (!res) return;
-    free(res->sum);
-    free(res->sum2);
+    if (!pinned_block_free(res->sum)) free(res->sum);
+    if (!pinned_block_free(res->sum2)) free(res->sum2);
     res->sum = res->sum2 = nullptr;
     omc_csc_free(&res->dij);
 }
