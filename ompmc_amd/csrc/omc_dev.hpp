@@ -1598,6 +1598,7 @@ template <class R, class G, class T> struct Engine {
         if (count) cnt.nelectron++;
         int imed = medium(p);
         R rhof = rhof_of(p);
+        R rhof_voxel = rhof;             // density of the current voxel, loaded with its medium at a move
         R eie = p.e;
         int iq = p.iq, qel = (1 + iq) / 2, medold = imed;
         bool resume = es.phase != 0;
@@ -1676,7 +1677,7 @@ template <class R, class G, class T> struct Engine {
                     const PW<R> q_msc = gld(t.blcce + ix);
                     const R e_lo = gld(t.e_array + ix);
                     rep_lo = gld(rep + (lelke));
-                    rhof = rhof_of(p);
+                    rhof = rhof_voxel;           // the reference reloads the density here, not at the move
                     if (sig0 <= (R)0.0) {
                         tstep = (R)10.0E8;
                         sig0 = (R)1.0E-15;
@@ -1774,6 +1775,7 @@ template <class R, class G, class T> struct Engine {
                     if (mv != 0) {
                         apply_move(p, mv, leaves);
                         imed = medium(p);
+                        rhof_voxel = rhof_of(p);    // taken into rhof where the reference reloads it
                     }
                     if (eie <= ecut_of(imed)) return electron_stop(p, q, iq, eie, false);
                     break;
@@ -1829,6 +1831,7 @@ template <class R, class G, class T> struct Engine {
                 if (mv != 0) {
                     apply_move(p, mv, leaves);
                     imed = medium(p);
+                    rhof_voxel = rhof_of(p);    // taken into rhof where the reference reloads it
                 }
                 if (eie <= ecut_of(imed)) return electron_stop(p, q, iq, eie, false);
                 if (imed != medold) break;
