@@ -112,7 +112,8 @@ typedef struct omc_run_opts {
     int form;               /* OMC_FORM_HISTORY | OMC_FORM_EVENT */
     int rng;                /* OMC_RNG_PHILOX | OMC_RNG_REPLAY */
     uint64_t seed;          /* Philox key */
-    int nsplit;             /* photon splitting factor (vrt.nsplit, 1..64); > 1 needs the history form */
+    int nsplit;             /* photon splitting factor (vrt.nsplit, 1..64); both forms.  In the event
+                               form split photons fly voxel by voxel (no delta tracking) */
     const uint32_t *replay;       /* OMC_RNG_REPLAY: draws (24-bit integers) */
     const int64_t *replay_offsets;/* first draw of each history, nhist+1 entries */
     int64_t replay_len;

@@ -542,8 +542,6 @@ int omc_check_opts(const omc_run_opts *o) {
     if (!o) return omc_seterr(OMC_ERR_ARG, "null run options");
     if (o->precision != OMC_FP32 && o->precision != OMC_FP64) return omc_seterr(OMC_ERR_ARG, "precision %d", o->precision);
     if (o->nsplit < 1 || o->nsplit > 64) return omc_seterr(OMC_ERR_ARG, "nsplit %d (1..64)", o->nsplit);
-    if (o->nsplit > 1 && o->form != OMC_FORM_HISTORY)
-        return omc_seterr(OMC_ERR_ARG, "photon splitting (nsplit %d) runs in the history form", o->nsplit);
     if (o->form != OMC_FORM_HISTORY && o->form != OMC_FORM_EVENT) return omc_seterr(OMC_ERR_ARG, "form %d", o->form);
     if (o->rng == OMC_RNG_REPLAY && (!o->replay || !o->replay_offsets))
         return omc_seterr(OMC_ERR_ARG, "replay random numbers requested without a stream");

@@ -708,13 +708,20 @@ template <class R, class G, class T> struct Engine {
     // Rayleigh with probability 1 - cohfac, else pair / Compton / photo by the
     // branching ratios of the medium at the site
     __device__ __forceinline__ int interact(Part<R> &p, Part<R> &q, int imed, R eig, R gle, int lgle, R cohfac) {
-        const R RM = (R)0.5109989461;
         R r = rnd();
         if (r <= (R)1.0 - cohfac) {
             rayleigh(p, imed, eig, gle, lgle);
             return 0;
         }
-        r = rnd();
+        return interact_pcp(p, q, imed, eig, gle, lgle);
+    }
+
+    // pair production, Compton or photo-electric absorption by the branching
+    // ratios of the medium at the site (src/ompmc.c:2060-2086); returns 1
+    // when q holds a second particle
+    __device__ __forceinline__ int interact_pcp(Part<R> &p, Part<R> &q, int imed, R eig, R gle, int lgle) {
+        const R RM = (R)0.5109989461;
+        R r = rnd();
         R gbr1 = pwlf(t.gbr1 + (imed * MXGE + lgle), gle);
         if (r <= gbr1 && eig > (R)2.0 * RM) {
             pair(imed, p, q);
@@ -836,6 +843,48 @@ template <class R, class G, class T> struct Engine {
         return interact(p, q, imed, eig, gle, lgle, cohfac);
     }
 
+    // Flight of one sub-photon of a split (src/ompmc.c:1960-2032): from its
+    // start, `dpmfp` mean free paths along its direction, voxel by voxel.
+    // Returns false when it leaves the geometry (the reference discards it
+    // and ends the split), true at its interaction site, with imed the
+    // medium there.  lgle, cohfac and gmfp are the caller's: like the
+    // reference's function-scope variables they keep the values of the last
+    // voxel with a medium, which the interaction then uses.
+    __device__ __forceinline__ bool split_fly(Part<R> &p, R dpmfp, R gle, int &imed, int &lgle, R &cohfac, R &gmfp) {
+        imed = medium(p);
+        for (;;) {
+            R tstep;
+            if (imed != -1) {
+                const Med<R> &m = t.med[imed];
+                lgle = pwlf_i(gle, m.ge1, m.ge0) - 1;
+                R gmfpr0 = pwlf(t.gmfp + (imed * MXGE + lgle), gle);
+                gmfp = gmfpr0 / rhof_of(p);
+                cohfac = pwlf(t.cohe + (imed * MXGE + lgle), gle);
+                gmfp *= cohfac;
+                tstep = gmfp * dpmfp;
+            } else {
+                tstep = (R)1.0E8;
+            }
+            R ustep = tstep;
+            if (p.ix < 0) {           // howfar: outside the geometry -> discard
+                if (count) cnt.nhowfar++;
+                p.x += ustep * p.u; p.y += ustep * p.v; p.z += ustep * p.w;
+                return false;
+            }
+            bool leaves;
+            int mv = howfar(p, ustep, leaves);
+            p.x += ustep * p.u;
+            p.y += ustep * p.v;
+            p.z += ustep * p.w;
+            if (imed != -1) dpmfp = M<R>::fmax((R)0.0, dpmfp - ustep / gmfp);
+            if (mv != 0) {
+                apply_move(p, mv, leaves);
+                imed = medium(p);
+            }
+            if (imed != -1 && dpmfp <= (R)1.0E-05) return true;
+        }
+    }
+
     // photon() with splitting (src/ompmc.c:1867-2139) on a stack array: the
     // photon on top of st[0..np] is split into nsplit sub-photons with
     // stratified numbers of mean free paths along its ray; each interacts at
@@ -883,41 +932,7 @@ template <class R, class G, class T> struct Engine {
                 p.u = save.u; p.v = save.v; p.w = save.w;
                 p.e = save.e; p.wt = wtsave;
                 p.ix = save.ix; p.iy = save.iy; p.iz = save.iz; p.iq = 0;
-                imed = medium(p);
-                bool discard = false;
-                for (;;) {
-                    R tstep;
-                    if (imed != -1) {
-                        const Med<R> &m = t.med[imed];
-                        lgle = pwlf_i(gle, m.ge1, m.ge0) - 1;
-                        R gmfpr0 = pwlf(t.gmfp + (imed * MXGE + lgle), gle);
-                        gmfp = gmfpr0 / rhof_of(p);
-                        cohfac = pwlf(t.cohe + (imed * MXGE + lgle), gle);
-                        gmfp *= cohfac;
-                        tstep = gmfp * dpmfp;
-                    } else {
-                        tstep = (R)1.0E8;
-                    }
-                    R ustep = tstep;
-                    if (p.ix < 0) {       // howfar: outside the geometry -> discard
-                        if (count) cnt.nhowfar++;
-                        p.x += ustep * p.u; p.y += ustep * p.v; p.z += ustep * p.w;
-                        discard = true;
-                        break;
-                    }
-                    bool leaves;
-                    int mv = howfar(p, ustep, leaves);
-                    p.x += ustep * p.u;
-                    p.y += ustep * p.v;
-                    p.z += ustep * p.w;
-                    if (imed != -1) dpmfp = M<R>::fmax((R)0.0, dpmfp - ustep / gmfp);
-                    if (mv != 0) {
-                        apply_move(p, mv, leaves);
-                        imed = medium(p);
-                    }
-                    if (imed != -1 && dpmfp <= (R)1.0E-05) break;
-                }
-                if (discard) {
+                if (!split_fly(p, dpmfp, gle, imed, lgle, cohfac, gmfp)) {
                     np -= 1;
                     if (np < 0) return np;
                     break;                          // end of the splitting loop

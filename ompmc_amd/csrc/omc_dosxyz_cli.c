@@ -119,11 +119,7 @@ int main(int argc, char **argv) {
     ro.seed = have_seed ? seed : ((uint64_t)(unsigned)ixx << 32) | (unsigned)jxx;
     ro.precision = fp64 ? OMC_FP64 : OMC_FP32;
     ro.form = history ? OMC_FORM_HISTORY : OMC_FORM_EVENT;
-    if (ro.nsplit > 1 && ro.form != OMC_FORM_HISTORY) {
-        /* photon splitting runs in the history form */
-        ro.form = OMC_FORM_HISTORY;
-        printf("Photon splitting enabled, nsplit = %d (history form)\n", ro.nsplit);
-    }
+    if (ro.nsplit > 1) printf("Photon splitting enabled, nsplit = %d\n", ro.nsplit);
     if (!need(&inp, "ncase", buf)) goto fail;
     long nhist = atol(buf);
     if (!need(&inp, "nbatch", buf)) goto fail;

@@ -131,7 +131,11 @@ template <class R>
 int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int dest, bool count, bool timing,
             JobStats &js) {
     constexpr int K = 4;
-    CK(launch_job_start(S.ctl, nhist, S.target, S.cap, dest, st));
+    // With photon splitting a photon puts up to 2 x nsplit charged particles
+    // into the queues in one launch: inject nsplit times fewer primaries, so
+    // that the particles in flight stay near the target and within the pools.
+    const long long inj_target = L.nsplit > 1 ? std::max<long long>(64, S.target / L.nsplit) : S.target;
+    CK(launch_job_start(S.ctl, nhist, inj_target, S.cap, dest, st));
     static_assert(2 * sizeof(EvCtl) <= PINNED_SMALL_BYTES, "control-block snapshots exceed a pinned buffer");
     PinnedSmall snapbuf;
     if (!snapbuf.p) return omc_seterr(OMC_ERR_NOMEM, "pinned host buffer");
@@ -163,7 +167,7 @@ int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int 
         }
         js.n_source += K; js.n_photon += K; js.n_charged += K; js.n_other += K;
     };
-    const long rmax = max_rounds(nhist, S.target);
+    const long rmax = max_rounds(nhist, inj_target);
     long r = 0;
     int cur = 0;
     CK(enqueue(r, 0));
@@ -226,6 +230,7 @@ template <class R> void fill_launch(omc_ctx *c, const PhantomDev *pd, EvState<R>
     const long steps = o->max_particle_steps > 0 ? (long)o->max_particle_steps : (1L << 20);
     L.max_age = L.turns > 0 ? (int)std::min<long>(steps / ((long)L.turns * L.budget) + 1, 1L << 22) : 0;
     L.delta = o->voxel_walk ? 0 : 1;
+    L.nsplit = o->nsplit;
     int cus = device_cus(c->device);
     L.grid_event = o->grid > 0 ? o->grid : cus * 8;
     L.grid_source = cus * 4;

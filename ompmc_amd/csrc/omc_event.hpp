@@ -92,6 +92,7 @@ template <class R> struct EvArgs {
     int turns;                        // electrons: refills a particle may hold its lane before it is requeued (0: no limit)
     int max_age;                      // electrons: requeues after which a particle counts as stuck (dropped, err bit 1)
     int delta;                        // photons: 1 delta tracking, 0 voxel-by-voxel flight
+    int nsplit;                       // photon splitting factor (vrt.nsplit); > 1 runs k_event_split for photons
 };
 
 template <class R> __device__ __forceinline__ void load(const Pool<R> &P, long i, Part<R> &p) {
@@ -124,11 +125,13 @@ __device__ __forceinline__ long long wave_reserve(unsigned long long *counter, b
     return want ? (long long)(base + __popcll(below)) : -1;
 }
 
-// Appends a particle to the photon or the charged output pool.
+// Appends a particle to the photon or the charged output pool.  A photon of
+// zero weight (one that lost the roulette of photon splitting) would deposit
+// nothing and is not queued.
 template <class R>
 __device__ __forceinline__ void emit(const EvArgs<R> &a, bool want, const Part<R> &p, R demfp, R sig0, int phase,
                                      uint64_t stream, uint32_t ctr, int slot) {
-    bool ph = want && p.iq == 0, ch = want && p.iq != 0;
+    bool ph = want && p.iq == 0 && p.wt != (R)0, ch = want && p.iq != 0;
     long long ip = wave_reserve(a.n_out + 0, ph);
     long long ie = wave_reserve(a.n_out + 1, ch);
     if (ph) {
@@ -225,6 +228,7 @@ k_event(EvArgs<R> a) {
         uint32_t qpos = 0;
         if (have) {
             Engine<R, PhiloxAt, TallyAtomic> e{*a.t, *a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
+            e.nsplit = a.nsplit;          // roulette of the photons that charged particles produce
             if (PHOTON) {
                 d = a.delta ? e.photon_delta(p, q) : e.photon(p, q);
             } else {
@@ -263,6 +267,155 @@ k_event(EvArgs<R> a) {
         emit(a, have && done && d >= 0, p, demfp, sig0, phase | (age << 8), stream, qpos, slot);
         emit(a, have && d == 1, q, (R)0, (R)0, 0, child_stream(stream, qpos), 0u, slot);
         if (done) have = false;
+    }
+    flush_counters<COUNT>(a.counters, tot);
+}
+
+// Persistent transport of photons with splitting (src/ompmc.c:1867-2139,
+// nsplit > 1).  A lane holds one photon and splits it: nsplit sub-photons of
+// weight wt/nsplit fly stratified numbers of mean free paths along its ray,
+// each from the interaction site of the one before; a sub-photon that leaves
+// the geometry ends the split.  Charged products keep the sub-photon's
+// weight and are queued.  Of the scattered photons only those of one
+// randomly chosen sub-photon survive, with the full weight; there is at most
+// one, and it stays in the lane and is split in turn.  The reference keeps
+// the same particles on its stack, in another order.
+//
+// The loop over sub-photons has the same trip count on every lane, so a
+// wave queues the products of each sub-photon together (one atomic per
+// wave).  Each product gets a random stream of its own, keyed by the
+// photon's stream and its draw position after the interaction; the second
+// product of a pair by the complement of the stream.
+template <class R, bool COUNT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1)))
+k_event_split(EvArgs<R> a) {
+    const long long n = (long long)*a.n_in;
+    const int ns = a.nsplit;
+    Counters tot = {0, 0, 0, 0, 0, 0, 0};
+    bool have = false;
+    Part<R> p;
+    uint64_t stream = 0;
+    int slot = 0;
+    PhiloxAt rng;
+    TallyAtomic tally;
+    tally.init(a.endep);
+    int iters = 0;
+    for (;;) {
+        long long idx = -1;
+        if (__any(!have)) {
+            const unsigned long long seen = __shfl(*(volatile const unsigned long long *)a.claim, 0);
+            if ((long long)seen < n) idx = wave_reserve(a.claim, !have);
+        }
+        if (!have && idx >= 0 && idx < n) {
+            iters = 0;
+            load(a.in, idx, p);
+            stream = a.in.stream[idx];
+            slot = a.in.slot[idx];
+            rng.at(a.seed, stream, a.in.ctr[idx]);
+            tally.init(a.endep + (size_t)slot * a.slot_stride);
+            have = true;
+        }
+        if (!__any(have)) break;
+        Engine<R, PhiloxAt, TallyAtomic> e{*a.t, *a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
+        e.nsplit = ns;
+        // set-up of the split (src/ompmc.c:1900-1940)
+        bool live = have, surv = false;
+        R eig = (R)0, gle = (R)0, eta = (R)0, d_eta = (R)0, dpmfp_old = (R)0, wtsave = (R)0;
+        int i_survive = -1, imed = -1, lgle = 0;
+        R cohfac = (R)0, gmfp = (R)0;
+        Part<R> save = p;
+        if (live) {
+            if (COUNT) e.cnt.nphoton++;
+            imed = e.medium(p);
+            eig = p.e;
+            if (eig <= e.pcut_of(imed) || p.wt == (R)0) {
+                e.ausgab(p, eig);
+                live = false;
+            } else {
+                gle = M<R>::log(eig);
+                R rnno = e.rnd();
+                rnno /= (R)ns;
+                d_eta = (R)1.0 / (R)ns;
+                eta = (R)1.0 - rnno + d_eta;
+                wtsave = p.wt / (R)ns;
+                rnno = e.rnd();
+                i_survive = (int)(rnno * (R)ns);
+            }
+        }
+        // the sub-photons (src/ompmc.c:1942-2111)
+        for (int isplit = 0; isplit < ns; isplit++) {
+            if (!__any(live)) break;
+            Part<R> c0, c1;
+            bool w0 = false, w1 = false;
+            uint32_t cpos = 0;
+            if (live) {
+                eta -= d_eta;
+                if (eta <= (R)0.0) {
+                    live = false;
+                } else {
+                    R dpmfp = -M<R>::log(eta) - dpmfp_old;
+                    dpmfp_old += dpmfp;
+                    Part<R> sp = save;
+                    sp.wt = wtsave;
+                    sp.iq = 0;
+                    if (!e.split_fly(sp, dpmfp, gle, imed, lgle, cohfac, gmfp)) {
+                        live = false;
+                    } else {
+                        save.x = sp.x; save.y = sp.y; save.z = sp.z;
+                        save.ix = sp.ix; save.iy = sp.iy; save.iz = sp.iz;
+                        const bool mine = isplit == i_survive;
+                        R r = e.rnd();
+                        if (r <= (R)1.0 - cohfac) {
+                            if (mine) {
+                                sp.wt *= (R)ns;
+                                e.rayleigh(sp, imed, eig, gle, lgle);
+                                p = sp;
+                                surv = true;
+                            }
+                        } else {
+                            Part<R> q;
+                            int k = e.interact_pcp(sp, q, imed, eig, gle, lgle);
+                            if (sp.iq != 0) {
+                                c0 = sp;
+                                w0 = true;
+                            } else if (mine) {
+                                sp.wt *= (R)ns;
+                                p = sp;
+                                surv = true;
+                            }
+                            if (k == 1) {
+                                if (q.iq != 0) {
+                                    c1 = q;
+                                    w1 = true;
+                                } else if (mine) {
+                                    q.wt *= (R)ns;
+                                    p = q;
+                                    surv = true;
+                                }
+                            }
+                        }
+                        cpos = rng.n;
+                    }
+                }
+            }
+            emit(a, w0, c0, (R)0, (R)0, 0, child_stream(stream, cpos), 0u, slot);
+            emit(a, w1, c1, (R)0, (R)0, 0, child_stream(~stream, cpos), 0u, slot);
+        }
+        if (have) {
+            if (e.aborted || ++iters > (1 << 22)) {
+                atomicOr(a.err, 2);
+                surv = false;
+            }
+            if (!surv) {
+                tally.flush();
+                have = false;
+            }
+            if (COUNT) {
+                tot.nhowfar += e.cnt.nhowfar; tot.nhownear += e.cnt.nhownear; tot.ncross += e.cnt.ncross;
+                tot.ndep += e.cnt.ndep; tot.nrn += e.cnt.nrn; tot.nphoton += e.cnt.nphoton;
+                tot.nelectron += e.cnt.nelectron;
+            }
+        }
     }
     flush_counters<COUNT>(a.counters, tot);
 }

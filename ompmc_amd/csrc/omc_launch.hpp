@@ -31,6 +31,7 @@ template <class R> struct EvLaunch {
     int turns;
     int max_age;
     int delta;
+    int nsplit;                  // photon splitting factor; > 1 transports photons with k_event_split
     SrcArgs<R> src;              // source of the job (ctl and out are set per round)
     int grid_event, grid_source;
 };

@@ -58,10 +58,14 @@ template <class R> hipError_t launch_round_part(const EvLaunch<R> &L, long r, in
         a.turns = L.turns;
         a.max_age = L.max_age;
         a.delta = L.delta;
+        a.nsplit = L.nsplit > 1 ? L.nsplit : 1;
         a.in = L.pool[par][part - 1];
         a.n_in = &L.ctl->cnt[par][part - 1];
         a.claim = &L.ctl->claim[par][part - 1];
-        if (part == 1) {
+        if (part == 1 && a.nsplit > 1) {
+            if (count) hipLaunchKernelGGL((k_event_split<R, true>), dim3(L.grid_event), dim3(256), 0, st, a);
+            else hipLaunchKernelGGL((k_event_split<R, false>), dim3(L.grid_event), dim3(256), 0, st, a);
+        } else if (part == 1) {
             if (count) hipLaunchKernelGGL((k_event<R, true, true>), dim3(L.grid_event), dim3(256), 0, st, a);
             else hipLaunchKernelGGL((k_event<R, true, false>), dim3(L.grid_event), dim3(256), 0, st, a);
         } else {
