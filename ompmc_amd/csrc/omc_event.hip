@@ -131,11 +131,11 @@ template <class R>
 int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int dest, bool count, bool timing,
             JobStats &js) {
     constexpr int K = 4;
-    // With photon splitting a photon puts up to 2 x nsplit charged particles
-    // into the queues in one launch: inject nsplit times fewer primaries, so
-    // that the particles in flight stay near the target and within the pools.
-    const long long inj_target = L.nsplit > 1 ? std::max<long long>(64, S.target / L.nsplit) : S.target;
-    CK(launch_job_start(S.ctl, nhist, inj_target, S.cap, dest, st));
+    // With photon splitting one split puts up to 2 x nsplit charged particles
+    // into the queues: a round injects at most target / nsplit primaries, so
+    // that a round's photons fit their products into the pools.
+    const long long inj_max = L.nsplit > 1 ? std::max<long long>(64, S.target / L.nsplit) : 0;
+    CK(launch_job_start(S.ctl, nhist, S.target, S.cap, inj_max, dest, st));
     static_assert(2 * sizeof(EvCtl) <= PINNED_SMALL_BYTES, "control-block snapshots exceed a pinned buffer");
     PinnedSmall snapbuf;
     if (!snapbuf.p) return omc_seterr(OMC_ERR_NOMEM, "pinned host buffer");
@@ -167,7 +167,7 @@ int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int 
         }
         js.n_source += K; js.n_photon += K; js.n_charged += K; js.n_other += K;
     };
-    const long rmax = max_rounds(nhist, inj_target);
+    const long rmax = max_rounds(nhist, inj_max > 0 ? inj_max : S.target);
     long r = 0;
     int cur = 0;
     CK(enqueue(r, 0));

@@ -66,6 +66,7 @@ struct EvCtl {
     long long nhist, next_hist;       // histories of the job / injected so far
     long long inj_h0, inj_n, inj_at;  // injection of the coming round
     long long target, cap;
+    long long inj_max;                // primaries injected per round at most (0: no limit)
     long long rounds, done_round;     // rounds retired; round at which the job drained (-1 while running)
     unsigned long long events[2];     // photon / charged particles taken from the queues
     unsigned long long peak;          // largest number of particles in flight
@@ -278,8 +279,11 @@ k_event(EvArgs<R> a) {
 // the geometry ends the split.  Charged products keep the sub-photon's
 // weight and are queued.  Of the scattered photons only those of one
 // randomly chosen sub-photon survive, with the full weight; there is at most
-// one, and it stays in the lane and is split in turn.  The reference keeps
-// the same particles on its stack, in another order.
+// one, and it goes back to the photon queue, with its place in its random
+// stream, to be split in a later round.  (Kept in the lane, a photon that
+// scatters many times held the whole launch open; requeued, it follows the
+// same track.)  The reference keeps the same particles on its stack, in
+// another order.
 //
 // The loop over sub-photons has the same trip count on every lane, so a
 // wave queues the products of each sub-photon together (one atomic per
@@ -292,30 +296,26 @@ k_event_split(EvArgs<R> a) {
     const long long n = (long long)*a.n_in;
     const int ns = a.nsplit;
     Counters tot = {0, 0, 0, 0, 0, 0, 0};
-    bool have = false;
-    Part<R> p;
-    uint64_t stream = 0;
-    int slot = 0;
-    PhiloxAt rng;
-    TallyAtomic tally;
-    tally.init(a.endep);
-    int iters = 0;
     for (;;) {
+        // every lane is free here: the wave claims up to 64 photons
         long long idx = -1;
-        if (__any(!have)) {
-            const unsigned long long seen = __shfl(*(volatile const unsigned long long *)a.claim, 0);
-            if ((long long)seen < n) idx = wave_reserve(a.claim, !have);
-        }
-        if (!have && idx >= 0 && idx < n) {
-            iters = 0;
+        const unsigned long long seen = __shfl(*(volatile const unsigned long long *)a.claim, 0);
+        if ((long long)seen < n) idx = wave_reserve(a.claim, true);
+        const bool have = idx >= 0 && idx < n;
+        if (!__any(have)) break;
+        Part<R> p;
+        uint64_t stream = 0;
+        int slot = 0;
+        PhiloxAt rng;
+        TallyAtomic tally;
+        tally.init(a.endep);
+        if (have) {
             load(a.in, idx, p);
             stream = a.in.stream[idx];
             slot = a.in.slot[idx];
             rng.at(a.seed, stream, a.in.ctr[idx]);
             tally.init(a.endep + (size_t)slot * a.slot_stride);
-            have = true;
         }
-        if (!__any(have)) break;
         Engine<R, PhiloxAt, TallyAtomic> e{*a.t, *a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
         e.nsplit = ns;
         // set-up of the split (src/ompmc.c:1900-1940)
@@ -401,20 +401,17 @@ k_event_split(EvArgs<R> a) {
             emit(a, w0, c0, (R)0, (R)0, 0, child_stream(stream, cpos), 0u, slot);
             emit(a, w1, c1, (R)0, (R)0, 0, child_stream(~stream, cpos), 0u, slot);
         }
-        if (have) {
-            if (e.aborted || ++iters > (1 << 22)) {
-                atomicOr(a.err, 2);
-                surv = false;
-            }
-            if (!surv) {
-                tally.flush();
-                have = false;
-            }
-            if (COUNT) {
-                tot.nhowfar += e.cnt.nhowfar; tot.nhownear += e.cnt.nhownear; tot.ncross += e.cnt.ncross;
-                tot.ndep += e.cnt.ndep; tot.nrn += e.cnt.nrn; tot.nphoton += e.cnt.nphoton;
-                tot.nelectron += e.cnt.nelectron;
-            }
+        if (have && e.aborted) {
+            atomicOr(a.err, 2);
+            surv = false;
+        }
+        tally.flush();
+        // the survivor continues from its place in the photon's stream
+        emit(a, have && surv, p, (R)0, (R)0, 0, stream, rng.n, slot);
+        if (COUNT && have) {
+            tot.nhowfar += e.cnt.nhowfar; tot.nhownear += e.cnt.nhownear; tot.ncross += e.cnt.ncross;
+            tot.ndep += e.cnt.ndep; tot.nrn += e.cnt.nrn; tot.nphoton += e.cnt.nphoton;
+            tot.nelectron += e.cnt.nelectron;
         }
     }
     flush_counters<COUNT>(a.counters, tot);
@@ -483,6 +480,7 @@ __device__ __forceinline__ void round_end(EvCtl *c, int par) {
     long long space = c->cap - (long long)c->cnt[nx][c->inj_dest];
     long long n = left < room ? left : room;
     if (n > space) n = space;
+    if (c->inj_max > 0 && n > c->inj_max) n = c->inj_max;
     if (n < 0) n = 0;
     c->inj_h0 = c->next_hist;
     c->inj_n = n;
@@ -500,7 +498,8 @@ static __global__ void k_round_end(EvCtl *c, int par) {
 
 // Starts a job: clears the queues and plans the first injection into the
 // pools of parity 0.
-static __global__ void k_job_start(EvCtl *c, long long nhist, long long target, long long cap, int dest) {
+static __global__ void k_job_start(EvCtl *c, long long nhist, long long target, long long cap, long long inj_max,
+                                   int dest) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     c->cnt[0][0] = c->cnt[0][1] = c->cnt[1][0] = c->cnt[1][1] = 0;
     c->claim[0][0] = c->claim[0][1] = c->claim[1][0] = c->claim[1][1] = 0;
@@ -508,6 +507,7 @@ static __global__ void k_job_start(EvCtl *c, long long nhist, long long target, 
     c->next_hist = 0;
     c->target = target;
     c->cap = cap;
+    c->inj_max = inj_max;
     c->rounds = -1;
     c->done_round = -1;
     c->events[0] = c->events[1] = 0;

@@ -50,8 +50,9 @@ template hipError_t launch_sampler<float>(const SampArgs<float> &, hipStream_t);
 template hipError_t launch_rounds<float>(const EvLaunch<float> &, long, int, bool, hipStream_t);
 template hipError_t launch_round_part<float>(const EvLaunch<float> &, long, int, bool, hipStream_t);
 
-hipError_t launch_job_start(EvCtl *c, long long nhist, long long target, long long cap, int dest, hipStream_t st) {
-    hipLaunchKernelGGL(k_job_start, dim3(1), dim3(64), 0, st, c, nhist, target, cap, dest);
+hipError_t launch_job_start(EvCtl *c, long long nhist, long long target, long long cap, long long inj_max, int dest,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(k_job_start, dim3(1), dim3(64), 0, st, c, nhist, target, cap, inj_max, dest);
     return hipGetLastError();
 }
 

@@ -40,7 +40,8 @@ template <class R> struct EvLaunch {
 template <class R> hipError_t launch_rounds(const EvLaunch<R> &L, long r0, int nround, bool count, hipStream_t st);
 // One part of round r: 0 source, 1 photon events, 2 charged events, 3 round end.
 template <class R> hipError_t launch_round_part(const EvLaunch<R> &L, long r, int part, bool count, hipStream_t st);
-hipError_t launch_job_start(EvCtl *c, long long nhist, long long target, long long cap, int dest, hipStream_t st);
+hipError_t launch_job_start(EvCtl *c, long long nhist, long long target, long long cap, long long inj_max, int dest,
+                            hipStream_t st);
 
 // Dij post-processing (omc_dij.hpp), FP64
 hipError_t launch_accum_slots(double *e, double *sum, double *sum2, long n, long stride, int nslot, const double *nper,
