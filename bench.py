@@ -360,6 +360,30 @@ def bench_c3(a):
         out["fp64_plan_dose_rel_diff"] = {"value": d, "sigma": sig, "beamlets": int(sub.n), "region": "> 20% of max",
                                           "seeds": 3}
         log(f"fp32/fp64: {d:+.3e} +- {sig:.1e}")
+    if not a.no_split and form == omc.FORM_EVENT:
+        # photon splitting (the reference's nsplit): the same beamlets and
+        # histories at several splitting factors; efficiency = 1 / (plan dose
+        # uncertainty^2 x seconds), the figure splitting exists to raise
+        sub = subset(bix, 20)
+        kw = dict(nbatch=nbatch, rel_threshold=rel, want_plan=True, precision=prec, group=a.group,
+                  csc_on_device=True, download=False)
+        res = {}
+        for ns in (1, 4, 20):
+            t0 = time.perf_counter()
+            pl = ctx.run_dij(gph, spec, sub, nsplit=ns, **kw)[2]
+            secs = time.perf_counter() - t0
+            spd, spv = pl
+            sel = spd > 0.5 * spd.max()
+            u = float(np.mean(np.sqrt(spv[sel]) / spd[sel]))
+            res[str(ns)] = {"seconds": secs, "plan_dose_rel_unc_gt50": u, "efficiency": 1.0 / (u * u * secs)}
+        for v in res.values():
+            v["efficiency_vs_nsplit1"] = v["efficiency"] / res["1"]["efficiency"]
+        out["photon_splitting"] = {"beamlets": int(sub.n), "histories": int(np.sum(np.maximum(sub.nhist // nbatch, 1))
+                                                                                * nbatch),
+                                   "nsplit": res, "what": "event form, every 20th beamlet, the same histories at "
+                                                          "each splitting factor; efficiency = 1/(unc^2 s)"}
+        log("splitting: " + ", ".join(f"nsplit {k} {v['seconds']:.2f} s unc {v['plan_dose_rel_unc_gt50']:.4f} "
+                                      f"eff x{v['efficiency_vs_nsplit1']:.2f}" for k, v in res.items()))
     if not a.no_serve:
         loads = [float(x) for x in a.serve_loads.split(",")]
         runs = [serve_mix(ctx, value, precision=prec, load=ld, nreq=a.serve_requests, max_hist=a.serve_max_hist)
@@ -454,6 +478,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-split", action="store_true", help="c3: skip the photon-splitting measurement")
     ap.add_argument("--subset", type=int, default=1, help="c3: every n-th beamlet only (profiling runs)")
     ap.add_argument("--serve-loads", default="0.5,0.8,1.0",
                     help="offered loads of the served mix, as fractions of the batched throughput (e.g. 0.5,0.8,1.0)")

@@ -542,6 +542,12 @@ int omc_check_opts(const omc_run_opts *o) {
     if (!o) return omc_seterr(OMC_ERR_ARG, "null run options");
     if (o->precision != OMC_FP32 && o->precision != OMC_FP64) return omc_seterr(OMC_ERR_ARG, "precision %d", o->precision);
     if (o->nsplit < 1 || o->nsplit > 64) return omc_seterr(OMC_ERR_ARG, "nsplit %d (1..64)", o->nsplit);
+    // the history form keeps a history's particles in a 96-entry stack; the
+    // reference's algorithm fills ~70 entries at nsplit 8 and ~220 at 20
+    // (6 MV beam on a 38 cm slab), so larger factors run in the event form
+    if (o->form == OMC_FORM_HISTORY && o->nsplit > HISTORY_MAX_NSPLIT)
+        return omc_seterr(OMC_ERR_ARG, "nsplit %d: the history form splits by at most %d (its particle stack holds "
+                          "%d); use the event form", o->nsplit, HISTORY_MAX_NSPLIT, SPLIT_STACK);
     if (o->form != OMC_FORM_HISTORY && o->form != OMC_FORM_EVENT) return omc_seterr(OMC_ERR_ARG, "form %d", o->form);
     if (o->rng == OMC_RNG_REPLAY && (!o->replay || !o->replay_offsets))
         return omc_seterr(OMC_ERR_ARG, "replay random numbers requested without a stream");

@@ -267,6 +267,7 @@ __global__ void __launch_bounds__(256) k_history(HistArgs<R> a) {
 // because the splitting photon routine pushes sub-photons and compacts the
 // stack in place.
 constexpr int SPLIT_STACK = 96;
+constexpr int HISTORY_MAX_NSPLIT = 8;     // largest nsplit the history form accepts
 
 template <class R, class G, bool COUNT>
 __global__ void __launch_bounds__(256) k_history_split(HistArgs<R> a) {

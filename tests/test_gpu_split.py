@@ -159,3 +159,16 @@ def test_dij_split_event_matches_history_form(tmp_path):
           f"{np.mean(z <= 3):.3f}")
     assert sel.sum() > 20
     assert np.mean(z <= 2.0) >= 0.85 and np.mean(z <= 3.0) >= 0.97
+
+
+def test_history_form_refuses_splitting_beyond_its_stack():
+    """The history form holds a history's particles in a fixed stack and
+    refuses splitting factors that the reference's algorithm would overflow
+    it with; the event form takes them."""
+    name = "split"
+    c, ph = _ctx(name)
+    with pytest.raises(omc.EngineError, match="event form"):
+        _run(c, ph, name, 100, 2, precision=omc.FP32, form=omc.FORM_HISTORY, nsplit=9)
+    g1, _, st = _run(c, ph, name, 100, 2, precision=omc.FP32, form=omc.FORM_EVENT, nsplit=9)
+    c.close()
+    assert st.errors == 0 and g1[1:].sum() > 0
