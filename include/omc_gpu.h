@@ -120,7 +120,8 @@ typedef struct omc_run_opts {
     int grid, block;        /* 0 = automatic */
     int count_events;       /* 1: fill the event counters and per-kernel times of omc_run_stats;
                                2: per-kernel times only (events around every transport kernel) */
-    int64_t event_target;   /* event form: particles in flight per round (0 = automatic) */
+    int64_t event_target;   /* event form: particles in flight per round (0 = automatic: 4096 per compute
+                               unit, four times that with nsplit > 1) */
     int electron_budget;    /* event form: condensed-history steps per charged-particle event (0 = automatic) */
     int group;              /* Dij: beamlets transported together (0 = as many as hbm_budget allows,
                                1 = one beamlet at a time, the reference's loop order) */

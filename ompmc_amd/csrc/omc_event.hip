@@ -77,7 +77,13 @@ template <class R> size_t event_bytes(long long target) {
 // Particles in flight: 4096 per compute unit.  On the 1960-beamlet plan the
 // rate is flat from 1 M to 2 M in flight (12.3-12.6 M histories/s) and falls
 // below (10.1 M at 256 k, 12.0 M at 512 k) and above (11.6 M at 4 M).
-long long default_target(int dev) { return (long long)device_cus(dev) * 4096; }
+// With photon splitting a history carries nsplit times more electrons, and
+// rounds are fewer and fuller with four times the particles in flight: at
+// nsplit 20 on every 20th beamlet 1.40 s at 1 M, 1.13 s at 2 M, 1.05 s at
+// 4 M and 1.04 s at 8 M (15.3 s against 16.3 s on the whole plan).
+long long default_target(int dev, int nsplit = 1) {
+    return (long long)device_cus(dev) * 4096 * (nsplit > 1 ? 4 : 1);
+}
 
 // run_job's code for a pool overflow: the caller zeroes the job's tallies,
 // doubles the pools (EvState::grow) and runs the job again.  Every particle's
@@ -311,7 +317,7 @@ int omc_run_dosxyz_event(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const o
     // order afterwards, so the result is the batch-by-batch one while the
     // job drains its longest tracks once instead of once per batch.
     const int nbj = batches_per_job(o, nbatch, (size_t)nreg * sizeof(double));
-    long long tgt = o->event_target > 0 ? o->event_target : default_target(c->device);
+    long long tgt = o->event_target > 0 ? o->event_target : default_target(c->device, o->nsplit);
     if (tgt > 4 * (long long)nper * nbj) tgt = std::max<long long>(4 * (long long)nper * nbj, 64);
     int rc = S.alloc(tgt);
     if (rc) return rc;
@@ -778,7 +784,7 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
     D.nper = beamlet_nper(bix, nbatch);
     for (long n : D.nper)
         if (n >= (1L << 32)) return omc_seterr(OMC_ERR_LIMIT, "more than 2^32 histories per batch of a beamlet");
-    D.tgt = o->event_target > 0 ? o->event_target : default_target(c->device);
+    D.tgt = o->event_target > 0 ? o->event_target : default_target(c->device, o->nsplit);
     // lanes: up to three for an event-form run of several groups (o->streams
     // pins the number), each planned on its share of the memory budget.  On
     // the 1960-beamlet plan at 96 GiB: 1 lane 7.2, 2 lanes 10.1, 3 lanes
