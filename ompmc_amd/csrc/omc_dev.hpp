@@ -843,23 +843,64 @@ template <class R, class G, class T> struct Engine {
         return interact(p, q, imed, eig, gle, lgle, cohfac);
     }
 
+    // Photon data of every medium at one energy: the sub-photons of a split
+    // share the photon's energy, so their walks select these per voxel
+    // instead of loading them through the voxel's medium.  Unconditional
+    // loads, as in photon_delta (a medium beyond nmed reads medium 0's).
+    struct SplitXs { R gmfp0[MXMED], coh[MXMED]; int lgle[MXMED]; };
+    __device__ __forceinline__ void split_xs(R gle, SplitXs &xs) {
+#pragma unroll
+        for (int m = 0; m < MXMED; m++) {
+            const int mc = m < t.nmed ? m : 0;
+            const Med<R> &md = t.med[mc];
+            const int lg = pwlf_i(gle, md.ge1, md.ge0) - 1;
+            xs.lgle[m] = lg;
+            xs.gmfp0[m] = pwlf(t.gmfp + (mc * MXGE + lg), gle);
+            xs.coh[m] = pwlf(t.cohe + (mc * MXGE + lg), gle);
+        }
+    }
+
+    // medium (-1: vacuum or outside) and density ratio of the particle's
+    // voxel, loaded together
+    __device__ __forceinline__ void voxel_of(const Part<R> &p, int &imed, R &rho) const {
+        if (p.ix < 0) {
+            imed = -1;
+            rho = (R)0;
+            return;
+        }
+        const int v = p.ix + g.nx * (p.iy + g.ny * p.iz);
+        const uint8_t m = gld(g.med + v);
+        rho = gld(g.rhof + v);
+        imed = m == VACUUM ? -1 : (int)m;
+    }
+
     // Flight of one sub-photon of a split (src/ompmc.c:1960-2032): from its
     // start, `dpmfp` mean free paths along its direction, voxel by voxel.
     // Returns false when it leaves the geometry (the reference discards it
     // and ends the split), true at its interaction site, with imed the
     // medium there.  lgle, cohfac and gmfp are the caller's: like the
     // reference's function-scope variables they keep the values of the last
-    // voxel with a medium, which the interaction then uses.
-    __device__ __forceinline__ bool split_fly(Part<R> &p, R dpmfp, R gle, int &imed, int &lgle, R &cohfac, R &gmfp) {
-        imed = medium(p);
+    // voxel with a medium, which the interaction then uses.  The step
+    // arithmetic is the reference's, in its order, on values from `xs`.
+    __device__ __forceinline__ bool split_fly(Part<R> &p, R dpmfp, const SplitXs &xs, int &imed, int &lgle,
+                                              R &cohfac, R &gmfp) {
+        R rho;
+        voxel_of(p, imed, rho);
         for (;;) {
             R tstep;
             if (imed != -1) {
-                const Med<R> &m = t.med[imed];
-                lgle = pwlf_i(gle, m.ge1, m.ge0) - 1;
-                R gmfpr0 = pwlf(t.gmfp + (imed * MXGE + lgle), gle);
-                gmfp = gmfpr0 / rhof_of(p);
-                cohfac = pwlf(t.cohe + (imed * MXGE + lgle), gle);
+                R g0 = xs.gmfp0[0], ch = xs.coh[0];
+                int lg = xs.lgle[0];
+#pragma unroll
+                for (int m = 1; m < MXMED; m++) {       // no dynamic register index
+                    const bool s = imed == m;
+                    g0 = s ? xs.gmfp0[m] : g0;
+                    ch = s ? xs.coh[m] : ch;
+                    lg = s ? xs.lgle[m] : lg;
+                }
+                lgle = lg;
+                gmfp = g0 / rho;
+                cohfac = ch;
                 gmfp *= cohfac;
                 tstep = gmfp * dpmfp;
             } else {
@@ -879,7 +920,7 @@ template <class R, class G, class T> struct Engine {
             if (imed != -1) dpmfp = M<R>::fmax((R)0.0, dpmfp - ustep / gmfp);
             if (mv != 0) {
                 apply_move(p, mv, leaves);
-                imed = medium(p);
+                voxel_of(p, imed, rho);
             }
             if (imed != -1 && dpmfp <= (R)1.0E-05) return true;
         }
@@ -905,7 +946,9 @@ template <class R, class G, class T> struct Engine {
         int lgle = 0;
         R cohfac = (R)0, gmfp = (R)0;
         const int ns = nsplit;
+        SplitXs xs;
         for (;;) {                                  // start of the splitting of the top photon
+            split_xs(gle, xs);
             R rnno = rnd();
             rnno /= (R)ns;
             R d_eta = (R)1.0 / (R)ns;
@@ -932,7 +975,7 @@ template <class R, class G, class T> struct Engine {
                 p.u = save.u; p.v = save.v; p.w = save.w;
                 p.e = save.e; p.wt = wtsave;
                 p.ix = save.ix; p.iy = save.iy; p.iz = save.iz; p.iq = 0;
-                if (!split_fly(p, dpmfp, gle, imed, lgle, cohfac, gmfp)) {
+                if (!split_fly(p, dpmfp, xs, imed, lgle, cohfac, gmfp)) {
                     np -= 1;
                     if (np < 0) return np;
                     break;                          // end of the splitting loop

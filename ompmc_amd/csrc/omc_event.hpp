@@ -4,7 +4,8 @@
 // (photons, charged particles), double-buffered by round parity.  A round is
 // four launches on one stream:
 //   k_source     samples this round's injected primaries into the input pool,
-//   k_event<1>   transports the queued photons (persistent lanes, below),
+//   k_event<1>   transports the queued photons (persistent lanes, below;
+//                k_event_split when photons are split),
 //   k_event<0>   transports the queued electrons/positrons, `budget`
 //                condensed-history steps (electron_run) between refills,
 //   k_round_end  retires the consumed input pools and plans the next
@@ -324,6 +325,7 @@ k_event_split(EvArgs<R> a) {
         int i_survive = -1, imed = -1, lgle = 0;
         R cohfac = (R)0, gmfp = (R)0;
         Part<R> save = p;
+        typename Engine<R, PhiloxAt, TallyAtomic>::SplitXs xs;
         if (live) {
             if (COUNT) e.cnt.nphoton++;
             imed = e.medium(p);
@@ -333,6 +335,7 @@ k_event_split(EvArgs<R> a) {
                 live = false;
             } else {
                 gle = M<R>::log(eig);
+                e.split_xs(gle, xs);
                 R rnno = e.rnd();
                 rnno /= (R)ns;
                 d_eta = (R)1.0 / (R)ns;
@@ -358,7 +361,7 @@ k_event_split(EvArgs<R> a) {
                     Part<R> sp = save;
                     sp.wt = wtsave;
                     sp.iq = 0;
-                    if (!e.split_fly(sp, dpmfp, gle, imed, lgle, cohfac, gmfp)) {
+                    if (!e.split_fly(sp, dpmfp, xs, imed, lgle, cohfac, gmfp)) {
                         live = false;
                     } else {
                         save.x = sp.x; save.y = sp.y; save.z = sp.z;

@@ -28,13 +28,16 @@ def read_3ddose(path):
     return (nx, ny, nz), v[:o], v[o:o + n], v[o + n:o + 2 * n]
 
 
-@pytest.mark.parametrize("flags", [[], ["--fp64", "--history"]])
-def test_cli_3ddose_matches_reference_statistically(tmp_path, flags):
-    name = "w20"
+@pytest.mark.parametrize("name,flags", [("w20", []), ("w20", ["--fp64", "--history"]), ("split", [])])
+def test_cli_3ddose_matches_reference_statistically(tmp_path, name, flags):
+    """The last case has `nsplit 5` in its input, like the reference's sample
+    input, and runs in the default event form."""
     stem = write_case_inp(tmp_path, name, ncase=200000)
     r = subprocess.run([CLI, "-i", stem, "-o", "gpu"] + flags, cwd=str(tmp_path), stdout=subprocess.PIPE,
                        stderr=subprocess.STDOUT, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-2000:]
+    if DOSE_CASES[name][0].get("nsplit", 1) > 1:
+        assert "Photon splitting enabled, nsplit = 5" in r.stdout
     dims, bounds, dg, ug = read_3ddose(str(tmp_path / "gpu.3ddose"))
     # reference file: the same input run by the oracle (bitwise equal to the
     # reference's serial binary) and written by the host layer
