@@ -272,6 +272,7 @@ typedef struct omc_request {
     int nbatch;
     uint64_t seed;
     int precision;
+    int nsplit;                       /* photon splitting factor, 1..64 (0 = 1) */
 } omc_request;
 
 typedef struct omc_result {

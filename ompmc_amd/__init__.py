@@ -111,7 +111,8 @@ class CSC(C.Structure):
 class Request(C.Structure):
     _fields_ = [("kind", C.c_int), ("phantom", PhantomDesc), ("dose_src", DosxyzSourceDesc),
                 ("nperbatch", C.c_int64), ("spec", SpectrumDesc), ("bix", BeamletsDesc),
-                ("rel_threshold", C.c_double), ("nbatch", C.c_int), ("seed", C.c_uint64), ("precision", C.c_int)]
+                ("rel_threshold", C.c_double), ("nbatch", C.c_int), ("seed", C.c_uint64), ("precision", C.c_int),
+                ("nsplit", C.c_int)]
 
 
 class Result(C.Structure):
@@ -448,10 +449,11 @@ class Server:
                            ecut, pcut)
 
     def submit_dose(self, ph, spectrum, nperbatch, nbatch=10, collimator=(-5, 5, -5, 5), ssd=1.0e10, charge=0,
-                    seed=1802, precision=FP32, ecut=0.7, pcut=0.01):
+                    seed=1802, precision=FP32, ecut=0.7, pcut=0.01, nsplit=1):
         keep = []
         r = Request()
         r.kind = 0
+        r.nsplit = int(nsplit)
         r.phantom = self._phantom_desc(ph, ecut, pcut, keep)
         r.dose_src.spec = spectrum.desc()
         keep.append(spectrum)
@@ -464,10 +466,11 @@ class Server:
         return rid.value
 
     def submit_dij(self, ph, spectrum, bix, nbatch=10, rel_threshold=1e-4, seed=1802, precision=FP32, ecut=0.7,
-                   pcut=0.01, nhist_each=0):
+                   pcut=0.01, nhist_each=0, nsplit=1):
         keep = []
         r = Request()
         r.kind = 1
+        r.nsplit = int(nsplit)
         r.phantom = self._phantom_desc(ph, ecut, pcut, keep)
         r.spec = spectrum.desc()
         keep.append(spectrum)

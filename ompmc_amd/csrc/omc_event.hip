@@ -1091,7 +1091,7 @@ size_t request_bytes(omc_server *s, const omc_request &r) {
     const omc_phantom_desc &p = r.phantom;
     long nreg = (long)p.nx * p.ny * p.nz + 1;
     size_t b = omc_phantom_bytes(&p);
-    long long tgt = default_target(s->ctx->device);
+    long long tgt = default_target(s->ctx->device, r.nsplit);
     size_t ev = r.precision == OMC_FP64 ? event_bytes<double>(tgt) : event_bytes<float>(tgt);
     // dose: every batch in one job, each with its own tallies
     if (r.kind == OMC_REQ_DOSE) return b + (size_t)(r.nbatch + 2) * nreg * 8 + ev;
@@ -1122,6 +1122,7 @@ void run_request(omc_server *s, omc_server::Req *q) {
     o.precision = q->r.precision == OMC_FP64 ? OMC_FP64 : OMC_FP32;
     o.form = OMC_FORM_EVENT;
     o.seed = q->r.seed;
+    o.nsplit = q->r.nsplit;
     o.batches_per_job = q->r.kind == OMC_REQ_DOSE ? q->r.nbatch : 0;
     // concurrent requests already overlap one another's drains; extra group
     // streams per Dij request only contend for the hardware queues (served
@@ -1223,8 +1224,10 @@ extern "C" int omc_server_submit(omc_server *s, const omc_request *r, int64_t *r
     const omc_phantom_desc &p = r->phantom;
     if (p.nx < 1 || p.ny < 1 || p.nz < 1 || !p.xb || !p.yb || !p.zb || !p.med || !p.rho || r->nbatch < 1)
         return omc_seterr(OMC_ERR_ARG, "omc_server_submit: bad phantom or batches");
+    if (r->nsplit < 0 || r->nsplit > 64) return omc_seterr(OMC_ERR_ARG, "nsplit %d (1..64)", r->nsplit);
     auto *q = new omc_server::Req();
     q->r = *r;
+    if (q->r.nsplit < 1) q->r.nsplit = 1;
     long nvox = (long)p.nx * p.ny * p.nz;
     q->xb.assign(p.xb, p.xb + p.nx + 1);
     q->yb.assign(p.yb, p.yb + p.ny + 1);

@@ -113,3 +113,48 @@ def test_spectrum_inverse_cdf_matches_host_layer():
     p = w / w.sum()
     sigma = np.sqrt(p * (1 - p) / n)
     assert np.all(np.abs(obs / n - p) <= 1.0e-3 + 5 * sigma)
+
+
+def test_python_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors in ompmc_amd/__init__.py have the size and field
+    offsets of the C structs in include/omc_gpu.h (a probe compiled by gcc
+    prints them)."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    omc = pytest.importorskip("ompmc_amd")
+    if not shutil.which("gcc"):
+        pytest.skip("gcc absent")
+    pairs = {"omc_spectrum_desc": omc.SpectrumDesc, "omc_phantom_desc": omc.PhantomDesc,
+             "omc_dosxyz_source_desc": omc.DosxyzSourceDesc, "omc_run_opts": omc.RunOpts,
+             "omc_run_stats": omc.RunStats, "omc_beamlets_desc": omc.BeamletsDesc, "omc_csc": omc.CSC,
+             "omc_request": omc.Request, "omc_result": omc.Result}
+    lines = ['#include <stdio.h>', '#include "omc_gpu.h"', "int main(void) {"]
+    for cname, py in pairs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in py._fields_:
+            lines.append(f'  printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I" + os.path.join(REPO, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True, check=True).stdout.split("\n")
+    got = {(a, b): int(c) for a, b, c in (l.split() for l in out if l)}
+    for cname, py in pairs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f in py._fields_:
+            assert got[(cname, f[0])] == getattr(py, f[0]).offset, (cname, f[0])
+    # and the same members in the same order (a trailing int can hide in padding)
+    hdr = re.sub(r"/\*.*?\*/", "", open(os.path.join(REPO, "include", "omc_gpu.h")).read(), flags=re.S)
+    for cname, py in pairs.items():
+        body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (cname, cname), hdr, re.S).group(1)
+        names = []
+        for decl in body.split(";"):
+            decl = decl.strip()
+            if not decl:
+                continue
+            parts = decl.split(",")
+            names.append(re.findall(r"(\w+)\s*(?:\[[^\]]*\])?\s*$", parts[0])[0])
+            names += [re.sub(r"[\s*]|\[.*\]", "", q) for q in parts[1:]]
+        assert names == [f[0] for f in py._fields_], (cname, names)

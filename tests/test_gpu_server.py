@@ -23,6 +23,7 @@ def test_server_results_same_alone_and_in_the_mix():
     ids.append(("dij", 1, srv.submit_dij(phs[1], spec, bix, nbatch=4, seed=32)))
     ids.append(("dose", 1, srv.submit_dose(phs[1], spec, 30000, 5, collimator=(-1, 1, -1, 1), seed=33)))
     ids.append(("dij", 0, srv.submit_dij(phs[0], spec, bix, nbatch=4, seed=34)))
+    ids.append(("dij", 1, srv.submit_dij(phs[1], spec, bix, nbatch=4, seed=35, nsplit=4)))
     res = [(k, i, srv.wait(r)) for k, i, r in ids]
     srv.close()
     for k, i, r in res:
@@ -35,8 +36,8 @@ def test_server_results_same_alone_and_in_the_mix():
     assert np.allclose(res[0][2]["sum"], s1, rtol=1e-9, atol=1e-12 * s1.max())
     s1, _, _ = ctx.run_dosxyz(gph[1], spec, 30000, 5, collimator=(-1, 1, -1, 1), seed=33)
     assert np.allclose(res[2][2]["sum"], s1, rtol=1e-9, atol=1e-12 * s1.max())
-    for j, (ph, seed) in ((1, (gph[1], 32)), (3, (gph[0], 34))):
-        (jc, ir, val), _, _, _ = ctx.run_dij(ph, spec, bix, nbatch=4, seed=seed)
+    for j, (ph, seed, ns) in ((1, (gph[1], 32, 1)), (3, (gph[0], 34, 1)), (4, (gph[1], 35, 4))):
+        (jc, ir, val), _, _, _ = ctx.run_dij(ph, spec, bix, nbatch=4, seed=seed, nsplit=ns)
         jc2, ir2, val2 = res[j][2]["dij"]
         assert np.array_equal(jc, jc2) and np.array_equal(ir, ir2)
         assert np.allclose(val, val2, rtol=1e-9)
