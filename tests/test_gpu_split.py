@@ -115,9 +115,12 @@ def test_event_split_result_independent_of_schedule():
     ref = _run(c, ph, name, 4000, 4, precision=omc.FP64, form=omc.FORM_EVENT, seed=3)[0]
     for kw in (dict(event_target=100, electron_budget=1), dict(event_target=777, electron_budget=3, grid=7),
                dict(event_target=1 << 16, electron_budget=1000), dict(electron_turns=1, electron_budget=2),
-               dict(batches_per_job=1)):
-        g = _run(c, ph, name, 4000, 4, precision=omc.FP64, form=omc.FORM_EVENT, seed=3, **kw)[0]
+               dict(batches_per_job=1), dict(count_events=True)):
+        g, _, st = _run(c, ph, name, 4000, 4, precision=omc.FP64, form=omc.FORM_EVENT, seed=3, **kw)
         assert np.allclose(g, ref, rtol=1e-10, atol=1e-12 * ref.max()), kw
+        if kw.get("count_events"):
+            # every split counts once as a photon step; its sub-photons cross voxels
+            assert st.n_photon >= st.events_photon > 0 and st.ev_cross[0] > 0 and st.n_electron > 0
     c.close()
 
 
