@@ -363,12 +363,16 @@ def bench_c3(a):
     if not a.no_split and form == omc.FORM_EVENT:
         # photon splitting (the reference's nsplit): the same beamlets and
         # histories at several splitting factors; efficiency = 1 / (plan dose
-        # uncertainty^2 x seconds), the figure splitting exists to raise
+        # uncertainty^2 x seconds), the figure splitting exists to raise.
+        # Each factor runs once untimed first, like the main step's warm-up
+        # (split runs keep four times the particles in flight, and the first
+        # one allocates those pools)
         sub = subset(bix, 20)
         kw = dict(nbatch=nbatch, rel_threshold=rel, want_plan=True, precision=prec, group=a.group,
                   csc_on_device=True, download=False)
         res = {}
         for ns in (1, 4, 20):
+            ctx.run_dij(gph, spec, sub, nsplit=ns, **kw)
             t0 = time.perf_counter()
             pl = ctx.run_dij(gph, spec, sub, nsplit=ns, **kw)[2]
             secs = time.perf_counter() - t0
@@ -381,7 +385,8 @@ def bench_c3(a):
         out["photon_splitting"] = {"beamlets": int(sub.n), "histories": int(np.sum(np.maximum(sub.nhist // nbatch, 1))
                                                                                 * nbatch),
                                    "nsplit": res, "what": "event form, every 20th beamlet, the same histories at "
-                                                          "each splitting factor; efficiency = 1/(unc^2 s)"}
+                                                          "each splitting factor, after one untimed run each; "
+                                                          "efficiency = 1/(unc^2 s)"}
         log("splitting: " + ", ".join(f"nsplit {k} {v['seconds']:.2f} s unc {v['plan_dose_rel_unc_gt50']:.4f} "
                                       f"eff x{v['efficiency_vs_nsplit1']:.2f}" for k, v in res.items()))
     if not a.no_serve:
