@@ -148,6 +148,11 @@ def test_autotuner_picks_a_measured_setting(tmp_path, monkeypatch):
     assert best["electron_turns"] in (3, -1)
     assert (tmp_path / "tune.json").exists()
     assert tune.tune_dij(c, ph, spec, bix, nbatch=4, sample=8) == best          # from the cache
+    # a split plan is tuned with splitting, under its own cache entry
+    bs = tune.tune_dij(c, ph, spec, bix, nbatch=4, sample=8, targets=(4096, 65536), budgets=(8,), turns=(24,),
+                       nsplit=4)
+    assert bs["nsplit"] == 4 and best["nsplit"] == 1
+    assert tune.tune_dij(c, ph, spec, bix, nbatch=4, sample=8, nsplit=4) == bs
     a = c.run_dij(ph, spec, bix, nbatch=4, seed=5, **best)[0]
     b = c.run_dij(ph, spec, bix, nbatch=4, seed=5, event_target=999, electron_budget=3, electron_turns=2)[0]
     c.close()
