@@ -418,8 +418,9 @@ template <class R> static void fill_geom(omc_ctx *c, PhantomDev *pd, Geom<R> &G)
         return true;
     };
     double ix, iy, iz;
-    bool u = even(ph.xb, ph.nx, ix) & even(ph.yb, ph.ny, iy) & even(ph.zb, ph.nz, iz);
-    G.uniform = u ? 1 : 0;
+    // every axis is evaluated: each call also sets that axis' 1 / spacing
+    const bool ex = even(ph.xb, ph.nx, ix), ey = even(ph.yb, ph.ny, iy), ez = even(ph.zb, ph.nz, iz);
+    G.uniform = ex && ey && ez ? 1 : 0;
     G.inv_dx = (R)ix;
     G.inv_dy = (R)iy;
     G.inv_dz = (R)iz;

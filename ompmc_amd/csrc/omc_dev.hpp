@@ -652,7 +652,6 @@ template <class R, class G, class T> struct Engine {
     // photon step (src/ompmc.c:1867-2139) for nsplit = 1: transport to the
     // next interaction (or out of the phantom / below PCUT), interact, return.
     __device__ OMC_INL int photon(Part<R> &p, Part<R> &q) {
-        const R RM = (R)0.5109989461;
         if (count) cnt.nphoton++;
         int imed = medium(p);
         R eig = p.e;
