@@ -228,3 +228,47 @@ def test_stuck_particle_guard_reports_limit():
                     electron_budget=2)
     c.close()
     assert st.errors == 0
+
+
+def test_unequal_voxel_spacing_agrees_with_oracle(tmp_path):
+    """Voxel boundaries at unequal spacing, which the egsphant format allows:
+    the event form then finds the voxel of a delta-tracking site by bisection
+    instead of by multiplication.  FP64 event form against the oracle on the
+    same phantom file, and FP32 event form against the history form's
+    voxel-by-voxel walk, per voxel within the batch uncertainties."""
+    rng = np.random.default_rng(17)
+
+    def bounds(n, lo, hi):
+        w = rng.uniform(0.5, 1.5, n)
+        b = lo + np.concatenate([[0.0], np.cumsum(w)]) * (hi - lo) / w.sum()
+        b[-1] = hi
+        return b
+    n = 16
+    xb, yb, zb = bounds(n, -4.0, 4.0), bounds(n, -4.0, 4.0), bounds(n, 0.0, 10.0)
+    assert np.ptp(np.diff(xb)) > 0.2 and np.ptp(np.diff(zb)) > 0.2
+    zc = 0.5 * (zb[1:] + zb[:-1])
+    lay = np.where(zc < 3.0, 1, np.where(zc < 6.0, 2, 3)).astype(np.int32)
+    med = np.broadcast_to(lay[:, None, None], (n, n, n)).copy()
+    rho = np.where(med == 1, 1.0, np.where(med == 2, 0.26, 1.85))
+    phf = str(tmp_path / "uneven.egsphant")
+    P.write_egsphant(phf, P.Phantom([P.WATER, P.LUNG, P.BONE], xb, yb, zb, med, rho, "uneven"))
+    ph_h = P.read_egsphant(phf)
+    nh, nb = 50000, 10
+    d = P.default_inp(phf, str(tmp_path) + "/", energy=2.0, ncase=nh, nbatch=nb, collimator=(-2, 2, -2, 2))
+    stem = P.write_inp(str(tmp_path / "uneven"), d)[:-4]
+    s1, s2, _ = B.Session(stem).dose_omp(nh // nb, nb)
+    c = omc.Context(ph_h.media)
+    ph = c.phantom(ph_h, ecut=0.7, pcut=0.01)
+    spec = omc.Spectrum.mono(2.0)
+    kw = dict(collimator=(-2, 2, -2, 2))
+    g1, g2, st = c.run_dosxyz(ph, spec, nh // nb, nb, precision=omc.FP64, form=omc.FORM_EVENT, seed=13, **kw)
+    e1, e2, se = c.run_dosxyz(ph, spec, nh // nb, nb, precision=omc.FP32, form=omc.FORM_EVENT, seed=14, **kw)
+    h1, h2, sh = c.run_dosxyz(ph, spec, nh // nb, nb, precision=omc.FP32, form=omc.FORM_HISTORY, seed=15, **kw)
+    c.close()
+    assert st.errors == 0 and se.errors == 0 and sh.errors == 0
+    for what, (a1, a2, b1, b2) in (("event FP64 vs oracle", (s1, s2, g1, g2)),
+                                   ("event FP32 vs history FP32", (h1, h2, e1, e2))):
+        f2, f3, nsel = _agreement(a1, a2, b1, b2, nb)
+        print(f"unequal spacing, {what}: {nsel} voxels, {f2:.3f} / {f3:.3f}")
+        assert nsel > 100
+        assert f2 >= 0.88 and f3 >= 0.97, (what, f2, f3)
