@@ -151,6 +151,18 @@ def water_cube(n: int = 60, vox: float = 0.5) -> Phantom:
                    np.ones((n, n, n), np.int32), np.ones((n, n, n)), f"water{n}")
 
 
+def vacuum_gap_phantom(n: int = 12, vox: float = 0.5, gap=(2.0, 3.5)) -> Phantom:
+    """Water cube with a vacuum layer (medium 0) across it for gap[0] < z <
+    gap[1] (voxel centres decide): particles cross it without interacting."""
+    ph = water_cube(n, vox)
+    zc = 0.5 * (ph.zb[1:] + ph.zb[:-1])
+    sel = (zc > gap[0]) & (zc < gap[1])
+    ph.med[sel, :, :] = 0
+    ph.rho[sel, :, :] = 0.0
+    ph.name = f"vacgap{n}"
+    return ph
+
+
 def slab_phantom(n: int = 128, vox: float = 0.3) -> Phantom:
     """Layers along z: water 0-9 cm, lung 0.26 9-21 cm, bone 1.85 21-27 cm,
     water to the end (voxel centres decide the layer)."""

@@ -53,7 +53,7 @@ def slab(tmp_path_factory):
     c.close()
 
 
-@pytest.mark.parametrize("name", ["slab", "electron", "positron"])
+@pytest.mark.parametrize("name", ["slab", "electron", "positron", "vacuum"])
 def test_event_fp64_agrees_with_oracle(tmp_path, name):
     """Event form (FP64, Philox) against the oracle (FP64, RANMAR) on the same
     phantom, source and history count."""
