@@ -543,12 +543,6 @@ int omc_check_opts(const omc_run_opts *o) {
     if (!o) return omc_seterr(OMC_ERR_ARG, "null run options");
     if (o->precision != OMC_FP32 && o->precision != OMC_FP64) return omc_seterr(OMC_ERR_ARG, "precision %d", o->precision);
     if (o->nsplit < 1 || o->nsplit > 64) return omc_seterr(OMC_ERR_ARG, "nsplit %d (1..64)", o->nsplit);
-    // the history form keeps a history's particles in a 96-entry stack; the
-    // reference's algorithm fills ~70 entries at nsplit 8 and ~220 at 20
-    // (6 MV beam on a 38 cm slab), so larger factors run in the event form
-    if (o->form == OMC_FORM_HISTORY && o->nsplit > HISTORY_MAX_NSPLIT)
-        return omc_seterr(OMC_ERR_ARG, "nsplit %d: the history form splits by at most %d (its particle stack holds "
-                          "%d); use the event form", o->nsplit, HISTORY_MAX_NSPLIT, SPLIT_STACK);
     if (o->form != OMC_FORM_HISTORY && o->form != OMC_FORM_EVENT) return omc_seterr(OMC_ERR_ARG, "form %d", o->form);
     if (o->rng == OMC_RNG_REPLAY && (!o->replay || !o->replay_offsets))
         return omc_seterr(OMC_ERR_ARG, "replay random numbers requested without a stream");
@@ -582,6 +576,8 @@ static int run_dosxyz_history(omc_ctx *c, hipStream_t stream, PhantomDev *pd, co
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> kev;
     float ms = 0.f;
+    const int hblock = o->block > 0 ? o->block : 256;
+    int max_grid = o->grid > 0 ? o->grid : 2048;
     auto wall0 = std::chrono::steady_clock::now();
     if (!A.ok) { rc = omc_seterr(OMC_ERR_NOMEM, "device allocation failed"); goto done; }
     rc = make_dosxyz_src<R>(A, pd, src, a.dos);
@@ -607,6 +603,8 @@ static int run_dosxyz_history(omc_ctx *c, hipStream_t stream, PhantomDev *pd, co
     a.replay = d_rn;
     a.chunk = 1;
     a.nsplit = o->nsplit;
+    rc = alloc_split_stacks<R>(A, a, o->nsplit, hblock, max_grid);
+    if (rc) goto done;
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
     if (o->count_events) {
@@ -619,9 +617,10 @@ static int run_dosxyz_history(omc_ctx *c, hipStream_t stream, PhantomDev *pd, co
         a.hist_base = (long)b * nper;
         a.replay_off = d_off ? d_off + (long)b * nper : nullptr;
         HIPCHK(hipMemsetAsync(d_next, 0, sizeof(unsigned long long), stream));
-        int block = o->block > 0 ? o->block : 256;
+        int block = hblock;
         long want = (nper + block - 1) / block;
         int grid = o->grid > 0 ? o->grid : (int)(want < 2048 ? want : 2048);
+        if (grid > max_grid) grid = max_grid;
         if (o->count_events) HIPCHK(hipEventRecord(kev[2 * b], stream));
         HIPCHK(launch_history<R>(a, o->rng == OMC_RNG_REPLAY ? RNG_REPLAY : RNG_PHILOX, (o->count_events & 1) != 0,
                                  grid, block, stream));

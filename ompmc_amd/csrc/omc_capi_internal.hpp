@@ -158,3 +158,23 @@ template <class R> inline int make_dosxyz_src(DevArena &A, const PhantomDev *pd,
         if (e_ != hipSuccess) { rc = omc_seterr(OMC_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); goto done; } \
     } while (0)
 
+// Device-memory particle stacks for history-form splitting beyond the
+// scratch stack: split_stack_depth(nsplit) entries for every lane of a
+// launch of up to max_grid blocks of `block` lanes.  The stacks take at most
+// 8 GiB; max_grid comes back lowered to what fits, and launches must not
+// exceed it.
+template <class R> int alloc_split_stacks(DevArena &A, omc::HistArgs<R> &a, int nsplit, int block, int &max_grid) {
+    a.stack = nullptr;
+    a.stack_depth = 0;
+    if (nsplit <= omc::HISTORY_SCRATCH_NSPLIT) return OMC_OK;
+    const int depth = omc::split_stack_depth(nsplit);
+    const size_t per_block = (size_t)block * depth * sizeof(omc::Part<R>);
+    size_t nb = ((size_t)8 << 30) / per_block;
+    if (nb < 1) nb = 1;
+    if (max_grid < 1) max_grid = 1;
+    if ((size_t)max_grid > nb) max_grid = (int)nb;
+    a.stack = A.alloc_n<omc::Part<R>>((size_t)max_grid * block * depth);
+    if (!A.ok) return omc_seterr(OMC_ERR_NOMEM, "particle stacks for nsplit %d", nsplit);
+    a.stack_depth = depth;
+    return OMC_OK;
+}

@@ -489,6 +489,7 @@ template <class R> struct DijLane {
     double t_hist = 0, t_other = 0, histories = 0;
     uint64_t n_hist = 0, n_other = 0;
     int rc = OMC_OK;
+    int max_grid = 2048;               // history form: largest launch the particle stacks cover
     std::string msg;
     ~DijLane() {
         A.release();      // cached blocks: each release synchronises st, so all go before st does
@@ -592,6 +593,9 @@ template <class R> int dij_lane_init(DijRun<R> &D, DijLane<R> &l) {
         H.replay_off = D.d_rstart;
         H.replay_end = D.d_rend;
         H.nsplit = o->nsplit;
+        l.max_grid = o->grid > 0 ? o->grid : 2048;
+        int rc = alloc_split_stacks<R>(A, H, o->nsplit, o->block > 0 ? o->block : 256, l.max_grid);
+        if (rc) return rc;
     }
     return OMC_OK;
 }
@@ -664,6 +668,7 @@ template <class R> int dij_lane_group(DijRun<R> &D, DijLane<R> &l, int gi) {
             int block = o->block > 0 ? o->block : 256;
             long want = (njob + block - 1) / block;
             int grid = o->grid > 0 ? o->grid : (int)std::min<long>(want, 2048);
+            if (grid > l.max_grid) grid = l.max_grid;
             if (D.timing) CK(hipEventRecord(ev.ev[0], stream));
             CK(launch_history<R>(H, D.replay ? RNG_REPLAY : RNG_PHILOX, D.count, grid, block, stream));
             if (D.timing) {

@@ -162,6 +162,8 @@ template <class R> struct HistArgs {
     unsigned long long *next;    // dynamic work counter (zeroed by the host)
     int chunk;
     int nsplit;                  // photon splitting factor; > 1 selects k_history_split
+    Part<R> *stack;              // nsplit > HISTORY_SCRATCH_NSPLIT: per-lane particle stacks in device memory,
+    int stack_depth;             // stack_depth entries for each lane of the launch (null: stacks in scratch)
 };
 
 // Random stream of history h_local of batch `batch` of a beamlet: the
@@ -263,15 +265,24 @@ __global__ void __launch_bounds__(256) k_history(HistArgs<R> a) {
 }
 
 // History form with photon splitting: the whole LIFO stack of a history
-// lives in an array (scratch), indexed like the reference's particle stack,
-// because the splitting photon routine pushes sub-photons and compacts the
-// stack in place.
+// lives in an array, indexed like the reference's particle stack, because
+// the splitting photon routine pushes sub-photons and compacts the stack in
+// place.  Up to nsplit HISTORY_SCRATCH_NSPLIT the array is in scratch
+// (SPLIT_STACK entries); the reference's algorithm fills ~70 entries at
+// nsplit 8, ~220 at 20 and ~790 at 64 (6 MV on a 38 cm slab), so larger
+// factors (DEEP) use a stack of a.stack_depth entries per lane in device
+// memory, which the host allocates for every lane of the launch.
 constexpr int SPLIT_STACK = 96;
-constexpr int HISTORY_MAX_NSPLIT = 8;     // largest nsplit the history form accepts
+constexpr int HISTORY_SCRATCH_NSPLIT = 8;
 
-template <class R, class G, bool COUNT>
+// depth of the device-memory stacks for a splitting factor
+inline int split_stack_depth(int nsplit) { return 32 * nsplit + 64; }
+
+template <class R, class G, bool COUNT, bool DEEP>
 __global__ void __launch_bounds__(256) k_history_split(HistArgs<R> a) {
-    Part<R> st[SPLIT_STACK];
+    Part<R> local[DEEP ? 1 : SPLIT_STACK];
+    Part<R> *st = DEEP ? a.stack + ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * (size_t)a.stack_depth : local;
+    const int cap = DEEP ? a.stack_depth : SPLIT_STACK;
     Counters tot = {0, 0, 0, 0, 0, 0, 0};
     for (;;) {
         long h0 = 0;
@@ -298,14 +309,14 @@ __global__ void __launch_bounds__(256) k_history_split(HistArgs<R> a) {
             long guard = 0;
             while (np >= 0) {
                 if (st[np].iq == 0) {
-                    np = e.photon_split(st, np, SPLIT_STACK, &err);
+                    np = e.photon_split(st, np, cap, &err);
                 } else {
                     Part<R> q;
                     int d = e.electron(st[np], q);
                     if (d < 0) {
                         np -= 1;
                     } else if (d > 0) {
-                        if (np + 1 >= SPLIT_STACK) { err = 1; break; }
+                        if (np + 1 >= cap) { err = 1; break; }
                         st[++np] = q;
                     }
                 }

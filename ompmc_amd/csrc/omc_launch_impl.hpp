@@ -7,13 +7,23 @@ namespace omc {
 
 template <class R> hipError_t launch_history(const HistArgs<R> &a, int rng, bool count, int grid, int block,
                                              hipStream_t st) {
+    if (a.nsplit > 1 && a.stack) {
+        if (rng == RNG_REPLAY) {
+            if (count) hipLaunchKernelGGL((k_history_split<R, Replay, true, true>), dim3(grid), dim3(block), 0, st, a);
+            else hipLaunchKernelGGL((k_history_split<R, Replay, false, true>), dim3(grid), dim3(block), 0, st, a);
+        } else {
+            if (count) hipLaunchKernelGGL((k_history_split<R, Philox, true, true>), dim3(grid), dim3(block), 0, st, a);
+            else hipLaunchKernelGGL((k_history_split<R, Philox, false, true>), dim3(grid), dim3(block), 0, st, a);
+        }
+        return hipGetLastError();
+    }
     if (a.nsplit > 1) {
         if (rng == RNG_REPLAY) {
-            if (count) hipLaunchKernelGGL((k_history_split<R, Replay, true>), dim3(grid), dim3(block), 0, st, a);
-            else hipLaunchKernelGGL((k_history_split<R, Replay, false>), dim3(grid), dim3(block), 0, st, a);
+            if (count) hipLaunchKernelGGL((k_history_split<R, Replay, true, false>), dim3(grid), dim3(block), 0, st, a);
+            else hipLaunchKernelGGL((k_history_split<R, Replay, false, false>), dim3(grid), dim3(block), 0, st, a);
         } else {
-            if (count) hipLaunchKernelGGL((k_history_split<R, Philox, true>), dim3(grid), dim3(block), 0, st, a);
-            else hipLaunchKernelGGL((k_history_split<R, Philox, false>), dim3(grid), dim3(block), 0, st, a);
+            if (count) hipLaunchKernelGGL((k_history_split<R, Philox, true, false>), dim3(grid), dim3(block), 0, st, a);
+            else hipLaunchKernelGGL((k_history_split<R, Philox, false, false>), dim3(grid), dim3(block), 0, st, a);
         }
         return hipGetLastError();
     }

@@ -122,8 +122,8 @@ def _replay_inputs(session, nh, nb):
     return s1, s2, stream, off
 
 
-@pytest.mark.parametrize("name", ["w20", "small", "spec", "electron", "positron", "slab", "split", "split_slab", "ct",
-                                  "vacuum"])
+@pytest.mark.parametrize("name", ["w20", "small", "spec", "electron", "positron", "slab", "split", "split_slab",
+                                  "split20", "ct", "vacuum"])
 def test_history_replay_fp64_matches_oracle(tmp_path, name):
     """Plain history kernel, FP64, each history on its own slice of the
     reference's RANMAR stream: per-region batch sums agree with the oracle."""
