@@ -139,17 +139,18 @@ def test_event_split_largest_factor_fits_the_pools():
     assert f2 >= 0.85 and f3 >= 0.95, (f2, f3)
 
 
-def test_dij_split_event_matches_history_form(tmp_path):
-    """Dij with photon splitting (nsplit 4): the event form's beamlet-summed
-    dose agrees with the plain per-beamlet history form's within their
-    uncertainties."""
+@pytest.mark.parametrize("ns", [4, 12])
+def test_dij_split_event_matches_history_form(tmp_path, ns):
+    """Dij with photon splitting: the event form's beamlet-summed dose agrees
+    with the plain per-beamlet history form's within their uncertainties (at
+    nsplit 12 the history form keeps its stacks in device memory)."""
     ph_h = P.synthetic_ct(nx=20, ny=20, nz=12, vx=0.5, vy=0.5, vz=0.5, seed=3, target_radius=1.0)
     bix = P.beamlets(nbeams=2, sad=100.0, bixel=0.5, radius=1.0, seed=3, hist_median=3000.0,
                      hist_sigma_log10=0.2, hist_range=(2000, 4000))
     c = omc.Context(ph_h.media)
     ph = c.phantom(ph_h, ecut=0.7, pcut=0.01)
     spec = omc.Spectrum.mono(2.0)
-    kw = dict(nbatch=5, rel_threshold=1e-4, want_plan=True, precision=omc.FP64, nsplit=4)
+    kw = dict(nbatch=5, rel_threshold=1e-4, want_plan=True, precision=omc.FP64, nsplit=ns)
     plain = c.run_dij(ph, spec, bix, form=omc.FORM_HISTORY, group=1, seed=7, **kw)
     event = c.run_dij(ph, spec, bix, form=omc.FORM_EVENT, group=4, seed=8, **kw)
     c.close()
@@ -158,7 +159,7 @@ def test_dij_split_event_matches_history_form(tmp_path):
     pd_e, pv_e = event[2]
     sel = pd_h > 0.5 * pd_h.max()
     z = np.abs(pd_h[sel] - pd_e[sel]) / np.sqrt(pv_h[sel] + pv_e[sel])
-    print(f"Dij nsplit 4, history vs event plan dose: {sel.sum()} voxels, {np.mean(z <= 2):.3f} / "
+    print(f"Dij nsplit {ns}, history vs event plan dose: {sel.sum()} voxels, {np.mean(z <= 2):.3f} / "
           f"{np.mean(z <= 3):.3f}")
     assert sel.sum() > 20
     assert np.mean(z <= 2.0) >= 0.85 and np.mean(z <= 3.0) >= 0.97
