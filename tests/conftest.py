@@ -28,6 +28,8 @@ DOSE_CASES = {
     "split": (dict(energy=2.0, collimator=(-2, 2, -2, 2), nsplit=5), 2000, 10),
     "split_slab": (dict(spectrum="mohan6", collimator=(-3, 3, -3, 3), nsplit=3), 2000, 10),
     "split20": (dict(spectrum="mohan6", collimator=(-3, 3, -3, 3), nsplit=20), 500, 10),
+    "electron_split": (dict(energy=6.0, charge=-1, collimator=(-1, 1, -1, 1), nsplit=10), 500, 10),
+    "positron_split": (dict(energy=3.0, charge=1, collimator=(-1, 1, -1, 1), nsplit=4), 500, 10),
     "ct": (dict(spectrum="mohan6", collimator=(-2, 2, -2, 2)), 2000, 10),
     "vacuum": (dict(energy=6.0, charge=-1, collimator=(-1, 1, -1, 1)), 1000, 10),
 }

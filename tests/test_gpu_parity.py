@@ -123,7 +123,7 @@ def _replay_inputs(session, nh, nb):
 
 
 @pytest.mark.parametrize("name", ["w20", "small", "spec", "electron", "positron", "slab", "split", "split_slab",
-                                  "split20", "ct", "vacuum"])
+                                  "split20", "electron_split", "positron_split", "ct", "vacuum"])
 def test_history_replay_fp64_matches_oracle(tmp_path, name):
     """Plain history kernel, FP64, each history on its own slice of the
     reference's RANMAR stream: per-region batch sums agree with the oracle."""
