@@ -59,7 +59,7 @@ def _ctx(name):
     return c, c.phantom(ph_h, ecut=0.7, pcut=0.01)
 
 
-@pytest.mark.parametrize("name", ["split", "split_slab"])
+@pytest.mark.parametrize("name", ["split", "split_slab", "electron_split", "positron_split"])
 def test_event_split_fp64_agrees_with_oracle(tmp_path, name):
     """Event form with splitting (FP64, Philox) against the oracle with the
     same nsplit (FP64, RANMAR): per voxel, and the total energy against the
