@@ -166,6 +166,14 @@ typedef struct omc_run_stats {
     int64_t groups, group_size, nnz;
     /* event form: batches transported together in one job; Dij: concurrent streams */
     int64_t batches_per_job, streams;
+    /* energy balance (FP64, MeV, weighted): energy of the primaries (photons E,
+     * electrons E - m, positrons E + m), of photons that left the phantom (the
+     * reference discards them untallied, src/ompmc.c:2008-2017), dropped
+     * without a deposit (quirk Q6 of src/ompmc.c:5051-5056, iteration
+     * guards), and deposited in all regions including region 0.  Without
+     * photon splitting e_source = e_escape + e_lost + e_deposit up to
+     * rounding; with it, in expectation. */
+    double e_source, e_escape, e_lost, e_deposit;
 } omc_run_stats;
 
 /* Defaults: FP32, event form, Philox, seed 1802, nsplit 1, automatic launch. */
@@ -229,6 +237,24 @@ int omc_run_dij(omc_ctx *ctx, int phantom_id, const omc_spectrum_desc *spec, con
 int omc_run_dij_ex(omc_ctx *ctx, int phantom_id, const omc_spectrum_desc *spec, const omc_beamlets_desc *bix,
                    int nbatch, double rel_threshold, const omc_run_opts *opts, omc_csc *dij, omc_csc *var,
                    double *plan_dose, double *plan_var, omc_run_stats *stats);
+
+/* The scoring and assembly of omc_matrad's beamlet loop on given batch
+ * tallies, by the device kernels of omc_run_dij: accumEndep per batch
+ * (omc_matrad.c:897-918), accumulateResults with iout = 1 (:920-1012: dose in
+ * Gy per history, the variance with the carry of the previous beamlet's
+ * variance array, rho < 0.044 zeroed), the maximum, the strict threshold
+ * rel_threshold * max and the CSC append (:1603-1705).  endep holds nbixels x
+ * nbatch arrays of nvox + 1 doubles, endep[(b * nbatch + k) * (nvox + 1) + irl]
+ * = beamlet b's energy (MeV) in batch k at region irl (0 = outside), and
+ * nperbatch[b] the histories per batch it was divided by.  opts (NULL: the
+ * defaults) chooses the grouping (group, batches_per_job, streams,
+ * hbm_budget) as for omc_run_dij.  The reference has the arithmetic inside its
+ * mexFunction only; this entry point has no counterpart of its own.  It feeds
+ * the Dij kernels with exact inputs for verification and lets a caller with
+ * tallies from elsewhere reuse the assembly. */
+int omc_dij_assemble(omc_ctx *ctx, int phantom_id, int nbixels, int nbatch, const int64_t *nperbatch,
+                     const double *endep, double rel_threshold, const omc_run_opts *opts, omc_csc *dij, omc_csc *var,
+                     omc_run_stats *stats);
 
 /* Injected-random-number sampler evaluation for verification: ncase cases of
  * the golden-vector layout (kinds 1..16: compton, pair, photo, rayleigh,

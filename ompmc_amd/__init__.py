@@ -73,7 +73,12 @@ class RunStats(C.Structure):
                 ("ev_dep", C.c_uint64 * 2), ("events_photon", C.c_uint64), ("events_charged", C.c_uint64),
                 ("rounds", C.c_uint64), ("peak_inflight", C.c_uint64), ("groups", C.c_int64),
                 ("group_size", C.c_int64), ("nnz", C.c_int64), ("batches_per_job", C.c_int64),
-                ("streams", C.c_int64)]
+                ("streams", C.c_int64), ("e_source", C.c_double), ("e_escape", C.c_double),
+                ("e_lost", C.c_double), ("e_deposit", C.c_double)]
+
+    def energy_residual(self):
+        """(deposit + escape + lost - source) / source of the run"""
+        return (self.e_deposit + self.e_escape + self.e_lost - self.e_source) / self.e_source
 
     def as_dict(self):
         out = {}
@@ -161,6 +166,8 @@ def lib():
                                  C.c_double, C.POINTER(RunOpts), C.POINTER(CSC), C.POINTER(CSC), _d, _d,
                                  C.POINTER(RunStats)]
     L.omc_csc_free.argtypes = [C.POINTER(CSC)]
+    L.omc_dij_assemble.argtypes = [vp, C.c_int, C.c_int, C.c_int, _i64, _d, C.c_double, C.POINTER(RunOpts),
+                                   C.POINTER(CSC), C.POINTER(CSC), C.POINTER(RunStats)]
     L.omc_server_create.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(vp)]
     L.omc_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_int64)]
     L.omc_server_poll.argtypes = [vp, C.c_int64, _i]
@@ -325,9 +332,12 @@ class Context:
         return Phantom(self, ph, ecut, pcut)
 
     def run_dosxyz(self, phantom: Phantom, spectrum: Spectrum, nperbatch, nbatch, collimator=(-5, 5, -5, 5),
-                   ssd=1.0e10, charge=0, **opts):
+                   ssd=1.0e10, charge=0, keep_diverged_replay=False, **opts):
         """Returns (sum, sum2, stats): per-region batch sums of deposited
-        energy (MeV, region 0 = outside) and run statistics."""
+        energy (MeV, region 0 = outside) and run statistics.  A replayed run in
+        which a history needed more draws than it was given fails with
+        OMC_ERR_ARG (stats.errors bit 1); keep_diverged_replay=True returns its
+        outputs instead, for verification runs that count such histories."""
         src = DosxyzSourceDesc()
         src.spec = spectrum.desc()
         src.charge, src.ssd = charge, ssd
@@ -337,8 +347,10 @@ class Context:
         s = np.zeros(phantom.nreg)
         s2 = np.zeros(phantom.nreg)
         st = RunStats()
-        _check(lib().omc_run_dosxyz(self.h, phantom.id, C.byref(src), nperbatch, nbatch, C.byref(o), _dp(s),
-                                    _dp(s2), C.byref(st)))
+        rc = lib().omc_run_dosxyz(self.h, phantom.id, C.byref(src), nperbatch, nbatch, C.byref(o), _dp(s), _dp(s2),
+                                  C.byref(st))
+        if not (keep_diverged_replay and rc == 5 and st.errors == 2):
+            _check(rc)
         del keep
         return s, s2, st
 
@@ -396,6 +408,30 @@ class Context:
         vout = take(var) if want_var else None
         del keep
         return out, vout, ((pd, pv) if want_plan else None), st
+
+    def dij_assemble(self, phantom: Phantom, endep, nperbatch, rel_threshold=1e-4, want_var=True, **opts):
+        """omc_dij_assemble: the Dij scoring and assembly of the device kernels
+        on given batch tallies endep [nbix, nbatch, nvox + 1] (MeV, region 0
+        outside) with nperbatch [nbix] histories per batch.  Returns (dij, var,
+        stats) with dij/var as (jc, ir, val)."""
+        e = np.ascontiguousarray(endep, np.float64)
+        nb, nbatch, nreg = e.shape
+        if nreg != phantom.nreg:
+            raise ValueError(f"tallies of {nreg} regions for a phantom of {phantom.nreg}")
+        npb = np.ascontiguousarray(nperbatch, np.int64)
+        o, keep = run_opts(**opts)
+        dij, var, st = CSC(), CSC(), RunStats()
+        _check(lib().omc_dij_assemble(self.h, phantom.id, nb, nbatch, npb.ctypes.data_as(_i64), _dp(e),
+                                      float(rel_threshold), C.byref(o), C.byref(dij),
+                                      C.byref(var) if want_var else None, C.byref(st)))
+        del keep
+        out = dij.to_numpy()
+        lib().omc_csc_free(C.byref(dij))
+        vout = None
+        if want_var:
+            vout = var.to_numpy()
+            lib().omc_csc_free(C.byref(var))
+        return out, vout, st
 
     def sampler_eval(self, phantom: Phantom, precision, kinds, inputs, rn_lists, in_stride=24, out_stride=64 + 2 * 512):
         """Evaluates golden-vector cases with injected random numbers on the

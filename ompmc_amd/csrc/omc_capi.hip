@@ -571,13 +571,16 @@ static int run_dosxyz_history(omc_ctx *c, hipStream_t stream, PhantomDev *pd, co
     double *d_e = A.alloc_n<double>(nreg), *d_s = A.alloc_n<double>(nreg), *d_s2 = A.alloc_n<double>(nreg);
     unsigned long long *d_cnt = A.alloc_n<unsigned long long>(8), *d_next = A.alloc_n<unsigned long long>(1);
     int *d_err = A.alloc_n<int>(4);
+    double *d_ebal = A.alloc_n<double>(EB_N);
     uint32_t *d_rn = nullptr;
     long *d_off = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> kev;
     float ms = 0.f;
     const int hblock = o->block > 0 ? o->block : 256;
-    int max_grid = o->grid > 0 ? o->grid : 2048;
+    // the largest launch of the run (the particle stacks of nsplit > 8 cover it)
+    int max_grid = o->grid > 0 ? o->grid : (int)std::min<int64_t>((nper + hblock - 1) / hblock, 2048);
+    int err[4] = {0, 0, 0, 0};
     auto wall0 = std::chrono::steady_clock::now();
     if (!A.ok) { rc = omc_seterr(OMC_ERR_NOMEM, "device allocation failed"); goto done; }
     rc = make_dosxyz_src<R>(A, pd, src, a.dos);
@@ -595,7 +598,9 @@ static int run_dosxyz_history(omc_ctx *c, hipStream_t stream, PhantomDev *pd, co
     HIPCHK(hipMemsetAsync(d_s2, 0, sizeof(double) * nreg, stream));
     HIPCHK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long) * 8, stream));
     HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int) * 4, stream));
+    HIPCHK(hipMemsetAsync(d_ebal, 0, sizeof(double) * EB_N, stream));
     a.endep = d_e;
+    a.ebal = d_ebal;
     a.counters = d_cnt;
     a.err = d_err;
     a.next = d_next;
@@ -625,18 +630,28 @@ static int run_dosxyz_history(omc_ctx *c, hipStream_t stream, PhantomDev *pd, co
         HIPCHK(launch_history<R>(a, o->rng == OMC_RNG_REPLAY ? RNG_REPLAY : RNG_PHILOX, (o->count_events & 1) != 0,
                                  grid, block, stream));
         if (o->count_events) HIPCHK(hipEventRecord(kev[2 * b + 1], stream));
-        HIPCHK(launch_accumulate(d_e, d_s, d_s2, nreg, 1.0, stream));
+        HIPCHK(launch_accumulate(d_e, d_s, d_s2, nreg, 1.0, d_ebal + EB_DEPOSIT, stream));
     }
     HIPCHK(hipEventRecord(ev1, stream));
     HIPCHK(hipStreamSynchronize(stream));
     HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
     HIPCHK(hipMemcpy(sum, d_s, sizeof(double) * nreg, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(sum2, d_s2, sizeof(double) * nreg, hipMemcpyDeviceToHost));
+    // the device's error flags decide the result whether or not statistics
+    // were asked for: a history cut short leaves a biased dose
+    HIPCHK(hipMemcpy(err, d_err, sizeof err, hipMemcpyDeviceToHost));
+    if (err[0]) rc = omc_seterr(OMC_ERR_STACK, "particle stack overflow on the device");
+    else if (err[1]) rc = omc_seterr(OMC_ERR_ARG, "replay stream underrun");
+    else if (err[2]) rc = omc_seterr(OMC_ERR_LIMIT, "history step guard reached");
     if (st) {
         unsigned long long cnt[8];
-        int err[4];
+        double eb[EB_N];
         HIPCHK(hipMemcpy(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(err, d_err, sizeof err, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(eb, d_ebal, sizeof eb, hipMemcpyDeviceToHost));
+        st->e_source = eb[EB_SOURCE];
+        st->e_escape = eb[EB_ESCAPE];
+        st->e_lost = eb[EB_LOST];
+        st->e_deposit = eb[EB_DEPOSIT];
         st->gpu_seconds = ms * 1e-3;
         st->histories = (double)nper * nbatch;
         st->n_howfar = cnt[0]; st->n_hownear = cnt[1]; st->n_cross = cnt[2]; st->n_dep = cnt[3];
@@ -650,7 +665,6 @@ static int run_dosxyz_history(omc_ctx *c, hipStream_t stream, PhantomDev *pd, co
         }
         st->kernel_seconds[OMC_K_HISTORY] = o->count_events ? tk : ms * 1e-3;
         st->kernel_launches[OMC_K_HISTORY] = (uint64_t)nbatch;
-        if (err[0]) rc = omc_seterr(OMC_ERR_STACK, "particle stack overflow on the device");
     }
 done:
     if (ev0) (void)hipEventDestroy(ev0);

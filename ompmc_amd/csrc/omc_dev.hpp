@@ -235,14 +235,33 @@ struct TallyNone {
     __device__ __forceinline__ void flush() {}
 };
 
+// With `touched` set (the tallies of a dose-influence run), every flush also
+// marks the tile of TALLY_TILE regions that received it, so the batch
+// statistics afterwards read only tiles that hold energy.  The mark is a
+// plain byte store of 1 (idempotent, no atomic).
+constexpr int TALLY_TILE_LOG2 = 6;
+constexpr int TALLY_TILE = 1 << TALLY_TILE_LOG2;    // one wavefront of regions
+
 struct TallyAtomic {
     double *endep;       // [nreg] with region 0 = outside
+    uint8_t *touched;    // optional [ceil(nreg / TALLY_TILE)] tile marks
     int cur;
     double acc;
-    __device__ __forceinline__ void init(double *p) { endep = p; cur = -1; acc = 0.0; }
+    __device__ __forceinline__ void init(double *p, uint8_t *tf = nullptr) {
+        endep = p;
+        touched = tf;
+        cur = -1;
+        acc = 0.0;
+    }
+    __device__ __forceinline__ void put() {
+        if (cur >= 0 && acc != 0.0) {
+            atomicAdd(endep + cur, acc);
+            if (touched) touched[cur >> TALLY_TILE_LOG2] = 1;
+        }
+    }
     __device__ __forceinline__ void add(int region, double v) {
         if (region != cur) {
-            if (cur >= 0 && acc != 0.0) atomicAdd(endep + cur, acc);
+            put();
             cur = region;
             acc = v;
         } else {
@@ -250,11 +269,28 @@ struct TallyAtomic {
         }
     }
     __device__ __forceinline__ void flush() {
-        if (cur >= 0 && acc != 0.0) atomicAdd(endep + cur, acc);
+        put();
         cur = -1;
         acc = 0.0;
     }
 };
+
+// Energy balance of a run (FP64, MeV): [0] energy brought in by the
+// primaries (photons E, electrons E - m, positrons E + m: the rest mass an
+// annihilation returns), [1] energy of photons that left the phantom (the
+// reference discards them without a tally, src/ompmc.c:2008-2017), [2]
+// energy dropped without a deposit (the reference's lambda anomaly, quirk
+// Q6, and particles ended by an iteration guard), [3] energy deposited in
+// all regions including region 0 (summed by the batch statistics kernels).
+// Without photon splitting [0] = [1] + [2] + [3] up to rounding.
+enum { EB_SOURCE = 0, EB_ESCAPE = 1, EB_LOST = 2, EB_DEPOSIT = 3, EB_N = 4 };
+
+// Sum over the wavefront, added once per wave (lane 0) to *dst when non-zero.
+__device__ __forceinline__ void wave_add(double *dst, double v) {
+    if (!dst) return;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(dst, v);
+}
 
 // Per-lane event counters feeding the algorithmic-bytes roofline.
 struct Counters {
@@ -271,6 +307,15 @@ template <class R, class G, class T> struct Engine {
     bool count;
     int nsplit = 1;                 // photon splitting factor (vrt.nsplit)
     int aborted = 0;                // set when a particle exceeded an iteration guard (it is dropped)
+    double esc = 0.0;               // wt * E of photons that left the phantom (energy balance)
+    double lost = 0.0;              // energy dropped without a deposit (energy balance)
+
+    // energy a charged particle carries into the balance: kinetic, plus the
+    // two rest masses an annihilation returns for a positron
+    static __device__ __forceinline__ double charged_energy(const Part<R> &p) {
+        const double RMD = 0.5109989461;
+        return (double)p.wt * ((double)p.e + (p.iq > 0 ? RMD : -RMD));
+    }
 
     __device__ __forceinline__ R rnd() {
         if (count) cnt.nrn++;
@@ -686,6 +731,7 @@ template <class R, class G, class T> struct Engine {
             if (p.ix < 0) {           // howfar: outside the geometry -> discard
                 if (count) cnt.nhowfar++;
                 p.x += ustep * p.u; p.y += ustep * p.v; p.z += ustep * p.w;
+                esc += (double)p.wt * (double)eig;
                 return -1;
             }
             bool leaves;
@@ -794,6 +840,7 @@ template <class R, class G, class T> struct Engine {
         }
         if (!(mu_max > (R)0)) {          // nothing to interact with: leaves
             p.ix = -1;
+            esc += (double)p.wt * (double)eig;
             return -1;
         }
         const R xlo = gld(g.xb + (0)), xhi = gld(g.xb + (g.nx)), ylo = gld(g.yb + (0)), yhi = gld(g.yb + (g.ny)), zlo = gld(g.zb + (0)), zhi = gld(g.zb + (g.nz));
@@ -801,6 +848,7 @@ template <class R, class G, class T> struct Engine {
             if (iters > (1 << 24)) {     // no progress: dropped, reported through `aborted`
                 aborted = 1;
                 p.ix = -1;
+                lost += (double)p.wt * (double)eig;
                 return -1;
             }
             R rn = rnd();
@@ -811,6 +859,7 @@ template <class R, class G, class T> struct Engine {
             p.z += s * p.w;
             if (!(p.x >= xlo && p.x <= xhi && p.y >= ylo && p.y <= yhi && p.z >= zlo && p.z <= zhi)) {
                 p.ix = -1;               // left the phantom (howfar discards it)
+                esc += (double)p.wt * (double)eig;
                 return -1;
             }
             p.ix = locate(g.xb, g.nx, p.x, g.inv_dx);
@@ -909,6 +958,7 @@ template <class R, class G, class T> struct Engine {
             if (p.ix < 0) {           // howfar: outside the geometry -> discard
                 if (count) cnt.nhowfar++;
                 p.x += ustep * p.u; p.y += ustep * p.v; p.z += ustep * p.w;
+                esc += (double)p.wt * (double)p.e;
                 return false;
             }
             bool leaves;
@@ -1685,6 +1735,8 @@ template <class R, class G, class T> struct Engine {
                 }
                 if (++iters > (1 << 24)) {
                     aborted = 1;
+                    p.e = eie;
+                    lost += charged_energy(p);
                     return -1;
                 }
             }
@@ -1805,6 +1857,8 @@ template <class R, class G, class T> struct Engine {
                                 do_single = true;
                             }
                         } else {
+                            p.e = eie;
+                            lost += charged_energy(p);
                             return -1;       // reference warns and drops (quirk Q6)
                         }
                         ustep = tustep;
@@ -1898,6 +1952,7 @@ template <class R, class G, class T> struct Engine {
                 if (total_tstep < M<R>::reach_tol(tt0)) demfp = (R)0.0;
                 if (++iters > (1 << 24)) {
                     aborted = 1;
+                    lost += charged_energy(p);
                     return -1;
                 }
                 if (demfp >= (R)1.0E-5 && --budget <= 0) {

@@ -246,15 +246,20 @@ template <class R> void fill_launch(omc_ctx *c, const PhantomDev *pd, EvState<R>
 }
 
 // Runs a job, growing the pools and rerunning it from zeroed tallies
-// (nslot slots of nreg doubles at `endep`) when the queues overflow.
+// (nslot slots of nreg doubles at `endep`) when the queues overflow.  The
+// energy balance ebal[0..EB_N) (optional) is copied to ebal[EB_N..2 EB_N)
+// before the job and restored before a rerun.  Tile marks of the failed
+// attempt stay set: they only make the statistics read tiles of zeros.
 template <class R>
 int run_job_grow(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int dest, bool count, bool timing,
-                 JobStats &js, double *endep, size_t ndouble) {
+                 JobStats &js, double *endep, size_t ndouble, double *ebal) {
+    if (ebal) CK(hipMemcpyAsync(ebal + EB_N, ebal, sizeof(double) * EB_N, hipMemcpyDeviceToDevice, st));
     for (int attempt = 0;; attempt++) {
         int rc = run_job(st, S, L, nhist, dest, count, timing, js);
         if (rc != RC_POOL_OVERFLOW) return rc;
         if (attempt >= MAX_POOL_GROWTH) return OMC_ERR_STACK;
         CK(hipMemsetAsync(endep, 0, sizeof(double) * ndouble, st));
+        if (ebal) CK(hipMemcpyAsync(ebal, ebal + EB_N, sizeof(double) * EB_N, hipMemcpyDeviceToDevice, st));
         CK(hipStreamSynchronize(st));
         rc = S.grow();
         if (rc) return rc;
@@ -279,6 +284,13 @@ void stats_from_job(omc_run_stats *st, const JobStats &js) {
     st->events_charged += js.ev_charged;
     st->rounds += js.rounds;
     st->peak_inflight = std::max<uint64_t>(st->peak_inflight, js.peak);
+}
+
+void stats_energy(omc_run_stats *st, const double *eb) {
+    st->e_source += eb[EB_SOURCE];
+    st->e_escape += eb[EB_ESCAPE];
+    st->e_lost += eb[EB_LOST];
+    st->e_deposit += eb[EB_DEPOSIT];
 }
 
 void stats_counters(omc_run_stats *st, const unsigned long long *cnt) {
@@ -325,6 +337,7 @@ int omc_run_dosxyz_event(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const o
            *d_s2 = A.alloc_n<double>(nreg);
     long *d_soff = A.alloc_n<long>(nbj + 1);
     unsigned long long *d_cnt = A.alloc_n<unsigned long long>(16);
+    double *d_ebal = A.alloc_n<double>(2 * EB_N);
     if (!A.ok) return omc_seterr(OMC_ERR_NOMEM, "device allocation failed");
     std::vector<long> soff(nbj + 1);
     for (int k = 0; k <= nbj; k++) soff[k] = (long)k * nper;
@@ -337,7 +350,9 @@ int omc_run_dosxyz_event(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const o
     L.src.slot_off = d_soff;
     L.endep = d_e;
     L.slot_stride = nreg;
+    L.ebal = d_ebal;
     L.counters = o->count_events ? d_cnt : nullptr;
+    CK(hipMemsetAsync(d_ebal, 0, sizeof(double) * 2 * EB_N, stream));
     CK(hipMemsetAsync(d_e, 0, sizeof(double) * nbj * nreg, stream));
     CK(hipMemsetAsync(d_s, 0, sizeof(double) * nreg, stream));
     CK(hipMemsetAsync(d_s2, 0, sizeof(double) * nreg, stream));
@@ -351,9 +366,10 @@ int omc_run_dosxyz_event(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const o
         L.src.hist_base = (long)b0 * nper;       // streams stay keyed by the history's place in the run
         L.src.nslot = nb;
         rc = run_job_grow(stream, S, L, (long long)nper * nb, dest, (o->count_events & 1) != 0,
-                          o->count_events != 0, js, d_e, (size_t)nb * nreg);
+                          o->count_events != 0, js, d_e, (size_t)nb * nreg, d_ebal);
         if (rc) return rc;
-        for (int k = 0; k < nb; k++) CK(launch_accumulate(d_e + (size_t)k * nreg, d_s, d_s2, nreg, 1.0, stream));
+        for (int k = 0; k < nb; k++)
+            CK(launch_accumulate(d_e + (size_t)k * nreg, d_s, d_s2, nreg, 1.0, d_ebal + EB_DEPOSIT, stream));
     }
     CK(hipEventRecord(ev.ev[1], stream));
     CK(hipStreamSynchronize(stream));
@@ -361,7 +377,10 @@ int omc_run_dosxyz_event(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const o
     CK(hipMemcpy(sum2, d_s2, sizeof(double) * nreg, hipMemcpyDeviceToHost));
     if (st) {
         unsigned long long cnt[16];
+        double eb[EB_N];
         CK(hipMemcpy(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost));
+        CK(hipMemcpy(eb, d_ebal, sizeof eb, hipMemcpyDeviceToHost));
+        stats_energy(st, eb);
         st->gpu_seconds = ev.ms(0, 1) * 1e-3;
         st->histories = (double)nper * nbatch;
         st->batches_per_job = nbj;
@@ -393,13 +412,25 @@ std::vector<long> beamlet_nper(const omc_beamlets_desc *bix, int nbatch) {
 }
 
 // Device bytes of one beamlet of a group whose jobs carry nbj batches: nbj
-// batch tallies plus the running sum and sum of squares.
-size_t dij_slot_bytes(long nreg, int nbj) { return (size_t)(nbj + 2) * nreg * sizeof(double) + (size_t)64 * nbj; }
+// batch tallies with their tile marks, plus two arrays for the running sum
+// and sum of squares (or the dose and variance), the beamlet's tile map and
+// its compaction counts.
+size_t dij_slot_bytes(long nreg, int nbj) {
+    return (size_t)(nbj + 2) * nreg * sizeof(double) + (size_t)(nbj + 1) * dij_ntile(nreg) +
+           (size_t)dij_nchunk(nreg) * (sizeof(int) + sizeof(int64_t)) + (size_t)dij_nblk(nreg) * sizeof(double) +
+           (size_t)64 * nbj;
+}
 
-size_t dij_fixed_bytes(long nreg, int precision, int form, long long target) {
+// Device bytes of a Dij run besides its beamlet slots: carry and plan
+// arrays, and the event pools or, for the history form with nsplit above
+// HISTORY_SCRATCH_NSPLIT, the particle stacks in device memory.
+size_t dij_fixed_bytes(long nreg, const omc_run_opts *o, long long target) {
     size_t b = (size_t)3 * nreg * sizeof(double) + ((size_t)64 << 20);
-    if (form == OMC_FORM_EVENT)
-        b += precision == OMC_FP32 ? event_bytes<float>(target) : event_bytes<double>(target);
+    if (o->form == OMC_FORM_EVENT)
+        b += o->precision == OMC_FP32 ? event_bytes<float>(target) : event_bytes<double>(target);
+    else if (o->nsplit > HISTORY_SCRATCH_NSPLIT)
+        b += (size_t)(o->grid > 0 ? o->grid : 2048) * (o->block > 0 ? o->block : 256) * split_stack_depth(o->nsplit) *
+             (o->precision == OMC_FP32 ? sizeof(Part<float>) : sizeof(Part<double>));
     return b;
 }
 
@@ -420,14 +451,13 @@ struct DijPlan {
 size_t dij_default_budget() { return std::min(default_budget(), (size_t)96 << 30); }
 
 size_t dij_plan(const omc_run_opts *o, long nreg, int nbix, int nbatch, long long target, size_t budget, DijPlan &P) {
-    const bool event = o->form == OMC_FORM_EVENT;
     if (!budget) budget = o->hbm_budget > 0 ? (size_t)o->hbm_budget : dij_default_budget();
-    const size_t fixed = dij_fixed_bytes(nreg, o->precision, o->form, target);
+    const size_t fixed = dij_fixed_bytes(nreg, o, target);
     const size_t room = budget > fixed ? budget - fixed : 0;
     const int gmax = std::min(nbix, 4096);
     long best_jobs = -1;
-    int lo = 1, hi = event ? nbatch : 1;
-    if (event && o->batches_per_job > 0) lo = hi = std::min(o->batches_per_job, nbatch);
+    int lo = 1, hi = nbatch;
+    if (o->batches_per_job > 0) lo = hi = std::min(o->batches_per_job, nbatch);
     for (int nbj = hi; nbj >= lo; nbj--) {
         long g = o->group > 0 ? std::min(o->group, nbix) : (long)(room / dij_slot_bytes(nreg, nbj));
         g = std::max(1L, std::min<long>(g, gmax));
@@ -480,11 +510,15 @@ template <class R> struct DijLane {
     EvLaunch<R> L;
     HistArgs<R> H;
     double *d_e = nullptr, *d_s = nullptr, *d_s2 = nullptr, *d_bmax = nullptr, *d_thresh = nullptr, *d_nper = nullptr;
+    double *d_ebal = nullptr;          // [2 * EB_N]: the lane's energy balance and its copy before a job
+    uint8_t *d_touch = nullptr;        // [nslot][ntile] tile marks of the batch slots (fused finish)
+    uint8_t *d_any = nullptr;          // [G][ntile] tiles a beamlet received energy in
     int *d_counts = nullptr, *d_slot_bix = nullptr, *d_slot_bt = nullptr;
     int64_t *d_off = nullptr;
     long *d_slot_off = nullptr;
     hipEvent_t fin = nullptr;          // finish of the lane's latest group
     std::vector<DevArena> gouts;       // csc_on_device: the compacted columns of this lane's groups
+    std::vector<uint8_t> hmark;        // given tallies: tile marks computed on the host
     JobStats js;
     double t_hist = 0, t_other = 0, histories = 0;
     uint64_t n_hist = 0, n_other = 0;
@@ -507,13 +541,20 @@ struct DijGroupOut {
     double *d_val = nullptr, *d_var = nullptr;
 };
 
+// Batch tallies handed in by the caller (omc_dij_assemble) instead of
+// transported: endep[(b * nbatch + k) * nreg + irl] for beamlet b, batch k.
+struct DijGiven {
+    const double *endep;
+};
+
 template <class R> struct DijRun {
     omc_ctx *c;
     PhantomDev *pd;
     const omc_run_opts *o;
+    const DijGiven *given;
     int nbix, nbatch, G, NBJ, nblk, nchunk, ngroups;
-    long nreg, nvox;
-    bool event, count, timing, replay, want_var, on_device;
+    long nreg, nvox, ntile;
+    bool event, count, timing, replay, want_var, on_device, fused;
     double rel;
     long long tgt;
     std::vector<long> nper, gfirst;
@@ -543,6 +584,9 @@ template <class R> int dij_lane_init(DijRun<R> &D, DijLane<R> &l) {
     l.d_bmax = A.alloc_n<double>((size_t)G * D.nblk);
     l.d_thresh = A.alloc_n<double>(G);
     l.d_nper = A.alloc_n<double>(G);
+    l.d_ebal = A.alloc_n<double>(2 * EB_N);
+    l.d_any = A.alloc_n<uint8_t>((size_t)G * D.ntile);
+    if (D.fused) l.d_touch = A.alloc_n<uint8_t>(nslot * D.ntile);
     l.d_counts = A.alloc_n<int>((size_t)G * D.nchunk);
     l.d_off = A.alloc_n<int64_t>((size_t)G * D.nchunk);
     l.d_slot_off = A.alloc_n<long>(nslot + 1);
@@ -552,7 +596,11 @@ template <class R> int dij_lane_init(DijRun<R> &D, DijLane<R> &l) {
         return omc_seterr(OMC_ERR_NOMEM, "Dij tallies for a group of %d beamlets x %d batches do not fit", G, D.NBJ);
     CK(hipEventCreateWithFlags(&l.fin, hipEventDisableTiming));
     CK(hipMemsetAsync(l.d_e, 0, sizeof(double) * nslot * D.nreg, l.st));
+    CK(hipMemsetAsync(l.d_ebal, 0, sizeof(double) * 2 * EB_N, l.st));
+    if (D.fused) CK(hipMemsetAsync(l.d_touch, 0, nslot * D.ntile, l.st));
+    else CK(hipMemsetAsync(l.d_any, 1, (size_t)G * D.ntile, l.st));     // every tile counts as touched
     const omc_run_opts *o = D.o;
+    if (D.given) return OMC_OK;
     if (D.event) {
         long long maxjob = 0;
         for (int g0 = 0; g0 < D.nbix; g0 += G) {
@@ -572,6 +620,9 @@ template <class R> int dij_lane_init(DijRun<R> &D, DijLane<R> &l) {
         l.L.src.slot_batch = l.d_slot_bt;
         l.L.endep = l.d_e;
         l.L.slot_stride = D.nreg;
+        l.L.touched = l.d_touch;
+        l.L.touch_stride = D.ntile;
+        l.L.ebal = l.d_ebal;
         l.L.counters = D.count ? D.d_cnt : nullptr;
     } else {
         HistArgs<R> &H = l.H;
@@ -583,6 +634,8 @@ template <class R> int dij_lane_init(DijRun<R> &D, DijLane<R> &l) {
         H.slot_off = l.d_slot_off;
         H.slot_beamlet = l.d_slot_bix;
         H.slot_stride = D.nreg;
+        H.touch_stride = D.ntile;
+        H.ebal = l.d_ebal;
         H.seed = o->seed;
         H.endep = l.d_e;
         H.counters = D.count ? D.d_cnt : nullptr;
@@ -600,12 +653,33 @@ template <class R> int dij_lane_init(DijRun<R> &D, DijLane<R> &l) {
     return OMC_OK;
 }
 
-// Transport, batch statistics, finish (in group order) and compaction of
-// group gi on lane l; the group's columns land in D.outs[gi].
+// Given tallies (omc_dij_assemble): batches bt .. bt+nb-1 of the group's
+// beamlets into their slots, with the tile marks the transport would set.
+template <class R> int dij_given_job(DijRun<R> &D, DijLane<R> &l, int g0, int ng, int bt, int nb) {
+    const hipStream_t stream = l.st;
+    const long nreg = D.nreg, ntile = D.ntile;
+    if (D.fused) l.hmark.assign((size_t)nb * ng * ntile, 0);
+    for (int k = 0; k < nb; k++)
+        for (int sl = 0; sl < ng; sl++) {
+            const double *src = D.given->endep + ((size_t)(g0 + sl) * D.nbatch + bt + k) * nreg;
+            const size_t slot = (size_t)k * ng + sl;
+            CK(hipMemcpyAsync(l.d_e + slot * nreg, src, sizeof(double) * nreg, hipMemcpyHostToDevice, stream));
+            if (D.fused)
+                for (long r = 0; r < nreg; r++)
+                    if (src[r] != 0.0) l.hmark[slot * ntile + (r >> TALLY_TILE_LOG2)] = 1;
+        }
+    if (D.fused) CK(hipMemcpyAsync(l.d_touch, l.hmark.data(), l.hmark.size(), hipMemcpyHostToDevice, stream));
+    CK(hipStreamSynchronize(stream));    // the host marks may go
+    return OMC_OK;
+}
+
+// Transport (or given tallies), batch statistics, finish (in group order)
+// and compaction of group gi on lane l; the group's columns land in
+// D.outs[gi].
 template <class R> int dij_lane_group(DijRun<R> &D, DijLane<R> &l, int gi) {
     const omc_run_opts *o = D.o;
     const hipStream_t stream = l.st;
-    const long nreg = D.nreg, nvox = D.nvox;
+    const long nreg = D.nreg;
     const int nchunk = D.nchunk, g0 = gi * D.G, ng = std::min(D.G, D.nbix - g0);
     std::vector<long> soff(ng + 1, 0);
     std::vector<int> sbix(ng);
@@ -616,17 +690,22 @@ template <class R> int dij_lane_group(DijRun<R> &D, DijLane<R> &l, int gi) {
         soff[s + 1] = soff[s] + D.nper[g0 + s];
     }
     const long njob = soff[ng];
-    if (!D.event) {
+    if (!D.event && !D.given) {
         CK(hipMemcpyAsync(l.d_slot_off, soff.data(), sizeof(long) * (ng + 1), hipMemcpyHostToDevice, stream));
         CK(hipMemcpyAsync(l.d_slot_bix, sbix.data(), sizeof(int) * ng, hipMemcpyHostToDevice, stream));
     }
     CK(hipMemcpyAsync(l.d_nper, snper.data(), sizeof(double) * ng, hipMemcpyHostToDevice, stream));
-    CK(hipMemsetAsync(l.d_s, 0, sizeof(double) * ng * nreg, stream));
-    CK(hipMemsetAsync(l.d_s2, 0, sizeof(double) * ng * nreg, stream));
+    if (!D.fused) {
+        CK(hipMemsetAsync(l.d_s, 0, sizeof(double) * ng * nreg, stream));
+        CK(hipMemsetAsync(l.d_s2, 0, sizeof(double) * ng * nreg, stream));
+    }
     Events ev(2);
     for (int bt = 0; bt < D.nbatch; bt += D.NBJ) {
         const int nb = std::min(D.NBJ, D.nbatch - bt);
-        if (D.event) {
+        if (D.given) {
+            int rc = dij_given_job(D, l, g0, ng, bt, nb);
+            if (rc) return rc;
+        } else if (D.event) {
             // one job for batches bt .. bt+nb-1 of the group: slot k * ng + s
             // is beamlet g0 + s in batch bt + k, with that batch's stream keys
             std::vector<long> voff((size_t)nb * ng + 1, 0);
@@ -644,45 +723,52 @@ template <class R> int dij_lane_group(DijRun<R> &D, DijLane<R> &l, int gi) {
             l.L.src.nslot = nb * ng;
             l.L.src.batch = bt;
             int rc = run_job_grow(stream, l.S, l.L, voff.back(), 0, D.count, D.timing, l.js, l.d_e,
-                                  (size_t)nb * ng * nreg);
+                                  (size_t)nb * ng * nreg, l.d_ebal);
             if (rc) return rc;     // run_job synchronised the stream: the host vectors may go
         } else {
+            // one launch per batch; batch k of the job tallies into slots k * ng + s
             HistArgs<R> &H = l.H;
-            H.nslot = ng;
-            H.batch = bt;
-            H.nhist = njob;
-            H.hist_base = 0;
-            if (D.replay) {
-                std::vector<long> rs(njob), re(njob);
-                for (int sl = 0; sl < ng; sl++)
-                    for (long i = 0; i < D.nper[g0 + sl]; i++) {
-                        long gh = D.gfirst[g0 + sl] + (long)bt * D.nper[g0 + sl] + i;
-                        rs[soff[sl] + i] = (long)o->replay_offsets[gh];
-                        re[soff[sl] + i] = (long)o->replay_offsets[gh + 1];
-                    }
-                CK(hipMemcpyAsync(D.d_rstart, rs.data(), sizeof(long) * njob, hipMemcpyHostToDevice, stream));
-                CK(hipMemcpyAsync(D.d_rend, re.data(), sizeof(long) * njob, hipMemcpyHostToDevice, stream));
-                CK(hipStreamSynchronize(stream));   // rs/re go out of scope
+            for (int k = 0; k < nb; k++) {
+                const int b = bt + k;
+                H.nslot = ng;
+                H.batch = b;
+                H.nhist = njob;
+                H.hist_base = 0;
+                H.endep = l.d_e + (size_t)k * ng * nreg;
+                H.touched = D.fused ? l.d_touch + (size_t)k * ng * D.ntile : nullptr;
+                if (D.replay) {
+                    std::vector<long> rs(njob), re(njob);
+                    for (int sl = 0; sl < ng; sl++)
+                        for (long i = 0; i < D.nper[g0 + sl]; i++) {
+                            long gh = D.gfirst[g0 + sl] + (long)b * D.nper[g0 + sl] + i;
+                            rs[soff[sl] + i] = (long)o->replay_offsets[gh];
+                            re[soff[sl] + i] = (long)o->replay_offsets[gh + 1];
+                        }
+                    CK(hipMemcpyAsync(D.d_rstart, rs.data(), sizeof(long) * njob, hipMemcpyHostToDevice, stream));
+                    CK(hipMemcpyAsync(D.d_rend, re.data(), sizeof(long) * njob, hipMemcpyHostToDevice, stream));
+                    CK(hipStreamSynchronize(stream));   // rs/re go out of scope
+                }
+                CK(hipMemsetAsync(D.d_next, 0, sizeof(unsigned long long), stream));
+                int block = o->block > 0 ? o->block : 256;
+                long want = (njob + block - 1) / block;
+                int grid = o->grid > 0 ? o->grid : (int)std::min<long>(want, 2048);
+                if (grid > l.max_grid) grid = l.max_grid;
+                if (D.timing) CK(hipEventRecord(ev.ev[0], stream));
+                CK(launch_history<R>(H, D.replay ? RNG_REPLAY : RNG_PHILOX, D.count, grid, block, stream));
+                if (D.timing) {
+                    CK(hipEventRecord(ev.ev[1], stream));
+                    CK(hipEventSynchronize(ev.ev[1]));
+                    l.t_hist += ev.ms(0, 1) * 1e-3;
+                }
+                l.n_hist++;
             }
-            CK(hipMemsetAsync(D.d_next, 0, sizeof(unsigned long long), stream));
-            int block = o->block > 0 ? o->block : 256;
-            long want = (njob + block - 1) / block;
-            int grid = o->grid > 0 ? o->grid : (int)std::min<long>(want, 2048);
-            if (grid > l.max_grid) grid = l.max_grid;
-            if (D.timing) CK(hipEventRecord(ev.ev[0], stream));
-            CK(launch_history<R>(H, D.replay ? RNG_REPLAY : RNG_PHILOX, D.count, grid, block, stream));
-            if (D.timing) {
-                CK(hipEventRecord(ev.ev[1], stream));
-                CK(hipEventSynchronize(ev.ev[1]));
-                l.t_hist += ev.ms(0, 1) * 1e-3;
-            }
-            l.n_hist++;
         }
+        l.histories += (double)njob * nb;
         // batch statistics in batch order, as batch-by-batch jobs take them
-        for (int k = 0; k < nb; k++) {
-            CK(launch_accum_slots(l.d_e + (size_t)k * ng * nreg, l.d_s, l.d_s2, nreg, nreg, ng, l.d_nper, stream));
-            l.histories += (double)njob;
-        }
+        if (!D.fused)
+            for (int k = 0; k < nb; k++)
+                CK(launch_accum_slots(l.d_e + (size_t)k * ng * nreg, l.d_s, l.d_s2, nreg, nreg, ng, l.d_nper,
+                                      l.d_ebal + EB_DEPOSIT, stream));
     }
     if (D.timing) CK(hipEventRecord(ev.ev[0], stream));
     {
@@ -691,16 +777,20 @@ template <class R> int dij_lane_group(DijRun<R> &D, DijLane<R> &l, int gi) {
         if (D.failed) return RC_LANE_STOPPED;
     }
     if (gi > 0) CK(hipStreamWaitEvent(stream, D.lanes[(gi - 1) % D.lanes.size()]->fin, 0));
-    int nb = 0;
-    CK(launch_dij_finish(l.d_s, l.d_s2, D.d_carry, ng, nreg, nvox, D.pd->dmass, D.pd->drho, D.nbatch, l.d_bmax,
-                         D.d_tdose, D.d_tvar, &nb, stream));
+    if (D.fused)
+        CK(launch_dij_fold_finish(l.d_e, l.d_touch, ng, D.nbatch, nreg, l.d_nper, l.d_s, l.d_s2, l.d_any, D.d_carry,
+                                  D.pd->dmass, D.pd->drho, l.d_bmax, D.d_tdose, D.d_tvar, l.d_ebal + EB_DEPOSIT,
+                                  stream));
+    else
+        CK(launch_dij_finish(l.d_s, l.d_s2, D.d_carry, ng, nreg, D.pd->dmass, D.pd->drho, D.nbatch, l.d_bmax,
+                             D.d_tdose, D.d_tvar, stream));
     CK(hipEventRecord(l.fin, stream));
     {
         std::lock_guard<std::mutex> lk(D.mu);
         D.next_finish = gi + 1;
         D.cv.notify_all();
     }
-    CK(launch_dij_select(l.d_s, nreg, nvox, ng, l.d_bmax, nb, D.rel, l.d_thresh, l.d_counts, stream));
+    CK(launch_dij_select(l.d_s, l.d_any, nreg, ng, l.d_bmax, D.rel, l.d_thresh, l.d_counts, stream));
     std::vector<int> counts((size_t)ng * nchunk);
     std::vector<int64_t> off((size_t)ng * nchunk);
     CK(hipMemcpyAsync(counts.data(), l.d_counts, sizeof(int) * ng * nchunk, hipMemcpyDeviceToHost, stream));
@@ -730,8 +820,8 @@ template <class R> int dij_lane_group(DijRun<R> &D, DijLane<R> &l, int gi) {
         if (!GA.ok) return omc_seterr(OMC_ERR_NOMEM, "Dij device output (%lld entries)", (long long)tot);
         if (tot > 0) {
             CK(hipMemcpyAsync(l.d_off, off.data(), sizeof(int64_t) * ng * nchunk, hipMemcpyHostToDevice, stream));
-            CK(launch_dij_compact(l.d_s, l.d_s2, nreg, nvox, ng, l.d_thresh, l.d_off, out.d_ir, out.d_val, out.d_var,
-                                  stream));
+            CK(launch_dij_compact(l.d_s, l.d_s2, l.d_any, nreg, ng, l.d_thresh, l.d_off, out.d_ir, out.d_val,
+                                  out.d_var, stream));
         }
     }
     if (D.timing) {
@@ -767,12 +857,13 @@ template <class R> void dij_lane_run(DijRun<R> &D, DijLane<R> &l, int first, int
 template <class R>
 int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_desc *spec, const omc_beamlets_desc *bix,
             int nbatch, double rel, const omc_run_opts *o, omc_csc *dij, omc_csc *var, double *plan_dose,
-            double *plan_var, omc_run_stats *st) {
+            double *plan_var, omc_run_stats *st, const DijGiven *given = nullptr) {
     auto wall0 = std::chrono::steady_clock::now();
     DijRun<R> D;
     D.c = c;
     D.pd = pd;
     D.o = o;
+    D.given = given;
     D.nbix = bix->nbixels;
     D.nbatch = nbatch;
     D.nreg = pd->nreg;
@@ -801,15 +892,19 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
     if (D.event) NL = o->streams > 0 ? o->streams : std::min(3, (nbix + plan.G - 1) / plan.G);
     if (NL > 1) dij_plan(o, nreg, nbix, nbatch, D.tgt, budget / NL, plan);
     D.G = plan.G;
-    D.NBJ = D.event ? plan.nbj : 1;
+    D.NBJ = plan.nbj;
+    // every batch of a group in one job: the batch statistics fold into the
+    // finish, over the tiles the transport touched
+    D.fused = D.NBJ == nbatch;
     D.ngroups = (nbix + D.G - 1) / D.G;
     NL = std::min(NL, D.ngroups);
-    D.nblk = (int)((nvox + DIJ_BLOCK - 1) / DIJ_BLOCK);
-    D.nchunk = (int)((nvox + DIJ_CHUNK - 1) / DIJ_CHUNK);
+    D.ntile = dij_ntile(nreg);
+    D.nblk = dij_nblk(nreg);
+    D.nchunk = dij_nchunk(nreg);
     DevArena A;
     ArenaGuard ag{A};
     A.use_cache(stream);
-    int rc = upload_beamlets<R>(A, spec, bix, D.bsrc);
+    int rc = given ? OMC_OK : upload_beamlets<R>(A, spec, bix, D.bsrc);
     if (rc) return rc;
     D.d_carry = A.alloc_n<double>(nreg);
     D.d_tdose = A.alloc_n<double>(nreg);
@@ -966,6 +1061,9 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
         stats_counters(st, cnt);
         double histories = 0;
         for (auto &l : lanes) {
+            double eb[EB_N];
+            CK(hipMemcpy(eb, l->d_ebal, sizeof eb, hipMemcpyDeviceToHost));
+            stats_energy(st, eb);
             histories += l->histories;
             if (D.event) stats_from_job(st, l->js);
             st->kernel_seconds[OMC_K_HISTORY] += l->t_hist;
@@ -1007,6 +1105,48 @@ int omc_dij_run(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectr
     return o->precision == OMC_FP32
                ? run_dij<float>(c, stream, pd, spec, bix, nbatch, rel, o, dij, var, plan_dose, plan_var, st)
                : run_dij<double>(c, stream, pd, spec, bix, nbatch, rel, o, dij, var, plan_dose, plan_var, st);
+}
+
+extern "C" int omc_dij_assemble(omc_ctx *c, int phantom_id, int nbixels, int nbatch, const int64_t *nperbatch,
+                                const double *endep, double rel_threshold, const omc_run_opts *opts, omc_csc *dij,
+                                omc_csc *var, omc_run_stats *stats) {
+    PhantomDev *pd = c ? omc_get_phantom(c, phantom_id) : nullptr;
+    if (!pd) return omc_seterr(OMC_ERR_ARG, "no phantom %d", phantom_id);
+    if (nbixels < 1 || nbatch < 1 || nbatch > 255 || !nperbatch || !endep || (!dij && !var))
+        return omc_seterr(OMC_ERR_ARG, "omc_dij_assemble: bad arguments");
+    omc_run_opts o;
+    omc_run_opts_default(&o);
+    if (opts) o = *opts;
+    o.rng = OMC_RNG_PHILOX;
+    o.count_events = 0;
+    int rc = omc_check_opts(&o);
+    if (rc) return rc;
+    // the histories per batch as a beamlet set: the plan and the finish read
+    // nothing else of it
+    std::vector<int64_t> nh(nbixels);
+    for (int b = 0; b < nbixels; b++) {
+        if (nperbatch[b] < 1) return omc_seterr(OMC_ERR_ARG, "omc_dij_assemble: beamlet %d has %lld histories per batch",
+                                                b, (long long)nperbatch[b]);
+        nh[b] = nperbatch[b] * nbatch;
+    }
+    omc_beamlets_desc bix;
+    memset(&bix, 0, sizeof bix);
+    bix.nbixels = nbixels;
+    bix.nbeams = 1;
+    bix.nhist = nh.data();
+    omc_spectrum_desc spec;
+    memset(&spec, 0, sizeof spec);
+    (void)hipSetDevice(c->device);
+    if (stats) memset(stats, 0, sizeof *stats);
+    if (dij) memset(dij, 0, sizeof *dij);
+    if (var) memset(var, 0, sizeof *var);
+    DijGiven given = {endep};
+    omc_csc scratch;
+    memset(&scratch, 0, sizeof scratch);
+    rc = run_dij<float>(c, c->stream, pd, &spec, &bix, nbatch, rel_threshold, &o, dij ? dij : &scratch, var, nullptr,
+                        nullptr, stats, &given);
+    if (!dij) omc_csc_free(&scratch);
+    return rc;
 }
 
 extern "C" int omc_run_dij_ex(omc_ctx *c, int phantom_id, const omc_spectrum_desc *spec, const omc_beamlets_desc *bix,

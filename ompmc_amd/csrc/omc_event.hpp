@@ -88,6 +88,9 @@ template <class R> struct EvArgs {
     long long cap;                    // capacity of each pool
     double *endep;                    // tally base
     long slot_stride;                 // tally stride per slot
+    uint8_t *touched;                 // optional tile marks of the tallies (TallyAtomic)
+    long touch_stride;                // tile marks per slot
+    double *ebal;                     // optional [EB_N] energy balance (escape, lost)
     unsigned long long *counters;     // [8] event counters (optional)
     int *err;                         // EvCtl::err
     int budget;                       // electron ustep iterations between refills of a wave
@@ -194,6 +197,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PHOTON
 k_event(EvArgs<R> a) {
     const long long n = (long long)*a.n_in;
     Counters tot = {0, 0, 0, 0, 0, 0, 0};
+    double eb_esc = 0.0;             // photons that left the phantom (photon kernel only)
     bool have = false;
     Part<R> p, q;
     uint64_t stream = 0;
@@ -221,7 +225,8 @@ k_event(EvArgs<R> a) {
             age = phase >> 8;
             phase &= 0xff;
             rng.at(a.seed, stream, a.in.ctr[idx]);
-            tally.init(a.endep + (size_t)slot * a.slot_stride);
+            tally.init(a.endep + (size_t)slot * a.slot_stride,
+                       a.touched ? a.touched + (size_t)slot * a.touch_stride : nullptr);
             have = true;
         }
         if (!__any(have)) break;
@@ -260,6 +265,9 @@ k_event(EvArgs<R> a) {
                 }
             }
             if (done) tally.flush();
+            if (PHOTON) eb_esc += e.esc;
+            // dropped energy is rare (quirk Q6, guards): added where it occurs
+            if (e.lost != 0.0 && a.ebal) atomicAdd(a.ebal + EB_LOST, e.lost);
             if (COUNT) {
                 tot.nhowfar += e.cnt.nhowfar; tot.nhownear += e.cnt.nhownear; tot.ncross += e.cnt.ncross;
                 tot.ndep += e.cnt.ndep; tot.nrn += e.cnt.nrn; tot.nphoton += e.cnt.nphoton;
@@ -270,6 +278,7 @@ k_event(EvArgs<R> a) {
         emit(a, have && d == 1, q, (R)0, (R)0, 0, child_stream(stream, qpos), 0u, slot);
         if (done) have = false;
     }
+    if (PHOTON && a.ebal) wave_add(a.ebal + EB_ESCAPE, eb_esc);
     flush_counters<COUNT>(a.counters, tot);
 }
 
@@ -297,6 +306,7 @@ k_event_split(EvArgs<R> a) {
     const long long n = (long long)*a.n_in;
     const int ns = a.nsplit;
     Counters tot = {0, 0, 0, 0, 0, 0, 0};
+    double eb_esc = 0.0, eb_lost = 0.0;
     for (;;) {
         // every lane is free here: the wave claims up to 64 photons
         long long idx = -1;
@@ -315,7 +325,8 @@ k_event_split(EvArgs<R> a) {
             stream = a.in.stream[idx];
             slot = a.in.slot[idx];
             rng.at(a.seed, stream, a.in.ctr[idx]);
-            tally.init(a.endep + (size_t)slot * a.slot_stride);
+            tally.init(a.endep + (size_t)slot * a.slot_stride,
+                       a.touched ? a.touched + (size_t)slot * a.touch_stride : nullptr);
         }
         Engine<R, PhiloxAt, TallyAtomic> e{*a.t, *a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
         e.nsplit = ns;
@@ -406,9 +417,12 @@ k_event_split(EvArgs<R> a) {
         }
         if (have && e.aborted) {
             atomicOr(a.err, 2);
+            if (surv) e.lost += (double)p.wt * (double)p.e;
             surv = false;
         }
         tally.flush();
+        eb_esc += e.esc;
+        eb_lost += e.lost;
         // the survivor continues from its place in the photon's stream
         emit(a, have && surv, p, (R)0, (R)0, 0, stream, rng.n, slot);
         if (COUNT && have) {
@@ -416,6 +430,10 @@ k_event_split(EvArgs<R> a) {
             tot.ndep += e.cnt.ndep; tot.nrn += e.cnt.nrn; tot.nphoton += e.cnt.nphoton;
             tot.nelectron += e.cnt.nelectron;
         }
+    }
+    if (a.ebal) {
+        wave_add(a.ebal + EB_ESCAPE, eb_esc);
+        wave_add(a.ebal + EB_LOST, eb_lost);
     }
     flush_counters<COUNT>(a.counters, tot);
 }
@@ -437,11 +455,13 @@ template <class R> struct SrcArgs {
     uint64_t seed;
     const EvCtl *ctl;
     Pool<R> out;                 // destination pool (photons or charged, by source charge)
+    double *ebal;                // optional [EB_N] energy balance: the primaries' energy
 };
 
 template <class R>
 __global__ void __launch_bounds__(256) k_source(SrcArgs<R> a) {
     const long n = (long)a.ctl->inj_n, h0 = (long)a.ctl->inj_h0, at = (long)a.ctl->inj_at;
+    double eb_src = 0.0;
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
         long h = h0 + k;
         int slot = 0, beamlet = 0;
@@ -459,7 +479,9 @@ __global__ void __launch_bounds__(256) k_source(SrcArgs<R> a) {
         if (a.use_beamlets) init_history(e, a.bix, beamlet, p);
         else init_history(e, a.dos, p);
         store(a.out, at + k, p, (R)0, (R)0, 0, stream, rng.n, slot);
+        eb_src += source_energy(p);
     }
+    if (a.ebal) wave_add(a.ebal + EB_SOURCE, eb_src);
 }
 
 // Retires the round that read the pools of parity `par` and plans the

@@ -1360,35 +1360,6 @@ int omc_write_3ddose(const char *path, const omc_phantom *ph, const double *dose
     return OMC_OK;
 }
 
-/* reference ucodes/omc_matrad/omc_matrad.c:920-1012 (iout = 1) */
-void omc_matrad_finish(const omc_phantom *ph, const double *sum, const double *sum2, int nbatch,
-                       double *dose, double *var) {
-    int nx = ph->nx, ny = ph->ny, nz = ph->nz;
-    for (int iz = 0; iz < nz; iz++)
-        for (int iy = 0; iy < ny; iy++)
-            for (int ix = 0; ix < nx; ix++) {
-                long irl = 1 + ix + (long)iy * nx + (long)iz * nx * ny;
-                double e = sum[irl], e2 = sum2[irl], u;
-                double mass = (ph->xb[ix + 1] - ph->xb[ix]) * (ph->yb[iy + 1] - ph->yb[iy]) * (ph->zb[iz + 1] - ph->zb[iz]);
-                mass *= ph->rho[irl - 1];
-                double factor = 1.602E-10 / (mass);
-                e *= factor;
-                e2 *= factor * factor;
-                e /= (double)nbatch;
-                e2 /= (double)(nbatch - 1);
-                if (e != 0.0) {
-                    u = e2 - e * e;
-                    u /= nbatch;
-                } else {
-                    e = 0.0;
-                    u = 0.0;
-                }
-                if (ph->rho[irl - 1] < 0.044) { e = 0.0; u = 0.0; }
-                dose[irl - 1] = e;
-                var[irl - 1] = u;
-            }
-}
-
 /* ------------------------------------------------------------------ RANMAR */
 /* reference src/omc_random.c:63-150 */
 void omc_ranmar_init(omc_ranmar *r, int ixx, int jxx) {

@@ -208,15 +208,6 @@ void omc_dosxyz_finish(const omc_phantom *ph, const double *sum, const double *s
 /* .3ddose writer (omc_dosxyz.c:841-879). */
 int omc_write_3ddose(const char *path, const omc_phantom *ph, const double *dose, const double *unc);
 
-/* matRad beamlet scoring (omc_matrad.c:897-1012): sum/sum2 [nreg] hold the
- * per-batch means (already divided by nperbatch) and their squares, summed in
- * batch order.  The reference never clears accum_endep2 between beamlets, so
- * sum2 must START from the previous beamlet's variance output (zeros for the
- * first beamlet) to reproduce it.  Writes dose and the absolute variance of
- * the mean [nvox]. */
-void omc_matrad_finish(const omc_phantom *ph, const double *sum, const double *sum2,
-                       int nbatch, double *dose, double *var);
-
 /* ---------------------------------------------------------------- RANMAR */
 /* Marsaglia-Zaman RANMAR as the reference implements it
  * (src/omc_random.c:63-192): 24-bit integers, refilled 128 at a time. */

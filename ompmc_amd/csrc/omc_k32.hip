@@ -3,16 +3,19 @@
 
 namespace omc {
 
-__global__ void k_accumulate(double *e, double *sum, double *sum2, long n, double scale) {
+__global__ void k_accumulate(double *e, double *sum, double *sum2, long n, double scale, double *dep) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long stride = (long)gridDim.x * blockDim.x;
+    double raw = 0.0;
     for (; i < n; i += stride) {
         double v = e[i];
+        raw += v;
         if (scale != 1.0) v /= scale;
         sum[i] += v;
         sum2[i] += v * v;
         e[i] = 0.0;
     }
+    wave_add(dep, raw);
 }
 
 // FP64 atomic throughput probe: every lane adds 1.0 at `nper` addresses of a
@@ -37,11 +40,12 @@ hipError_t launch_atomic_probe(double *t, unsigned long long mask, int grid, int
     return hipGetLastError();
 }
 
-hipError_t launch_accumulate(double *e, double *sum, double *sum2, long n, double scale, hipStream_t st) {
+hipError_t launch_accumulate(double *e, double *sum, double *sum2, long n, double scale, double *dep,
+                             hipStream_t st) {
     int block = 256;
     long g = (n + block - 1) / block;
     int grid = (int)(g > 4096 ? 4096 : (g < 1 ? 1 : g));
-    hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(block), 0, st, e, sum, sum2, n, scale);
+    hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(block), 0, st, e, sum, sum2, n, scale, dep);
     return hipGetLastError();
 }
 

@@ -157,6 +157,9 @@ template <class R> struct HistArgs {
     const long *replay_off;      // [nhist+1] first draw of each history
     const long *replay_end;      // optional [nhist] end of each history's draws (else replay_off[h+1])
     double *endep;               // [nreg] FP64 tally, region 0 outside
+    uint8_t *touched;            // optional tile marks of the tallies (TallyAtomic), touch_stride per slot
+    long touch_stride;
+    double *ebal;                // optional [EB_N] energy balance (source, escape, lost)
     unsigned long long *counters;// [8] event counters (optional)
     int *err;                    // [4] 0 overflow, 1 replay underrun, 2 step guard
     unsigned long long *next;    // dynamic work counter (zeroed by the host)
@@ -208,10 +211,17 @@ template <> __device__ __forceinline__ void make_rng<double, Replay>(Replay &g, 
 // Plain history-by-history form: one lane runs whole showers, LIFO order,
 // one random stream per history.  Lanes fetch chunks of histories from a
 // global counter so long showers do not idle the rest of the grid.
+// energy a primary brings into the balance (EB_SOURCE)
+template <class R> __device__ __forceinline__ double source_energy(const Part<R> &p) {
+    const double RMD = 0.5109989461;
+    return (double)p.wt * (p.iq == 0 ? (double)p.e : (double)p.e + (p.iq > 0 ? RMD : -RMD));
+}
+
 template <class R, class G, bool COUNT>
 __global__ void __launch_bounds__(256) k_history(HistArgs<R> a) {
     Part<R> deep[HIST_STACK];
     Counters tot = {0, 0, 0, 0, 0, 0, 0};
+    double eb_src = 0.0, eb_esc = 0.0, eb_lost = 0.0;
     for (;;) {
         long h0 = 0;
         if (threadIdx.x % warpSize == 0) h0 = (long)atomicAdd(a.next, (unsigned long long)a.chunk * warpSize);
@@ -228,11 +238,13 @@ __global__ void __launch_bounds__(256) k_history(HistArgs<R> a) {
             G rng;
             make_rng<R, G>(rng, a, h, stream);
             TallyAtomic tally;
-            tally.init(a.endep + (size_t)slot * a.slot_stride);
+            tally.init(a.endep + (size_t)slot * a.slot_stride,
+                       a.touched ? a.touched + (size_t)slot * a.touch_stride : nullptr);
             Engine<R, G, TallyAtomic> e{*a.t, *a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
             Part<R> p, q;
             if (a.use_beamlets) init_history(e, a.bix, beamlet, p);
             else init_history(e, a.dos, p);
+            eb_src += source_energy(p);
             int depth = 0;
             long guard = 0;
             for (;;) {
@@ -249,12 +261,19 @@ __global__ void __launch_bounds__(256) k_history(HistArgs<R> a) {
                 if (++guard > 100000000L) { atomicOr(a.err + 2, 1); break; }
             }
             tally.flush();
+            eb_esc += e.esc;
+            eb_lost += e.lost;
             if (COUNT) {
                 tot.nhowfar += e.cnt.nhowfar; tot.nhownear += e.cnt.nhownear; tot.ncross += e.cnt.ncross;
                 tot.ndep += e.cnt.ndep; tot.nrn += e.cnt.nrn; tot.nphoton += e.cnt.nphoton;
                 tot.nelectron += e.cnt.nelectron;
             }
         }
+    }
+    if (a.ebal) {
+        wave_add(a.ebal + EB_SOURCE, eb_src);
+        wave_add(a.ebal + EB_ESCAPE, eb_esc);
+        wave_add(a.ebal + EB_LOST, eb_lost);
     }
     if (COUNT && a.counters) {
         atomicAdd(a.counters + 0, tot.nhowfar); atomicAdd(a.counters + 1, tot.nhownear);
@@ -284,6 +303,7 @@ __global__ void __launch_bounds__(256) k_history_split(HistArgs<R> a) {
     Part<R> *st = DEEP ? a.stack + ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * (size_t)a.stack_depth : local;
     const int cap = DEEP ? a.stack_depth : SPLIT_STACK;
     Counters tot = {0, 0, 0, 0, 0, 0, 0};
+    double eb_src = 0.0, eb_esc = 0.0, eb_lost = 0.0;
     for (;;) {
         long h0 = 0;
         if (threadIdx.x % warpSize == 0) h0 = (long)atomicAdd(a.next, (unsigned long long)a.chunk * warpSize);
@@ -300,11 +320,13 @@ __global__ void __launch_bounds__(256) k_history_split(HistArgs<R> a) {
             G rng;
             make_rng<R, G>(rng, a, h, stream);
             TallyAtomic tally;
-            tally.init(a.endep + (size_t)slot * a.slot_stride);
+            tally.init(a.endep + (size_t)slot * a.slot_stride,
+                       a.touched ? a.touched + (size_t)slot * a.touch_stride : nullptr);
             Engine<R, G, TallyAtomic> e{*a.t, *a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
             e.nsplit = a.nsplit;
             if (a.use_beamlets) init_history(e, a.bix, beamlet, st[0]);
             else init_history(e, a.dos, st[0]);
+            eb_src += source_energy(st[0]);
             int np = 0, err = 0;
             long guard = 0;
             while (np >= 0) {
@@ -326,12 +348,19 @@ __global__ void __launch_bounds__(256) k_history_split(HistArgs<R> a) {
             }
             if (err) atomicOr(a.err, 1);
             tally.flush();
+            eb_esc += e.esc;
+            eb_lost += e.lost;
             if (COUNT) {
                 tot.nhowfar += e.cnt.nhowfar; tot.nhownear += e.cnt.nhownear; tot.ncross += e.cnt.ncross;
                 tot.ndep += e.cnt.ndep; tot.nrn += e.cnt.nrn; tot.nphoton += e.cnt.nphoton;
                 tot.nelectron += e.cnt.nelectron;
             }
         }
+    }
+    if (a.ebal) {
+        wave_add(a.ebal + EB_SOURCE, eb_src);
+        wave_add(a.ebal + EB_ESCAPE, eb_esc);
+        wave_add(a.ebal + EB_LOST, eb_lost);
     }
     if (COUNT && a.counters) {
         atomicAdd(a.counters + 0, tot.nhowfar); atomicAdd(a.counters + 1, tot.nhownear);
@@ -342,8 +371,10 @@ __global__ void __launch_bounds__(256) k_history_split(HistArgs<R> a) {
 }
 
 // Batch statistics (omc_dosxyz.c:696-717; omc_matrad.c:897-918): after each
-// batch, sum += e/scale, sum2 += (e/scale)^2, e = 0.
-__global__ void k_accumulate(double *e, double *sum, double *sum2, long n, double scale);
+// batch, sum += e/scale, sum2 += (e/scale)^2, e = 0; the raw energies are
+// also summed into *dep (the EB_DEPOSIT term of the energy balance) when
+// dep is set.
+__global__ void k_accumulate(double *e, double *sum, double *sum2, long n, double scale, double *dep);
 
 // ------------------------------------------------------------------ sampler parity kernel
 // One thread per case: the same inputs and the same injected random numbers

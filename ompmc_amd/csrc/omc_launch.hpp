@@ -11,7 +11,8 @@ namespace omc {
 template <class R> hipError_t launch_history(const HistArgs<R> &a, int rng, bool count, int grid, int block,
                                              hipStream_t st);
 template <class R> hipError_t launch_sampler(const SampArgs<R> &a, hipStream_t st);
-hipError_t launch_accumulate(double *e, double *sum, double *sum2, long n, double scale, hipStream_t st);
+hipError_t launch_accumulate(double *e, double *sum, double *sum2, long n, double scale, double *dep,
+                             hipStream_t st);
 // FP64 atomic throughput probe (grid x 256 lanes, nper scattered adds each)
 hipError_t launch_atomic_probe(double *t, unsigned long long mask, int grid, int nper, unsigned long long seed,
                                hipStream_t st);
@@ -26,6 +27,9 @@ template <class R> struct EvLaunch {
     long long cap;
     double *endep;
     long slot_stride;
+    uint8_t *touched;            // optional tile marks of the tallies, touch_stride per slot
+    long touch_stride;
+    double *ebal;                // optional [EB_N] energy balance of the run
     unsigned long long *counters;
     int budget;
     int turns;
@@ -44,14 +48,18 @@ hipError_t launch_job_start(EvCtl *c, long long nhist, long long target, long lo
                             hipStream_t st);
 
 // Dij post-processing (omc_dij.hpp), FP64
+hipError_t launch_dij_fold_finish(double *e, uint8_t *touched, int ng, int nbatch, long nreg, const double *nper,
+                                  double *dose, double *var, uint8_t *any, double *carry, const double *mass,
+                                  const double *rho, double *blockmax, double *tot_dose, double *tot_var, double *dep,
+                                  hipStream_t st);
 hipError_t launch_accum_slots(double *e, double *sum, double *sum2, long n, long stride, int nslot, const double *nper,
-                              hipStream_t st);
-hipError_t launch_dij_finish(double *sum, double *sum2, double *carry, int nslot, long stride, long nvox,
-                             const double *mass, const double *rho, int nbatch, double *blockmax, double *tot_dose,
-                             double *tot_var, int *nblk, hipStream_t st);
-hipError_t launch_dij_select(const double *dose, long stride, long nvox, int nslot, const double *blockmax, int nblk,
+                              double *dep, hipStream_t st);
+hipError_t launch_dij_finish(double *sum, double *sum2, double *carry, int ng, long nreg, const double *mass,
+                             const double *rho, int nbatch, double *blockmax, double *tot_dose, double *tot_var,
+                             hipStream_t st);
+hipError_t launch_dij_select(const double *dose, const uint8_t *any, long nreg, int ng, const double *blockmax,
                              double rel, double *thresh, int *counts, hipStream_t st);
-hipError_t launch_dij_compact(const double *dose, const double *var, long stride, long nvox, int nslot,
+hipError_t launch_dij_compact(const double *dose, const double *var, const uint8_t *any, long nreg, int ng,
                               const double *thresh, const int64_t *off, int64_t *ir, double *val, double *vval,
                               hipStream_t st);
 

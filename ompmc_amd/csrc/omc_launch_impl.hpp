@@ -49,6 +49,7 @@ template <class R> hipError_t launch_round_part(const EvLaunch<R> &L, long r, in
     if (part == 0) {
         SrcArgs<R> s = L.src;
         s.ctl = L.ctl;
+        s.ebal = L.ebal;
         s.out = L.pool[par][L.src.dos.charge != 0 && !L.src.use_beamlets ? 1 : 0];
         hipLaunchKernelGGL((k_source<R>), dim3(L.grid_source), dim3(256), 0, st, s);
     } else if (part == 1 || part == 2) {
@@ -62,6 +63,9 @@ template <class R> hipError_t launch_round_part(const EvLaunch<R> &L, long r, in
         a.cap = L.cap;
         a.endep = L.endep;
         a.slot_stride = L.slot_stride;
+        a.touched = L.touched;
+        a.touch_stride = L.touch_stride;
+        a.ebal = L.ebal;
         a.counters = L.counters ? L.counters + (part == 1 ? 0 : 8) : nullptr;
         a.err = &L.ctl->err;
         a.budget = L.budget;

@@ -31,15 +31,30 @@ def lib():
         L.orc_session_scalars.argtypes = [vp, dp]
         L.orc_session_pegs.argtypes = [vp, dp, ip]
         L.orc_session_case.argtypes = [vp, C.c_int, dp, C.c_int, dp, C.c_long, dp, C.c_int, lp]
-        L.orc_session_dose.argtypes = [vp, C.c_int, C.c_int, dp, dp, ip]
+        L.orc_session_dose.argtypes = [vp, C.c_int, C.c_int, dp, dp, ip, dp]
         L.orc_session_dose_omp.restype = C.c_double
         L.orc_session_dose_omp.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
         L.orc_session_finish.argtypes = [vp, dp, dp, C.c_int, C.c_int, dp, dp]
         L.orc_session_ranmar.argtypes = [vp, C.c_long, ip]
         L.orc_ranmar_stream.argtypes = [C.c_int, C.c_int, C.c_long, C.c_long, ip]
         L.orc_session_matrad.argtypes = [vp, C.c_int, C.c_double, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int,
-                                         C.c_double, ip, C.c_int, dp, dp, ip]
-        L.orc_session_matrad_finish.argtypes = [vp, dp, dp, C.c_int, dp, dp]
+                                         C.c_double, ip, C.c_int, dp, dp, ip, dp]
+        i64p = C.POINTER(C.c_int64)
+        L.orc_mr_create.restype = vp
+        L.orc_mr_create.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp, dp, dp, C.c_int]
+        L.orc_mr_destroy.argtypes = [vp]
+        L.orc_mr_endep.restype = dp
+        L.orc_mr_endep.argtypes = [vp]
+        L.orc_mr_accum.restype = dp
+        L.orc_mr_accum.argtypes = [vp, C.c_int]
+        L.orc_mr_accum_endep.argtypes = [vp, C.c_int]
+        L.orc_mr_accumulate.argtypes = [vp, C.c_int, C.c_int]
+        L.orc_mr_beamlet_end.argtypes = [vp, C.c_int, C.c_double]
+        L.orc_mr_nnz.restype = C.c_int64
+        L.orc_mr_nnz.argtypes = [vp]
+        L.orc_mr_copy.argtypes = [vp, i64p, i64p, dp, dp]
+        L.orc_mr_run.restype = vp
+        L.orc_mr_run.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp, dp, dp, C.c_int, C.c_int, ip, dp, C.c_double]
         L.orc_session_write_3ddose.argtypes = [vp, C.c_char_p, dp, dp]
         L.orc_session_matrad_source.argtypes = [vp, C.c_int, C.c_double, dp, dp, C.c_int, ip, dp, dp, dp, dp,
                                                 C.c_int, C.c_double, C.c_int, C.c_int, dp]
@@ -111,14 +126,23 @@ class Session:
                                    C.byref(used))
         return (out[:k] if k >= 0 else None), used.value
 
-    def dose(self, nperbatch, nbatch, want_counts=False):
+    def dose(self, nperbatch, nbatch, want_counts=False, want_energy=False):
+        """Serial dose run: (sum, sum2), then the draws per history and the
+        energy balance [source, escape, lost, deposit] (MeV) when asked."""
         s = np.zeros(self.nreg)
         s2 = np.zeros(self.nreg)
         cnt = np.zeros(nperbatch * nbatch, np.int32)
-        rc = lib().orc_session_dose(self.h, nperbatch, nbatch, _dp(s), _dp(s2), _ip(cnt) if want_counts else None)
+        eb = np.zeros(4)
+        rc = lib().orc_session_dose(self.h, nperbatch, nbatch, _dp(s), _dp(s2), _ip(cnt) if want_counts else None,
+                                    _dp(eb))
         if rc:
             raise RuntimeError(f"oracle dose run failed: {rc}")
-        return (s, s2, cnt) if want_counts else (s, s2)
+        out = (s, s2)
+        if want_counts:
+            out += (cnt,)
+        if want_energy:
+            out += (eb,)
+        return out
 
     def dose_omp(self, nperbatch, nbatch, nthreads=0):
         s = np.zeros(self.nreg)
@@ -143,18 +167,20 @@ class Session:
         lib().orc_session_ranmar(self.h, n, _ip(out))
         return out
 
-    def matrad(self, spectrum, bix, nper, nbatch, want_counts=False):
+    def matrad(self, spectrum, bix, nper, nbatch, want_counts=False, want_batches=False):
         """Serial matRad Dij transport (reference loop order).  spectrum: an
         object with energy / cdfinv1 / cdfinv2 (cdfinv1 None for mono); bix:
         dict with ibeam, src, corner, side1, side2 (float arrays), gaussian,
         sigma; nper: histories per batch of each beamlet.  Returns per-beamlet
-        sums [nbix, nreg] and sums of squares (no carried variance), and the
-        draws per history when asked."""
+        sums [nbix, nreg] and sums of squares (no carried variance), then the
+        draws per history and the raw batch tallies [nbix, nbatch, nreg] when
+        asked."""
         nb = len(nper)
         s = np.zeros((nb, self.nreg))
         s2 = np.zeros((nb, self.nreg))
         nper = np.ascontiguousarray(nper, np.int32)
         cnt = np.zeros(int(nper.sum()) * nbatch, np.int32)
+        bat = np.zeros((nb, nbatch, self.nreg)) if want_batches else None
         sp = spectrum.cdfinv1 is not None
         c1 = np.ascontiguousarray(spectrum.cdfinv1 if sp else np.zeros(1000))
         c2 = np.ascontiguousarray(spectrum.cdfinv2 if sp else np.zeros(1000))
@@ -163,10 +189,16 @@ class Session:
         rc = lib().orc_session_matrad(self.h, int(sp), float(spectrum.energy), _dp(c1), _dp(c2), nb, _ip(ib),
                                       _dp(arr["src"]), _dp(arr["corner"]), _dp(arr["side1"]), _dp(arr["side2"]),
                                       int(bix.get("gaussian", 0)), float(bix.get("sigma", 0.0)), _ip(nper), nbatch,
-                                      _dp(s), _dp(s2), _ip(cnt) if want_counts else None)
+                                      _dp(s), _dp(s2), _ip(cnt) if want_counts else None,
+                                      _dp(bat) if want_batches else None)
         if rc:
             raise RuntimeError(f"oracle matrad run failed: {rc}")
-        return (s, s2, cnt) if want_counts else (s, s2)
+        out = (s, s2)
+        if want_counts:
+            out += (cnt,)
+        if want_batches:
+            out += (bat,)
+        return out
 
     def matrad_source(self, spectrum, bix, ib, n):
         """n primaries of beamlet ib: rows iq ir e x y z u v w wt dnear."""
@@ -201,25 +233,86 @@ class Session:
                                             float(bix.get("sigma", 0.0)), _ip(nper), nbatch, nthreads, _dp(s), _dp(s2))
         return s, s2, secs
 
-    def matrad_dij(self, s, s2, nbatch, rel):
-        """The reference's per-beamlet finish with the variance carried from
-        beamlet to beamlet (omc_matrad.c:1603-1709), thresholded strictly above
-        rel * max: CSC (jc, ir, val, var) with 0-based voxel rows."""
-        nb, nreg = s.shape
-        nv = nreg - 1
-        carry = np.zeros(nreg)
-        jc, irs, vals, vars_ = [0], [], [], []
-        for b in range(nb):
-            dose, var = np.zeros(nv), np.zeros(nv)
-            s2b = np.ascontiguousarray(s2[b] + carry)
-            lib().orc_session_matrad_finish(self.h, _dp(np.ascontiguousarray(s[b])), _dp(s2b), nbatch, _dp(dose),
-                                            _dp(var))
-            carry = np.concatenate([[0.0], var])
-            th = dose.max() * rel
-            idx = np.nonzero(dose > th)[0]
-            irs.append(idx)
-            vals.append(dose[idx])
-            vars_.append(var[idx])
-            jc.append(jc[-1] + len(idx))
-        return (np.array(jc, np.int64), np.concatenate(irs).astype(np.int64), np.concatenate(vals),
-                np.concatenate(vars_))
+
+def _geom(ph):
+    xb, yb, zb = (np.ascontiguousarray(a, np.float64) for a in (ph.xb, ph.yb, ph.zb))
+    rho = np.ascontiguousarray(ph.rho, np.float64).reshape(-1)
+    return len(xb) - 1, len(yb) - 1, len(zb) - 1, xb, yb, zb, rho
+
+
+def matrad_dij(ph, batches, nper, nbatch, rel):
+    """omc_matrad's beamlet loop (omc_matrad.c:1583-1702) over given batch
+    tallies: accumEndep per batch, accumulateResults (Gy, the Q11 variance,
+    rho < 0.044 zeroed), the strict threshold rel * max, the CSC append and
+    the reference's carry of accum_endep2 into the next beamlet.  ph: an object
+    with xb, yb, zb (boundaries) and rho (x fastest); batches [nbix, nbatch,
+    nvox + 1] raw energy per batch (region 0 = outside); nper [nbix] histories
+    per batch.  Returns (jc, ir, val, var) with 0-based voxel rows."""
+    nx, ny, nz, xb, yb, zb, rho = _geom(ph)
+    batches = np.ascontiguousarray(batches, np.float64)
+    nb = batches.shape[0]
+    assert batches.shape == (nb, nbatch, nx * ny * nz + 1)
+    npr = np.ascontiguousarray(nper, np.int32)
+    h = lib().orc_mr_run(nx, ny, nz, _dp(xb), _dp(yb), _dp(zb), _dp(rho), nb, nbatch, _ip(npr), _dp(batches),
+                         float(rel))
+    if not h:
+        raise MemoryError("oracle Dij assembly")
+    try:
+        n = int(lib().orc_mr_nnz(h))
+        jc, ir = np.zeros(nb + 1, np.int64), np.zeros(max(n, 1), np.int64)
+        val, var = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        i64p = C.POINTER(C.c_int64)
+        lib().orc_mr_copy(h, jc.ctypes.data_as(i64p), ir.ctypes.data_as(i64p), _dp(val), _dp(var))
+    finally:
+        lib().orc_mr_destroy(h)
+    return jc, ir[:n], val[:n], var[:n]
+
+
+class MatradScore:
+    """Step-by-step access to the restated omc_matrad scoring (the closed-form
+    tests drive accumEndep / accumulateResults / the beamlet end one call at a
+    time and read the accumulators between them)."""
+
+    def __init__(self, ph, nbeamlets):
+        nx, ny, nz, xb, yb, zb, rho = _geom(ph)
+        self.nreg = nx * ny * nz + 1
+        self.nb = nbeamlets
+        self.h = lib().orc_mr_create(nx, ny, nz, _dp(xb), _dp(yb), _dp(zb), _dp(rho), nbeamlets)
+        if not self.h:
+            raise MemoryError("oracle Dij scoring")
+
+    def close(self):
+        if self.h:
+            lib().orc_mr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _view(self, p):
+        return np.ctypeslib.as_array(p, shape=(self.nreg,))
+
+    def set_endep(self, e):
+        self._view(lib().orc_mr_endep(self.h))[:] = e
+
+    def accum(self):
+        """copies of (accum_endep, accum_endep2)"""
+        return self._view(lib().orc_mr_accum(self.h, 0)).copy(), self._view(lib().orc_mr_accum(self.h, 1)).copy()
+
+    def accum_endep(self, nperbatch):
+        lib().orc_mr_accum_endep(self.h, int(nperbatch))
+
+    def accumulate(self, nbatch, iout=1):
+        lib().orc_mr_accumulate(self.h, int(iout), int(nbatch))
+
+    def beamlet_end(self, ib, rel):
+        if lib().orc_mr_beamlet_end(self.h, int(ib), float(rel)):
+            raise MemoryError("oracle Dij assembly")
+
+    def csc(self):
+        n = int(lib().orc_mr_nnz(self.h))
+        jc, ir = np.zeros(self.nb + 1, np.int64), np.zeros(max(n, 1), np.int64)
+        val, var = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        i64p = C.POINTER(C.c_int64)
+        lib().orc_mr_copy(self.h, jc.ctypes.data_as(i64p), ir.ctypes.data_as(i64p), _dp(val), _dp(var))
+        return jc, ir[:n], val[:n], var[:n]

@@ -43,6 +43,9 @@ struct orc {
     int tcap, tn;
     long nhowfar, nhownear;
     double ensrc;
+    /* energy balance (not in the reference): photons that left the phantom,
+     * energy dropped without a deposit (quirk Q6) */
+    double esc, lost;
 };
 
 /* ------------------------------------------------------------------ RNG */
@@ -580,6 +583,7 @@ start_mfp_loop:
             o->y[np] += ustep * o->v[np];
             o->z[np] += ustep * o->w[np];
             if (idisc > 0) {
+                o->esc += o->wt[np] * o->e[np];     /* leaves untallied (src/ompmc.c:2008-2017) */
                 np -= 1;
                 o->np = np;
                 if (np < 0) return;
@@ -1435,6 +1439,7 @@ void orc_electron(orc *o) {
                         }
                     } else {
                         /* the reference warns and drops the particle (quirk Q6) */
+                        o->lost += o->wt[np] * (eie + (iq > 0 ? RM : -RM));
                         o->np -= 1;
                         return;
                     }
@@ -1716,16 +1721,27 @@ void orc_init_history_matrad(orc *o, const omc_spectrum *spec, const orc_beamlet
 }
 
 /* ------------------------------------------------------------------ drivers */
+/* Energy balance of a run (not in the reference): ebal[0] the primaries'
+ * energy (photons E, electrons E - m, positrons E + m), [1] photons that
+ * left the phantom, [2] energy dropped without a deposit (quirk Q6), [3]
+ * energy deposited in all regions including region 0. */
+static double primary_energy(const orc *o) {
+    int np = o->np;
+    return o->wt[np] * (o->iq[np] == 0 ? o->e[np] : o->e[np] + (o->iq[np] > 0 ? RM : -RM));
+}
+
 int orc_run_dosxyz(const omc_tables *t, const omc_regions *r, const orc_geom *g, const omc_dosxyz_source *s,
                    int nsplit, int ixx, int jxx, int nperbatch, int nbatch, double *sum, double *sum2,
-                   int *rn_per_history) {
+                   int *rn_per_history, double *ebal) {
     orc *o = orc_create(t, r, g, nsplit, ixx, jxx);
     if (!o) return OMC_ERR_NOMEM;
     long h = 0;
+    double src = 0.0, dep = 0.0;
     for (int b = 0; b < nbatch; b++) {
         for (int i = 0; i < nperbatch; i++) {
             long before = o->count;
             orc_init_history_dosxyz(o, s);
+            src += primary_energy(o);
             orc_shower(o);
             if (rn_per_history) rn_per_history[h] = (int)(o->count - before);
             h++;
@@ -1733,10 +1749,17 @@ int orc_run_dosxyz(const omc_tables *t, const omc_regions *r, const orc_geom *g,
         }
         for (int i = 0; i < r->nreg; i++) {
             double e = o->endep[i];
+            dep += e;
             sum[i] += e;
             sum2[i] += e * e;
         }
         memset(o->endep, 0, sizeof(double) * r->nreg);
+    }
+    if (ebal) {
+        ebal[0] = src;
+        ebal[1] = o->esc;
+        ebal[2] = o->lost;
+        ebal[3] = dep;
     }
     orc_destroy(o);
     return OMC_OK;
@@ -1802,10 +1825,12 @@ int orc_run_matrad_beamlet(orc *o, const omc_spectrum *spec, const orc_beamlets 
  * (omc_matrad.c:1576-1600): beamlet by beamlet, batch by batch, nper[b]
  * histories per batch on one RANMAR stream.  sum/sum2 + b*nreg receive
  * beamlet b's per-batch sums of endep/nper[b] and of its square (accumEndep,
- * :897-918); rn_per_history (optional) the draws of each history. */
+ * :897-918); rn_per_history (optional) the draws of each history; batches
+ * (optional, [nbix][nbatch][nreg]) each batch's energy tally before it is
+ * folded in. */
 int orc_run_matrad(const omc_tables *t, const omc_regions *r, const orc_geom *g, const omc_spectrum *spec,
                    const orc_beamlets *b, int nsplit, int ixx, int jxx, const int *nper, int nbatch, double *sum,
-                   double *sum2, int *rn_per_history) {
+                   double *sum2, int *rn_per_history, double *batches) {
     orc *o = orc_create(t, r, g, nsplit, ixx, jxx);
     if (!o) return OMC_ERR_NOMEM;
     long h = 0;
@@ -1821,6 +1846,7 @@ int orc_run_matrad(const omc_tables *t, const omc_regions *r, const orc_geom *g,
                 h++;
                 if (o->err) { orc_destroy(o); return OMC_ERR_STACK; }
             }
+            if (batches) memcpy(batches + ((long)ib * nbatch + bt) * nreg, o->endep, sizeof(double) * nreg);
             for (int i = 0; i < nreg; i++) {
                 double e = o->endep[i];
                 e /= (double)nper[ib];

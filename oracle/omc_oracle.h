@@ -97,7 +97,7 @@ void orc_init_history_matrad(orc *o, const omc_spectrum *spec, const orc_beamlet
  * optionally per-history random-number counts. */
 int orc_run_dosxyz(const omc_tables *t, const omc_regions *r, const orc_geom *g, const omc_dosxyz_source *s,
                    int nsplit, int ixx, int jxx, int nperbatch, int nbatch, double *sum, double *sum2,
-                   int *rn_per_history);
+                   int *rn_per_history, double *ebal);
 /* OpenMP dosxyz run as the reference runs it: thread k seeds (ixx, jxx+k),
  * histories scheduled dynamically; used for CPU timing. Returns seconds. */
 double orc_run_dosxyz_omp(const omc_tables *t, const omc_regions *r, const orc_geom *g,
@@ -112,7 +112,7 @@ int orc_run_matrad_beamlet(orc *o, const omc_spectrum *spec, const orc_beamlets 
  * without the carried variance, draws per history optional. */
 int orc_run_matrad(const omc_tables *t, const omc_regions *r, const orc_geom *g, const omc_spectrum *spec,
                    const orc_beamlets *b, int nsplit, int ixx, int jxx, const int *nper, int nbatch, double *sum,
-                   double *sum2, int *rn_per_history);
+                   double *sum2, int *rn_per_history, double *batches);
 /* OpenMP variant for CPU timing (thread k seeds (ixx, jxx+k)); returns seconds. */
 double orc_run_matrad_omp(const omc_tables *t, const omc_regions *r, const orc_geom *g, const omc_spectrum *spec,
                           const orc_beamlets *b, int nsplit, int ixx, int jxx, const int *nper, int nbatch,

@@ -226,9 +226,10 @@ int orc_session_case(orc_session *s, int kind, const double *in, int nin, const 
 }
 
 /* Serial dosxyz run from the input file's source and seeds. */
-int orc_session_dose(orc_session *s, int nperbatch, int nbatch, double *sum, double *sum2, int *rn_per_history) {
+int orc_session_dose(orc_session *s, int nperbatch, int nbatch, double *sum, double *sum2, int *rn_per_history,
+                     double *ebal) {
     return orc_run_dosxyz(&s->t, &s->r, &s->g, &s->src, s->nsplit, s->ixx, s->jxx, nperbatch, nbatch, sum, sum2,
-                          rn_per_history);
+                          rn_per_history, ebal);
 }
 
 double orc_session_dose_omp(orc_session *s, int nperbatch, int nbatch, int nthreads, double *sum, double *sum2) {
@@ -258,11 +259,12 @@ void orc_ranmar_stream(int ixx, int jxx, long skip, long n, int *out) {
  * beamlet b, sum/sum2 + b*nreg receive the per-batch sums of endep/nper[b]
  * and of its square (accumEndep, :897-918) without the variance carried from
  * the previous beamlet; rn_per_history (optional) the draws of each history
- * in run order. */
+ * in run order; batches (optional) every batch's raw energy tally,
+ * [nbix][nbatch][nreg]. */
 int orc_session_matrad(orc_session *s, int spectrum, double energy, const double *cdf1, const double *cdf2, int nbix,
                        const int *ibeam, const double *src, const double *corner, const double *side1,
                        const double *side2, int gaussian, double sigma, const int *nper, int nbatch, double *sum,
-                       double *sum2, int *rn_per_history) {
+                       double *sum2, int *rn_per_history, double *batches) {
     omc_spectrum sp;
     memset(&sp, 0, sizeof sp);
     if (spectrum) {
@@ -275,15 +277,7 @@ int orc_session_matrad(orc_session *s, int spectrum, double energy, const double
     }
     orc_beamlets bl = {nbix, ibeam, src, corner, side1, side2, gaussian, sigma};
     return orc_run_matrad(&s->t, &s->r, &s->g, &sp, &bl, s->nsplit, s->ixx, s->jxx, nper, nbatch, sum, sum2,
-                          rn_per_history);
-}
-
-/* The reference's per-beamlet finish (omc_matrad.c:920-1012) on the
- * session's phantom: dose and variance [nvox] from sum/sum2 [nreg], where
- * sum2 already holds the carried variance. */
-void orc_session_matrad_finish(orc_session *s, const double *sum, const double *sum2, int nbatch, double *dose,
-                               double *var) {
-    omc_matrad_finish(&s->ph, sum, sum2, nbatch, dose, var);
+                          rn_per_history, batches);
 }
 
 /* OpenMP matRad transport for CPU timing; returns the transport seconds. */

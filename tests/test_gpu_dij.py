@@ -2,9 +2,10 @@
 
 Replay: the history form in FP64, each history fed its own slice of the
 oracle's RANMAR stream (the reference's serial loop order beamlet, batch,
-history), reproduces the oracle's sparse matrix: the same columns and rows,
-values within 1e-9, including the variance output with the reference's
-carry-over from one beamlet to the next.  The batched forms (beamlet groups,
+history), reproduces the oracle's sparse matrix -- its transport
+(oracle/omc_oracle.c) and its scoring and assembly (oracle/orc_matrad.c) --:
+the same columns and rows, values within 1e-9, including the variance output
+with the reference's carry-over from one beamlet to the next.  The batched forms (beamlet groups,
 event queues) are checked against the plain per-beamlet form."""
 import os
 
@@ -68,8 +69,9 @@ def test_dij_replay_fp64_matches_oracle(tmp_path, group, gaussian):
     sess = B.Session(stem)
     spec = omc.Spectrum.read(os.path.join(omc.DATA, "spectra", "mohan6.spectrum"))
     nper = _nper(bix, nbatch)
-    s, s2, cnt = sess.matrad(spec, _bix_dict(bix, gaussian, sigma), nper, nbatch, want_counts=True)
-    jc, ir, val, var = sess.matrad_dij(s, s2, nbatch, rel)
+    s, s2, cnt, bat = sess.matrad(spec, _bix_dict(bix, gaussian, sigma), nper, nbatch, want_counts=True,
+                                  want_batches=True)
+    jc, ir, val, var = B.matrad_dij(ph_h, bat, nper, nbatch, rel)
     off = np.zeros(len(cnt) + 1, np.int64)
     off[1:] = np.cumsum(cnt)
     stream = sess.ranmar(int(off[-1])).astype(np.uint32)

@@ -139,7 +139,8 @@ def test_history_replay_fp64_matches_oracle(tmp_path, name):
         if kw.get("spectrum") else omc.Spectrum.mono(kw["energy"])
     g1, g2, st = c.run_dosxyz(ph, spec, nh // nb, nb, collimator=kw.get("collimator", (-5, 5, -5, 5)),
                               charge=kw.get("charge", 0), precision=omc.FP64, form=omc.FORM_HISTORY,
-                              rng=omc.RNG_REPLAY, replay=stream, replay_offsets=off, nsplit=kw.get("nsplit", 1))
+                              rng=omc.RNG_REPLAY, replay=stream, replay_offsets=off, nsplit=kw.get("nsplit", 1),
+                              keep_diverged_replay=True)
     c.close()
     assert st.errors & 1 == 0
     inside = s1[1:] > 0
