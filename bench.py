@@ -23,7 +23,7 @@ phantom and tables resident in HBM.  Beside it the line reports:
 * a CPU baseline: the oracle restatement (OpenMP, the reference's loop) on a
   sample of the same beamlets.
 
-    python bench.py [--steps K] [--warmup W] [--config c3|c2|c1] [--form event|history]
+    python bench.py [--steps K] [--warmup W] [--config c3|c2|c1|c5] [--form event|history]
                     [--precision fp32|fp64] [--no-cpu] [--no-serve] [--no-plain]
 """
 import argparse
@@ -48,6 +48,9 @@ def log(*a):
 
 
 def cpu_cores():
+    """Host cores the CPU baseline uses: those this process may run on, at
+    most 16 -- the host share that goes with one GPU on the measurement
+    hosts, whatever nproc reports there (bench.py reports both)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
@@ -203,13 +206,57 @@ def cpu_baseline_c3(ph, bix, spec, nbatch, seconds_target=12.0):
         secs = runs[1]
         # one thread on the same beamlets with 1/cores of their histories
         secs1, nh1 = run(nb, 1, cores)
-        return dict(value=nh / secs, unit="histories/s", cores=cores, kind="port",
+        try:
+            xc = cpu_cross_check(cores)
+        except Exception as e:          # reported, not required
+            xc = {"error": str(e)}
+        return dict(value=nh / secs, unit="histories/s", cores=cores, kind="port", cross_check=xc,
                     sample=f"first {nb} beamlets of the plan ({nh} histories, {nbatch} batches), oracle "
                            f"restatement of the reference's matRad loop, OpenMP {cores} threads, median of 3 runs "
                            f"({', '.join(f'{x:.1f}' for x in runs)} s)",
                     one_core={"value": nh1 / secs1, "sample": f"the same {nb} beamlets with 1/{cores} of their "
                                                                f"histories ({nh1}), 1 thread, {secs1:.1f} s"},
                     host={"cpu_model": cpu_model(), "nproc": os.cpu_count(), "threads_allowed": cores})
+
+
+def cpu_cross_check(cores, nhist=200000):
+    """The reference's own omc_dosxyz, compiled from its sources into
+    oracle/_ref, against the oracle restatement on the same input: the 60^3
+    water cube with 2 MeV photons in a 10x10 cm field, nhist histories in 10
+    batches, both with OpenMP on the same `cores` host cores (the reference
+    pinned with taskset, since it takes omp_get_num_procs() threads).
+    Transport time only: the reference's two "Execution time up to this
+    point" lines around its batch loop (omc_dosxyz.c:1234,1267), the
+    restatement's own timer around the same loop.  None where the reference
+    binary was not built."""
+    import re
+    import subprocess
+    from oracle import bindings as OB
+    from ompmc_amd import phantoms as P
+    ref = os.path.join(REPO, "oracle", "_ref", "omc_dosxyz")
+    if not os.path.exists(ref):
+        return None
+    with tempfile.TemporaryDirectory() as tmp:
+        phf = os.path.join(tmp, "w60.egsphant")
+        P.write_egsphant(phf, P.water_cube(60, 0.5))
+        d = P.default_inp(phf, tmp + "/", energy=2.0, ncase=nhist, nbatch=10, collimator=(-5, 5, -5, 5))
+        stem = P.write_inp(os.path.join(tmp, "w60"), d)[:-4]
+        cpus = ",".join(str(c) for c in sorted(os.sched_getaffinity(0))[:cores])
+        t_ref = []
+        for _ in range(3):
+            r = subprocess.run(["taskset", "-c", cpus, ref, "-i", stem, "-o", "ref"], stdout=subprocess.PIPE,
+                               stderr=subprocess.STDOUT, text=True, timeout=600, cwd=tmp)
+            t = [float(x) for x in re.findall(r"Execution time up to this point :\s*([0-9.]+)", r.stdout)]
+            if r.returncode != 0 or len(t) < 2:
+                return {"error": f"reference run failed ({r.returncode})"}
+            t_ref.append(t[1] - t[0])
+        s = OB.Session(stem)
+        t_orc = sorted(s.dose_omp(nhist // 10, 10, cores)[2] for _ in range(3))
+        s.close()
+    t_ref.sort()
+    return {"reference_seconds": t_ref[1], "oracle_seconds": t_orc[1], "ratio_oracle_over_reference": t_orc[1] / t_ref[1],
+            "cores": cores, "sample": f"60^3 water, 2 MeV, 10x10 cm, {nhist} histories in 10 batches, median of 3",
+            "reference_histories_per_s": nhist / t_ref[1], "oracle_histories_per_s": nhist / t_orc[1]}
 
 
 # ------------------------------------------------------------------ c3
@@ -239,6 +286,7 @@ def bench_c3(a):
     log("warmup done")
     walls, hist, plan, st_last = [], 0.0, None, None
     ksec = np.zeros(5)
+    kbusy = np.zeros(5)
     klaunch = np.zeros(5)
     for k in range(a.steps):
         t0 = time.perf_counter()
@@ -247,6 +295,7 @@ def bench_c3(a):
         walls.append(time.perf_counter() - t0)
         hist += st.histories
         ksec += np.array(st.kernel_seconds[:])
+        kbusy += np.array(st.kernel_busy_seconds[:])
         klaunch += np.array(st.kernel_launches[:])
         if pl is not None:
             plan = pl
@@ -265,7 +314,12 @@ def bench_c3(a):
     else:
         dom = 0
         bph = total_bytes(stc) / stc.histories
-    achieved = bph * hist / ksec[dom] / 1e9
+    # launches of the three group streams overlap: the kernel's time on the
+    # device is the union of its launches (busy), not their sum; the history
+    # form runs one launch at a time
+    busy = kbusy[dom] if kbusy[dom] > 0 else ksec[dom]
+    achieved = bph * hist / busy / 1e9
+    achieved_per_launch = bph * hist / ksec[dom] / 1e9
     traffic = read_traffic(dom, hist, klaunch[dom]) if (a.config == "c3" and form == omc.FORM_EVENT
                                                         and prec == omc.FP32) else None
     out = {
@@ -294,9 +348,17 @@ def bench_c3(a):
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_history": bph,
+                     "busy_seconds": float(busy), "summed_launch_seconds": float(ksec[dom]),
                      "per_launch_bytes": bph * hist / max(klaunch[dom], 1),
-                     "per_launch_seconds": ksec[dom] / max(klaunch[dom], 1)},
+                     "per_launch_seconds": ksec[dom] / max(klaunch[dom], 1),
+                     "per_launch_frac": achieved_per_launch / HBM_PEAK_GBS,
+                     "what": "algorithmic bytes of the kernel's histories over the time the device ran at least "
+                             "one of its launches (union over the concurrent streams); per_launch_frac divides by "
+                             "the summed launch durations instead"},
     }
+    valu = read_valu(dom, value)
+    if valu:
+        out["valu_roofline"] = valu
     # secondary roofline: FP64 tally atomics.  Every deposit call counts as one
     # atomic (an upper bound: a lane folds repeated deposits to one region into
     # one add), against the device's measured rate of scattered FP64 atomic adds
@@ -406,6 +468,34 @@ def bench_c3(a):
     return out
 
 
+VALU_LANES_PER_S = 256 * 4 * 16 * 2.4e9   # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (non-packed FP32/FP64)
+
+
+def read_valu(kernel_class, histories_per_s):
+    """VALU roofline of a kernel from the committed issue-counter summary:
+    lane-instructions per history (SQ_THREAD_CYCLES_VALU: one thread-cycle
+    per active lane of a VALU instruction) times the measured histories/s,
+    against the device's lane rate; and the issue rate of wave instructions
+    (SQ_INSTS_VALU) against its wave-instruction rate (a wave64 instruction
+    occupies a 16-lane SIMD for 4 cycles).  None without a summary."""
+    path = os.path.join(REPO, "profiles", "c3_charged_issue_counters.md")
+    try:
+        text = open(path).read()
+        t = json.loads(text.split("```json", 1)[1].split("```", 1)[0]).get(KERNEL_NAMES[kernel_class])
+    except (OSError, IndexError, ValueError):
+        return None
+    if not t:
+        return None
+    lanes = t["valu_thread_cycles_per_history"] * histories_per_s
+    waves = t["valu_insts_per_history"] * histories_per_s
+    return {"bound": "valu", "kernel": KERNEL_NAMES[kernel_class], "achieved": lanes, "peak": VALU_LANES_PER_S,
+            "unit": "lane-instructions/s", "frac": lanes / VALU_LANES_PER_S,
+            "issue_frac": waves / (VALU_LANES_PER_S / 64), "lane_utilisation": t["lane_utilisation"],
+            "per_history": {"valu_insts": t["valu_insts_per_history"],
+                            "valu_thread_cycles": t["valu_thread_cycles_per_history"]},
+            "source": "profiles/c3_charged_issue_counters.md (counters per history) x this run's histories/s"}
+
+
 def read_traffic(kernel_class, histories, launches):
     """HBM bytes per launch of a kernel from the committed PMC summary:
     (FETCH_SIZE + WRITE_SIZE) per history, each scaled by what the counter
@@ -421,6 +511,82 @@ def read_traffic(kernel_class, histories, launches):
     if not t or launches <= 0:
         return None
     return (t["read_bytes_per_history"] + t["write_bytes_per_history"]) * histories / launches
+
+
+# ------------------------------------------------------------------ c5 (large plan)
+def bench_c5(a):
+    """The large plan: Dij of 10044 beamlets over 9 beams on a 256x256x200 CT
+    at 0.2 cm, 1e9 histories (lognormal per beamlet), 10 batches, threshold
+    1e-4.  One step is the whole plan; the step runs in FP32 transport (the
+    value) and once in FP64, and the plan doses of the two are compared in
+    the voxels above 20% of the FP64 maximum."""
+    import ompmc_amd as omc
+    from ompmc_amd import phantoms as P
+    nbatch, rel = 10, 1e-4
+    t_setup = time.perf_counter()
+    ph, bix = P.large_plan(a.c5_histories)
+    spec = omc.Spectrum.from_histogram(*P.synthetic_6mv())
+    ctx = omc.Context(ph.media)
+    gph = ctx.phantom(ph, ecut=0.7, pcut=0.01)
+    log(f"c5: {bix.n} beamlets, {int(bix.nhist.sum())} histories, setup {time.perf_counter() - t_setup:.1f} s")
+    opts = dict(form=omc.FORM_EVENT, csc_on_device=True, download=False)
+    sub = subset(bix, 100)
+    _, _, _, stc = ctx.run_dij(gph, spec, sub, nbatch=nbatch, rel_threshold=rel, count_events=True, precision=omc.FP32,
+                               **opts)
+    ctx.run_dij(gph, spec, sub, nbatch=nbatch, rel_threshold=rel, precision=omc.FP64, **opts)   # warm-up
+    res = {}
+    for prec in (omc.FP32, omc.FP64):
+        t0 = time.perf_counter()
+        _, _, pl, st = ctx.run_dij(gph, spec, bix, nbatch=nbatch, rel_threshold=rel, want_plan=True,
+                                   count_events="time", precision=prec, seed=1802 + prec, **opts)
+        wall = time.perf_counter() - t0
+        res[prec] = (pl, st, wall)
+        log(f"c5 fp{prec}: {st.histories / wall:.4g} histories/s, {wall:.1f} s, {st.groups} groups of "
+            f"{st.group_size}, nnz {st.nnz}, energy residual {st.energy_residual():.2e}")
+    (p32, v32), st32, w32 = res[omc.FP32]
+    (p64, v64), st64, w64 = res[omc.FP64]
+    reg = p64 > 0.2 * p64.max()
+    d = p32[reg].sum() / p64[reg].sum() - 1.0
+    sig = math.sqrt(v32[reg].sum() + v64[reg].sum()) / p64[reg].sum()
+    z = np.abs(p32[reg] - p64[reg]) / np.maximum(np.sqrt(v32[reg] + v64[reg]), 1e-300)
+    sel50 = p64 > 0.5 * p64.max()
+    unc50 = float(np.mean(np.sqrt(v32[sel50]) / p32[sel50]))
+    ksec = np.array(st32.kernel_seconds[:])
+    kbusy = np.array(st32.kernel_busy_seconds[:])
+    dom = 3 if ksec[3] >= ksec[2] else 2
+    bph = kernel_bytes(stc, dom - 2) / stc.histories
+    busy = kbusy[dom] if kbusy[dom] > 0 else ksec[dom]
+    achieved = bph * st32.histories / busy / 1e9
+    value = st32.histories / w32
+    out = {
+        "metric": "histories/s + p50/p99 request latency vs plain GPU form at <=1% dose unc.",
+        "value": value, "unit": "histories/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": 1e3 * w32,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "tally_dtype": "f64",
+        "data": "synthetic CT, beamlets and 6 MV spectrum (seeded)",
+        "config": {"workload": "c5: Dij of a 9-beam plan, 256x256x200 synthetic CT at 0.2 cm, "
+                               f"{bix.n} beamlets of 0.5 cm, {bix.nhist.sum():.3g} histories, 10 batches, "
+                               "threshold 1e-4", "beamlets": int(bix.n), "histories_per_step": st32.histories,
+                   "group": int(st32.group_size), "groups": int(st32.groups)},
+        "fp64": {"histories_per_s": st64.histories / w64, "seconds": w64, "nnz": int(st64.nnz)},
+        "fp64_plan_dose_rel_diff": {"value": d, "sigma": sig, "region": "> 20% of the FP64 plan maximum",
+                                    "voxels": int(reg.sum()), "frac_within_2sigma": float(np.mean(z <= 2)),
+                                    "what": "one FP32 and one FP64 run of the whole plan, different seeds; sigma "
+                                            "from the plan-dose batch variances summed over the region"},
+        "plan_dose_rel_unc_gt50": unc50,
+        "nnz": int(st32.nnz),
+        "energy_residual": {"fp32": st32.energy_residual(), "fp64": st64.energy_residual()},
+        "kernels": {KERNEL_NAMES[k]: {"seconds": float(ksec[k]), "busy_seconds": float(kbusy[k]),
+                                      "launches": int(st32.kernel_launches[k])}
+                    for k in range(5) if st32.kernel_launches[k] > 0},
+        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_history": bph,
+                     "busy_seconds": float(busy)},
+        "cpu_baseline": None,
+    }
+    log(f"c5: fp32 {value:.4g}, fp64 {out['fp64']['histories_per_s']:.4g} histories/s; fp32/fp64 {d:+.2e} +- "
+        f"{sig:.1e}")
+    ctx.close()
+    return out
 
 
 # ------------------------------------------------------------------ c1 / c2 (single dose runs)
@@ -471,7 +637,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c3", choices=("c1", "c2", "c3"))
+    ap.add_argument("--config", default="c3", choices=("c1", "c2", "c3", "c5"))
+    ap.add_argument("--c5-histories", type=float, default=1e9, help="c5: histories of the whole plan")
     ap.add_argument("--form", default="event", choices=("event", "history"))
     ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64"))
     ap.add_argument("--group", type=int, default=0, help="beamlets per group (0: by HBM budget)")
@@ -506,7 +673,7 @@ def main():
               "(beamlets and batches would shard across GPUs with no reduction; see the design notes)",
               file=sys.stderr)
         sys.exit(2)
-    out = bench_c3(a) if a.config == "c3" else bench_dose(a)
+    out = bench_c3(a) if a.config == "c3" else bench_c5(a) if a.config == "c5" else bench_dose(a)
     print(json.dumps(out))
 
 

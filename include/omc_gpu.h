@@ -136,9 +136,14 @@ typedef struct omc_run_opts {
     int streams;            /* event-form Dij: beamlet groups transported concurrently, each on its own
                                stream (0 = automatic: up to 3 when the run has several groups; 1..4) */
     int64_t max_particle_steps; /* event form: condensed-history steps after which an electron counts as
-                               stuck, is dropped and the run fails with OMC_ERR_LIMIT (0 = 2^20) */
+                               stuck, is dropped and the run fails with OMC_ERR_LIMIT (0 = 2^20); history
+                               form: transport calls per history (photon flights, electron transports up
+                               to a secondary) after which the history is cut and the run fails with
+                               OMC_ERR_LIMIT (0 = 10^8) */
     int csc_on_device;      /* Dij: 1 = leave the matrices in device memory (omc_csc.device = 1) for a
                                consumer on the GPU; 0 = host arrays, as the mex interface needs */
+    int graph;              /* event form: 0 = each job's chunks of rounds replay a captured HIP graph
+                               (eager launches when count_events times the kernels); -1 = eager launches */
 } omc_run_opts;
 
 /* Kernel classes of omc_run_stats.kernel_seconds / kernel_launches */
@@ -174,6 +179,10 @@ typedef struct omc_run_stats {
      * photon splitting e_source = e_escape + e_lost + e_deposit up to
      * rounding; with it, in expectation. */
     double e_source, e_escape, e_lost, e_deposit;
+    /* event-form runs with count_events: per kernel class, the time during
+     * which at least one launch of the class ran (launches of concurrent
+     * streams overlap, so kernel_seconds, their sum, can exceed it) */
+    double kernel_busy_seconds[5];
 } omc_run_stats;
 
 /* Defaults: FP32, event form, Philox, seed 1802, nsplit 1, automatic launch. */

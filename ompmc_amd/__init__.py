@@ -60,7 +60,8 @@ class RunOpts(C.Structure):
                 ("grid", C.c_int), ("block", C.c_int), ("count_events", C.c_int), ("event_target", C.c_int64),
                 ("electron_budget", C.c_int), ("group", C.c_int), ("hbm_budget", C.c_int64),
                 ("voxel_walk", C.c_int), ("electron_turns", C.c_int), ("batches_per_job", C.c_int),
-                ("streams", C.c_int), ("max_particle_steps", C.c_int64), ("csc_on_device", C.c_int)]
+                ("streams", C.c_int), ("max_particle_steps", C.c_int64), ("csc_on_device", C.c_int),
+                ("graph", C.c_int)]
 
 
 class RunStats(C.Structure):
@@ -74,7 +75,8 @@ class RunStats(C.Structure):
                 ("rounds", C.c_uint64), ("peak_inflight", C.c_uint64), ("groups", C.c_int64),
                 ("group_size", C.c_int64), ("nnz", C.c_int64), ("batches_per_job", C.c_int64),
                 ("streams", C.c_int64), ("e_source", C.c_double), ("e_escape", C.c_double),
-                ("e_lost", C.c_double), ("e_deposit", C.c_double)]
+                ("e_lost", C.c_double), ("e_deposit", C.c_double),
+                ("kernel_busy_seconds", C.c_double * 5)]
 
     def energy_residual(self):
         """(deposit + escape + lost - source) / source of the run"""
@@ -199,7 +201,7 @@ def default_paths():
 def run_opts(precision=FP32, form=FORM_EVENT, seed=1802, rng=RNG_PHILOX, replay=None, replay_offsets=None,
              grid=0, block=0, count_events=False, nsplit=1, event_target=0, electron_budget=0, group=0,
              hbm_budget=0, voxel_walk=False, electron_turns=0, batches_per_job=0, streams=0, max_particle_steps=0,
-             csc_on_device=False):
+             csc_on_device=False, graph=True):
     o = RunOpts()
     lib().omc_run_opts_default(C.byref(o))
     o.precision, o.form, o.seed, o.rng, o.nsplit = precision, form, seed, rng, nsplit
@@ -210,6 +212,7 @@ def run_opts(precision=FP32, form=FORM_EVENT, seed=1802, rng=RNG_PHILOX, replay=
     o.electron_turns, o.batches_per_job, o.streams = electron_turns, batches_per_job, streams
     o.max_particle_steps = max_particle_steps
     o.csc_on_device = int(bool(csc_on_device))
+    o.graph = 0 if graph else -1
     keep = []
     if replay is not None:
         rp = np.ascontiguousarray(replay, np.uint32)

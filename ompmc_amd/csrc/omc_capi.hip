@@ -553,6 +553,7 @@ int omc_check_opts(const omc_run_opts *o) {
         return omc_seterr(OMC_ERR_ARG, "negative launch option");
     if (o->electron_turns < -1) return omc_seterr(OMC_ERR_ARG, "electron_turns %d (-1 no limit, 0 automatic)", o->electron_turns);
     if (o->streams < 0 || o->streams > 4) return omc_seterr(OMC_ERR_ARG, "streams %d (0 automatic, 1..4)", o->streams);
+    if (o->graph != 0 && o->graph != -1) return omc_seterr(OMC_ERR_ARG, "graph %d (0 graphs, -1 eager launches)", o->graph);
     return OMC_OK;
 }
 
@@ -607,6 +608,7 @@ static int run_dosxyz_history(omc_ctx *c, hipStream_t stream, PhantomDev *pd, co
     a.seed = o->seed;
     a.replay = d_rn;
     a.chunk = 1;
+    a.max_steps = hist_max_steps(o);
     a.nsplit = o->nsplit;
     rc = alloc_split_stacks<R>(A, a, o->nsplit, hblock, max_grid);
     if (rc) goto done;

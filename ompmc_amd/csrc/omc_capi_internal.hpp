@@ -158,6 +158,13 @@ template <class R> inline int make_dosxyz_src(DevArena &A, const PhantomDev *pd,
         if (e_ != hipSuccess) { rc = omc_seterr(OMC_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); goto done; } \
     } while (0)
 
+// History-form step guard: transport calls (one photon flight, or an
+// electron's transport up to its next secondary) one history may take before
+// it counts as stuck; max_particle_steps sets it, 0 means 10^8.
+inline long hist_max_steps(const omc_run_opts *o) {
+    return o->max_particle_steps > 0 ? (long)o->max_particle_steps : 100000000L;
+}
+
 // Device-memory particle stacks for history-form splitting beyond the
 // scratch stack: split_stack_depth(nsplit) entries for every lane of a
 // launch of up to max_grid blocks of `block` lanes.  The stacks take at most

@@ -925,13 +925,15 @@ template <class R, class G, class T> struct Engine {
     // Flight of one sub-photon of a split (src/ompmc.c:1960-2032): from its
     // start, `dpmfp` mean free paths along its direction, voxel by voxel.
     // Returns false when it leaves the geometry (the reference discards it
-    // and ends the split), true at its interaction site, with imed the
+    // and ends the split; the `left` sub-photons from this one on, which
+    // would follow it along the same ray, count as escaped in the energy
+    // balance), true at its interaction site, with imed the
     // medium there.  lgle, cohfac and gmfp are the caller's: like the
     // reference's function-scope variables they keep the values of the last
     // voxel with a medium, which the interaction then uses.  The step
     // arithmetic is the reference's, in its order, on values from `xs`.
     __device__ __forceinline__ bool split_fly(Part<R> &p, R dpmfp, const SplitXs &xs, int &imed, int &lgle,
-                                              R &cohfac, R &gmfp) {
+                                              R &cohfac, R &gmfp, int left) {
         R rho;
         voxel_of(p, imed, rho);
         for (;;) {
@@ -958,7 +960,7 @@ template <class R, class G, class T> struct Engine {
             if (p.ix < 0) {           // howfar: outside the geometry -> discard
                 if (count) cnt.nhowfar++;
                 p.x += ustep * p.u; p.y += ustep * p.v; p.z += ustep * p.w;
-                esc += (double)p.wt * (double)p.e;
+                esc += (double)left * (double)p.wt * (double)p.e;
                 return false;
             }
             bool leaves;
@@ -1024,7 +1026,7 @@ template <class R, class G, class T> struct Engine {
                 p.u = save.u; p.v = save.v; p.w = save.w;
                 p.e = save.e; p.wt = wtsave;
                 p.ix = save.ix; p.iy = save.iy; p.iz = save.iz; p.iq = 0;
-                if (!split_fly(p, dpmfp, xs, imed, lgle, cohfac, gmfp)) {
+                if (!split_fly(p, dpmfp, xs, imed, lgle, cohfac, gmfp, ns - isplit)) {
                     np -= 1;
                     if (np < 0) return np;
                     break;                          // end of the splitting loop

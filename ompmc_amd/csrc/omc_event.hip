@@ -125,18 +125,61 @@ struct JobStats {
     double t_source = 0, t_photon = 0, t_charged = 0, t_other = 0;
     uint64_t n_source = 0, n_photon = 0, n_charged = 0, n_other = 0;
     uint64_t ev_photon = 0, ev_charged = 0, rounds = 0, peak = 0;
+    // With an origin event (recorded before any of the run's work on every
+    // stream), the start and end of each timed launch in ms after it, per
+    // kernel class (source, photon, charged, round end): launches of
+    // concurrent streams overlap, and the union of these intervals is the
+    // time the device ran a class at all.
+    hipEvent_t origin = nullptr;
+    std::vector<std::pair<float, float>> iv[4];
 };
+
+// Seconds covered by the union of the intervals of one kernel class over
+// all given jobs (concurrent launches count once).
+double busy_seconds(const std::vector<const JobStats *> &jobs, int k) {
+    std::vector<std::pair<float, float>> v;
+    for (const JobStats *j : jobs) v.insert(v.end(), j->iv[k].begin(), j->iv[k].end());
+    std::sort(v.begin(), v.end());
+    double tot = 0, a = 0, b = -1;
+    for (const auto &x : v) {
+        if (x.first > b) {
+            if (b > a) tot += b - a;
+            a = x.first;
+            b = x.second;
+        } else if (x.second > b) {
+            b = x.second;
+        }
+    }
+    if (b > a) tot += b - a;
+    return tot * 1e-3;
+}
 
 // Runs one job (nhist primaries) to completion.  Rounds are enqueued in
 // chunks of K; the host reads a snapshot of the control block after each
 // chunk while the next chunk runs, so the device never waits for the host.
 // With `timing`, every kernel of every round is bracketed by events on the
 // job's stream and its duration is added to the job statistics when the
-// chunk has completed.
+// chunk has completed.  Without it and with L.graph, the first chunk's
+// kernels are captured into a HIP graph and every chunk of the job replays
+// it: a round's kernels depend on the round only through its parity, and
+// chunks start at even rounds, so every chunk is the same sequence of
+// launches.  A replay is one runtime call instead of 4 K launches.
+struct GraphExec {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    ~GraphExec() {
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
 template <class R>
 int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int dest, bool count, bool timing,
             JobStats &js) {
     constexpr int K = 4;
+    static_assert(K % 2 == 0, "chunks must start at even rounds");
+    const bool use_graph = L.graph && !timing;
+    GraphExec gx;
     // With photon splitting one split puts up to 2 x nsplit charged particles
     // into the queues: a round injects at most target / nsplit primaries, so
     // that a round's photons fit their products into the pools.
@@ -149,13 +192,29 @@ int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int 
     Events chunk_done(2);
     Events kev(timing ? 2 * K * 5 : 0);
     auto enqueue = [&](long r0, int slot) -> hipError_t {
-        for (int j = 0; j < K; j++) {
-            hipEvent_t *e = timing ? &kev.ev[(size_t)(slot * K + j) * 5] : nullptr;
-            hipError_t x;
-            if (e && (x = hipEventRecord(e[0], st)) != hipSuccess) return x;
-            for (int part = 0; part < 4; part++) {
-                if ((x = launch_round_part(L, r0 + j, part, count, st)) != hipSuccess) return x;
-                if (e && (x = hipEventRecord(e[part + 1], st)) != hipSuccess) return x;
+        hipError_t x;
+        if (use_graph) {
+            if (!gx.x) {
+                // capture in this thread's mode: the lanes of a Dij run
+                // enqueue from threads of their own meanwhile
+                if ((x = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) return x;
+                hipError_t xl = hipSuccess;
+                for (int j = 0; j < K && xl == hipSuccess; j++)
+                    for (int part = 0; part < 4 && xl == hipSuccess; part++) xl = launch_round_part(L, r0 + j, part, count, st);
+                x = hipStreamEndCapture(st, &gx.g);
+                if (xl != hipSuccess) return xl;
+                if (x != hipSuccess) return x;
+                if ((x = hipGraphInstantiate(&gx.x, gx.g, nullptr, nullptr, 0)) != hipSuccess) return x;
+            }
+            if ((x = hipGraphLaunch(gx.x, st)) != hipSuccess) return x;
+        } else {
+            for (int j = 0; j < K; j++) {
+                hipEvent_t *e = timing ? &kev.ev[(size_t)(slot * K + j) * 5] : nullptr;
+                if (e && (x = hipEventRecord(e[0], st)) != hipSuccess) return x;
+                for (int part = 0; part < 4; part++) {
+                    if ((x = launch_round_part(L, r0 + j, part, count, st)) != hipSuccess) return x;
+                    if (e && (x = hipEventRecord(e[part + 1], st)) != hipSuccess) return x;
+                }
             }
         }
         return hipMemcpyAsync(&snap[slot], S.ctl, sizeof(EvCtl), hipMemcpyDeviceToHost, st) == hipSuccess
@@ -170,6 +229,11 @@ int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int 
             js.t_photon += kev.ms(b + 1, b + 2) * 1e-3;
             js.t_charged += kev.ms(b + 2, b + 3) * 1e-3;
             js.t_other += kev.ms(b + 3, b + 4) * 1e-3;
+            if (js.origin) {
+                float t[5];
+                for (int q = 0; q < 5; q++) (void)hipEventElapsedTime(&t[q], js.origin, kev.ev[b + q]);
+                for (int q = 0; q < 4; q++) js.iv[q].emplace_back(t[q], t[q + 1]);
+            }
         }
         js.n_source += K; js.n_photon += K; js.n_charged += K; js.n_other += K;
     };
@@ -237,6 +301,7 @@ template <class R> void fill_launch(omc_ctx *c, const PhantomDev *pd, EvState<R>
     L.max_age = L.turns > 0 ? (int)std::min<long>(steps / ((long)L.turns * L.budget) + 1, 1L << 22) : 0;
     L.delta = o->voxel_walk ? 0 : 1;
     L.nsplit = o->nsplit;
+    L.graph = o->graph >= 0 ? 1 : 0;
     int cus = device_cus(c->device);
     L.grid_event = o->grid > 0 ? o->grid : cus * 8;
     L.grid_source = cus * 4;
@@ -268,6 +333,12 @@ int run_job_grow(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist,
         L.ctl = S.ctl;
         L.cap = S.cap;
     }
+}
+
+// Union busy time per kernel class of the given jobs (timed runs).
+void stats_busy(omc_run_stats *st, const std::vector<const JobStats *> &jobs) {
+    static const int cls[4] = {OMC_K_SOURCE, OMC_K_PHOTON, OMC_K_CHARGED, OMC_K_OTHER};
+    for (int q = 0; q < 4; q++) st->kernel_busy_seconds[cls[q]] = busy_seconds(jobs, q);
 }
 
 void stats_from_job(omc_run_stats *st, const JobStats &js) {
@@ -361,6 +432,7 @@ int omc_run_dosxyz_event(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const o
     JobStats js;
     Events ev(2);
     CK(hipEventRecord(ev.ev[0], stream));
+    js.origin = ev.ev[0];
     for (int b0 = 0; b0 < nbatch; b0 += nbj) {
         const int nb = std::min(nbj, nbatch - b0);
         L.src.hist_base = (long)b0 * nper;       // streams stay keyed by the history's place in the run
@@ -386,6 +458,7 @@ int omc_run_dosxyz_event(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const o
         st->batches_per_job = nbj;
         stats_counters(st, cnt);
         stats_from_job(st, js);
+        if (o->count_events) stats_busy(st, {&js});
         st->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
     }
     return OMC_OK;
@@ -428,9 +501,11 @@ size_t dij_fixed_bytes(long nreg, const omc_run_opts *o, long long target) {
     size_t b = (size_t)3 * nreg * sizeof(double) + ((size_t)64 << 20);
     if (o->form == OMC_FORM_EVENT)
         b += o->precision == OMC_FP32 ? event_bytes<float>(target) : event_bytes<double>(target);
-    else if (o->nsplit > HISTORY_SCRATCH_NSPLIT)
-        b += (size_t)(o->grid > 0 ? o->grid : 2048) * (o->block > 0 ? o->block : 256) * split_stack_depth(o->nsplit) *
-             (o->precision == OMC_FP32 ? sizeof(Part<float>) : sizeof(Part<double>));
+    else if (o->nsplit > HISTORY_SCRATCH_NSPLIT)    // particle stacks, capped as alloc_split_stacks caps them
+        b += std::min<size_t>((size_t)(o->grid > 0 ? o->grid : 2048) * (o->block > 0 ? o->block : 256) *
+                                  split_stack_depth(o->nsplit) *
+                                  (o->precision == OMC_FP32 ? sizeof(Part<float>) : sizeof(Part<double>)),
+                              (size_t)8 << 30);
     return b;
 }
 
@@ -642,12 +717,22 @@ template <class R> int dij_lane_init(DijRun<R> &D, DijLane<R> &l) {
         H.err = D.d_err;
         H.next = D.d_next;
         H.chunk = 1;
+        H.max_steps = hist_max_steps(o);
         H.replay = D.d_rn;
         H.replay_off = D.d_rstart;
         H.replay_end = D.d_rend;
         H.nsplit = o->nsplit;
-        l.max_grid = o->grid > 0 ? o->grid : 2048;
-        int rc = alloc_split_stacks<R>(A, H, o->nsplit, o->block > 0 ? o->block : 256, l.max_grid);
+        // the largest launch is one batch of the largest group: the particle
+        // stacks of nsplit > 8 cover that many blocks and no more
+        const int hblock = o->block > 0 ? o->block : 256;
+        long long maxlaunch = 1;
+        for (int g0 = 0; g0 < D.nbix; g0 += G) {
+            long long t = 0;
+            for (int b = g0; b < std::min(D.nbix, g0 + G); b++) t += D.nper[b];
+            maxlaunch = std::max(maxlaunch, t);
+        }
+        l.max_grid = o->grid > 0 ? o->grid : (int)std::min<long long>((maxlaunch + hblock - 1) / hblock, 2048);
+        int rc = alloc_split_stacks<R>(A, H, o->nsplit, hblock, l.max_grid);
         if (rc) return rc;
     }
     return OMC_OK;
@@ -957,6 +1042,7 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
     Events ev(2);
     CK(hipEventRecord(ev.ev[0], stream));
     for (int k = 1; k < NL; k++) CK(hipStreamWaitEvent(lanes[k]->st, ev.ev[0], 0));   // after the set-up above
+    for (auto &l : lanes) l->js.origin = ev.ev[0];       // every lane's work follows it
     for (auto &l : lanes)
         if ((rc = dij_lane_init(D, *l))) return rc;
     D.outs.resize(D.ngroups);
@@ -1070,6 +1156,11 @@ int run_dij(omc_ctx *c, hipStream_t stream, PhantomDev *pd, const omc_spectrum_d
             st->kernel_launches[OMC_K_HISTORY] += l->n_hist;
             st->kernel_seconds[OMC_K_OTHER] += l->t_other;
             st->kernel_launches[OMC_K_OTHER] += l->n_other;
+        }
+        if (D.event && D.timing) {
+            std::vector<const JobStats *> jobs;
+            for (auto &l : lanes) jobs.push_back(&l->js);
+            stats_busy(st, jobs);
         }
         st->histories = histories;
         st->groups = D.ngroups;

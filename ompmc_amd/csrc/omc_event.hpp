@@ -372,7 +372,7 @@ k_event_split(EvArgs<R> a) {
                     Part<R> sp = save;
                     sp.wt = wtsave;
                     sp.iq = 0;
-                    if (!e.split_fly(sp, dpmfp, xs, imed, lgle, cohfac, gmfp)) {
+                    if (!e.split_fly(sp, dpmfp, xs, imed, lgle, cohfac, gmfp, ns - isplit)) {
                         live = false;
                     } else {
                         save.x = sp.x; save.y = sp.y; save.z = sp.z;

@@ -164,6 +164,7 @@ template <class R> struct HistArgs {
     int *err;                    // [4] 0 overflow, 1 replay underrun, 2 step guard
     unsigned long long *next;    // dynamic work counter (zeroed by the host)
     int chunk;
+    long max_steps;              // transport calls per history after which it counts as stuck (step guard)
     int nsplit;                  // photon splitting factor; > 1 selects k_history_split
     Part<R> *stack;              // nsplit > HISTORY_SCRATCH_NSPLIT: per-lane particle stacks in device memory,
     int stack_depth;             // stack_depth entries for each lane of the launch (null: stacks in scratch)
@@ -258,7 +259,7 @@ __global__ void __launch_bounds__(256) k_history(HistArgs<R> a) {
                     deep[depth++] = p;
                     p = q;
                 }
-                if (++guard > 100000000L) { atomicOr(a.err + 2, 1); break; }
+                if (++guard > a.max_steps) { atomicOr(a.err + 2, 1); break; }
             }
             tally.flush();
             eb_esc += e.esc;
@@ -344,7 +345,7 @@ __global__ void __launch_bounds__(256) k_history_split(HistArgs<R> a) {
                 }
                 if (err) break;
                 if (e.aborted) { atomicOr(a.err + 2, 1); break; }
-                if (++guard > 100000000L) { atomicOr(a.err + 2, 1); break; }
+                if (++guard > a.max_steps) { atomicOr(a.err + 2, 1); break; }
             }
             if (err) atomicOr(a.err, 1);
             tally.flush();

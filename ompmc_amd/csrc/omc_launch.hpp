@@ -36,6 +36,7 @@ template <class R> struct EvLaunch {
     int max_age;
     int delta;
     int nsplit;                  // photon splitting factor; > 1 transports photons with k_event_split
+    int graph;                   // 1: untimed chunks of rounds replay a captured HIP graph
     SrcArgs<R> src;              // source of the job (ctl and out are set per round)
     int grid_event, grid_source;
 };

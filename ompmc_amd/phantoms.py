@@ -266,6 +266,20 @@ def beamlets(nbeams=5, sad=100.0, bixel=0.5, radius=5.6, iso=(0.0, 0.0, 0.0),
     return Beamlets(np.array(ib, np.int32), np.array(src), np.array(cor), np.array(s1), np.array(s2), nh)
 
 
+def large_plan(total_histories=1e9, seed=20261207):
+    """The large-plan workload: a 256x256x200 synthetic CT at 0.2 cm with the
+    tissue model of ``synthetic_ct`` (its target a 5 cm sphere), 9 coplanar
+    beams and 10044 bixels of 0.5 cm on a 5 mm grid (a disc of 9.4 cm at the
+    isocentre), lognormal histories per bixel rescaled so that the plan
+    carries ``total_histories``.  Returns (phantom, beamlets)."""
+    ph = synthetic_ct(nx=256, ny=256, nz=200, vx=0.2, vy=0.2, vz=0.2, seed=seed, target_radius=5.0)
+    ph.name = "ct256x256x200"
+    bix = beamlets(nbeams=9, radius=9.4, seed=seed)
+    w = bix.nhist / bix.nhist.sum()
+    bix.nhist = np.maximum(np.rint(w * total_histories), 10).astype(np.int64)
+    return ph, bix
+
+
 def synthetic_6mv(seed=20261204, nbins=24, emin=0.25, emax=6.0):
     """24 bins with upper edges 0.25..6.0 MeV; weights E*exp(-E/1.5) times
     seeded +-5% noise, smoothed with a 3-point kernel; counts per bin."""

@@ -62,3 +62,57 @@ def textured_data(tmp_path_factory):
     subprocess.check_call([sys.executable, os.path.join(REPO, "data", "gen_spinms.py"), "--variant", "textured",
                            str(d / "spinms.data")])
     return str(d) + "/"
+
+
+# ------------------------------------------------------------------ statistical agreement
+# Two independent batch estimates of a voxel's dose, means ma, mb with
+# standard errors ua, ub from na, nb batches (the batch method: ua^2 is the
+# sample variance of the batch values over na), give z = (ma - mb) /
+# sqrt(ua^2 + ub^2).  With normal batch values and equal means, z follows
+# Student's t with the Welch-Satterthwaite degrees of freedom
+#     nu = (ua^2 + ub^2)^2 / (ua^4 / (na - 1) + ub^4 / (nb - 1)),
+# between min(na, nb) - 1 and na + nb - 2.  The expected fraction of voxels
+# with |z| <= k is therefore the mean over voxels of P(|t_nu| <= k): 0.939
+# (k = 2) and 0.992 (k = 3) at 10 batches each, against 0.954 and 0.997 for
+# exact standard errors.  North of 95% within 2 sigma is out of reach of any
+# unbiased estimator at 10-20 batches, and a demand of 99% would fail on
+# correct code.  The observed fraction scatters around its expectation with
+# variance f(1 - f) / n for n independent voxels; neighbouring voxels share
+# particle tracks, so the variance is taken three times larger, and the
+# acceptance is the expectation less four of those standard errors (at least
+# 0.02).  A bias common to all voxels moves the mean of the signed z away
+# from zero long before it moves the fractions: |mean z| must stay within
+# four standard errors of that mean (again with the factor three).
+def z_agreement(ma, ua, mb, ub, na, nb=None):
+    """Per-voxel agreement of two batch estimates (arrays over the selected
+    voxels).  Returns a dict with the observed and accepted fractions within
+    2 and 3 sigma, the mean signed z and its bound, and the voxel count."""
+    from scipy import stats
+    import numpy as np
+    nb = nb or na
+    va, vb = np.asarray(ua, float) ** 2, np.asarray(ub, float) ** 2
+    den = np.maximum(va + vb, 1e-300)
+    z = (np.asarray(ma, float) - np.asarray(mb, float)) / np.sqrt(den)
+    nu = den ** 2 / np.maximum(va ** 2 / (na - 1) + vb ** 2 / (nb - 1), 1e-300)
+    nu = np.clip(nu, min(na, nb) - 1, na + nb - 2)
+    n = len(z)
+    out = {"n": n, "zbar": float(np.mean(z)) if n else 0.0}
+    for k in (2, 3):
+        e = float(np.mean(2.0 * stats.t.cdf(k, nu) - 1.0)) if n else 1.0
+        f = float(np.mean(np.abs(z) <= k)) if n else 1.0
+        out[f"f{k}"], out[f"e{k}"] = f, e
+        out[f"lim{k}"] = e - max(0.02, 4.0 * np.sqrt(3.0 * e * (1.0 - e) / max(n, 1)))
+    sdz = float(np.std(z)) if n > 1 else 1.0
+    out["zbar_lim"] = 4.0 * sdz * np.sqrt(3.0 / max(n, 1))
+    return out
+
+
+def assert_agreement(ag, what=""):
+    assert ag["f2"] >= ag["lim2"] and ag["f3"] >= ag["lim3"], (what, ag)
+    assert abs(ag["zbar"]) <= ag["zbar_lim"], (what, ag)
+
+
+def fmt_agreement(ag):
+    return (f"{ag['n']} voxels, {ag['f2']:.3f} within 2 sigma (expected {ag['e2']:.3f}, accept >= {ag['lim2']:.3f}), "
+            f"{ag['f3']:.3f} within 3 (expected {ag['e3']:.3f}, >= {ag['lim3']:.3f}), mean z {ag['zbar']:+.3f} "
+            f"(|.| <= {ag['zbar_lim']:.3f})")

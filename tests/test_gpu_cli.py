@@ -9,7 +9,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from tests.conftest import DOSE_CASES, REPO, write_case_inp
+from tests.conftest import DOSE_CASES, REPO, assert_agreement, fmt_agreement, write_case_inp, z_agreement
 
 pytestmark = pytest.mark.gpu
 
@@ -49,9 +49,9 @@ def test_cli_3ddose_matches_reference_statistically(tmp_path, name, flags):
     dims_r, bounds_r, dr, ur = read_3ddose(str(tmp_path / "ref.3ddose"))
     assert dims == dims_r and np.allclose(bounds, bounds_r)
     sel = dr > 0.1 * dr.max()
-    z = np.abs(dg[sel] - dr[sel]) / np.hypot(ug[sel] * dg[sel], ur[sel] * dr[sel])
-    print(f"{flags}: {sel.sum()} voxels, {np.mean(z <= 2):.3f} within 2 sigma")
-    assert np.mean(z <= 2.0) >= 0.88 and np.mean(z <= 3.0) >= 0.97
+    ag = z_agreement(dg[sel], ug[sel] * dg[sel], dr[sel], ur[sel] * dr[sel], nb)
+    print(f"{flags}: {fmt_agreement(ag)}")
+    assert_agreement(ag, flags)
     assert abs(dg[sel].sum() / dr[sel].sum() - 1) < 0.02
 
 

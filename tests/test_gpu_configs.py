@@ -3,9 +3,10 @@
 * 60^3 water at 0.5 cm, 2 MeV photons, 10x10 cm parallel field, 1e6
   histories in 10 batches, FP64: the event form on the GPU against the
   oracle (the reference's algorithm with RANMAR, OpenMP over the histories of
-  each batch) -- per voxel above 10% of the maximum, the two batch means
-  differ by at most 2 (3) combined standard errors in at least 88% (97%) of
-  the voxels, and the deposited energies agree within 1% (about ten
+  each batch) -- per voxel above 10% of the maximum, the fractions of
+  voxels whose batch means differ by at most 2 (3) combined standard errors
+  reach the levels the t distribution of the batch estimates gives, with no
+  common bias (tests/conftest.py), and the deposited energies agree within 1% (about ten
   standard errors of the total at 1e6 histories).
 * 128^3 layered slab at 0.3 cm (water, lung 0.26, bone 1.85, water),
   synthetic 6 MV spectrum, 10x10 cm field, 1e7 histories: FP32 transport
@@ -17,6 +18,8 @@ import os
 
 import numpy as np
 import pytest
+
+from tests.conftest import assert_agreement, fmt_agreement, z_agreement
 
 pytestmark = pytest.mark.gpu
 
@@ -47,12 +50,12 @@ def test_water_1e6_fp64_agrees_with_oracle(tmp_path):
     mo, uo = _batch_stats(o1[1:], o2[1:], nb)
     mg, ug = _batch_stats(g1[1:], g2[1:], nb)
     sel = np.maximum(mo, mg) > 0.1 * max(mo.max(), mg.max())
-    z = np.abs(mo[sel] - mg[sel]) / np.maximum(np.hypot(uo[sel], ug[sel]), 1e-300)
+    ag = z_agreement(mo[sel], uo[sel], mg[sel], ug[sel], nb)
     eo, eg = mo.sum(), mg.sum()
-    print(f"water 1e6 FP64: {sel.sum()} voxels, {np.mean(z <= 2):.3f} within 2 sigma, {np.mean(z <= 3):.3f} within "
-          f"3; energy per batch {eg:.6g} vs oracle {eo:.6g} ({(eg / eo - 1):+.2e}); oracle {secs:.1f} s")
-    assert sel.sum() > 1000
-    assert np.mean(z <= 2.0) >= 0.88 and np.mean(z <= 3.0) >= 0.97
+    print(f"water 1e6 FP64: {fmt_agreement(ag)}; energy per batch {eg:.6g} vs oracle {eo:.6g} ({(eg / eo - 1):+.2e}); "
+          f"oracle {secs:.1f} s")
+    assert ag["n"] > 1000
+    assert_agreement(ag)
     assert abs(eg / eo - 1.0) < 0.01
 
 
@@ -80,9 +83,8 @@ def test_slab_1e7_fp32_within_half_percent_of_fp64():
     m32, u32 = _batch_stats(*runs[omc.FP32][0], nb)
     m64, u64 = _batch_stats(*runs[omc.FP64][0], nb)
     sel = np.maximum(m32, m64) > 0.1 * max(m32.max(), m64.max())
-    z = np.abs(m32[sel] - m64[sel]) / np.maximum(np.hypot(u32[sel], u64[sel]), 1e-300)
-    print(f"slab 1e7 FP32 vs FP64: region {reg.sum()} voxels, {rel:+.3e} +- {sem:.1e}; per voxel "
-          f"{np.mean(z <= 2):.3f} within 2 sigma over {sel.sum()} voxels")
+    ag = z_agreement(m32[sel], u32[sel], m64[sel], u64[sel], nb)
+    print(f"slab 1e7 FP32 vs FP64: region {reg.sum()} voxels, {rel:+.3e} +- {sem:.1e}; per voxel {fmt_agreement(ag)}")
     assert sem < 2.5e-3
     assert abs(rel) < 5e-3
-    assert np.mean(z <= 2.0) >= 0.88 and np.mean(z <= 3.0) >= 0.97
+    assert_agreement(ag)

@@ -12,6 +12,8 @@ import os
 import numpy as np
 import pytest
 
+from tests.conftest import assert_agreement, fmt_agreement, z_agreement
+
 pytestmark = pytest.mark.gpu
 
 omc = pytest.importorskip("ompmc_amd")
@@ -130,9 +132,9 @@ def test_dij_grouping_and_forms_agree(tmp_path):
     pd_h, pv_h = plain[2]
     pd_e, pv_e = event[2]
     sel = pd_h > 0.5 * pd_h.max()
-    z = np.abs(pd_h[sel] - pd_e[sel]) / np.sqrt(pv_h[sel] + pv_e[sel])
-    print(f"event vs history plan dose: {sel.sum()} voxels, {np.mean(z <= 2):.3f} within 2 sigma")
-    assert np.mean(z <= 2.0) >= 0.85 and np.mean(z <= 3.0) >= 0.97
+    ag = z_agreement(pd_h[sel], np.sqrt(pv_h[sel]), pd_e[sel], np.sqrt(pv_e[sel]), 5)
+    print(f"event vs history plan dose: {fmt_agreement(ag)}")
+    assert_agreement(ag)
 
 
 def test_autotuner_picks_a_measured_setting(tmp_path, monkeypatch):

@@ -19,7 +19,7 @@ checks are statistical:
 import numpy as np
 import pytest
 
-from tests.conftest import DOSE_CASES, golden, write_case_inp
+from tests.conftest import DOSE_CASES, assert_agreement, fmt_agreement, golden, write_case_inp, z_agreement
 
 pytestmark = pytest.mark.gpu
 
@@ -46,11 +46,12 @@ def _batch_stats(s1, s2, nb):
 
 
 def _agreement(a1, a2, b1, b2, nb):
+    """Per-voxel agreement above 10% of the maximum, at the levels the
+    t distribution of the batch estimates gives (tests/conftest.py)."""
     ma, ua = _batch_stats(a1[1:], a2[1:], nb)
     mb, ub = _batch_stats(b1[1:], b2[1:], nb)
     sel = np.maximum(ma, mb) > 0.1 * max(ma.max(), mb.max())
-    z = np.abs(ma[sel] - mb[sel]) / np.maximum(np.hypot(ua[sel], ub[sel]), 1e-300)
-    return float(np.mean(z <= 2.0)), float(np.mean(z <= 3.0)), int(sel.sum())
+    return z_agreement(ma[sel], ua[sel], mb[sel], ub[sel], nb)
 
 
 def _ctx(name):
@@ -70,10 +71,10 @@ def test_event_split_fp64_agrees_with_oracle(tmp_path, name):
     c, ph = _ctx(name)
     g1, g2, st = _run(c, ph, name, nh, nb, precision=omc.FP64, form=omc.FORM_EVENT, seed=21)
     assert st.errors == 0
-    f2, f3, n = _agreement(s1, s2, g1, g2, nb)
-    print(f"{name}: {n} voxels, {f2:.3f} within 2 sigma, {f3:.3f} within 3 sigma")
-    assert n > 20
-    assert f2 >= 0.88 and f3 >= 0.97, (f2, f3)
+    ag = _agreement(s1, s2, g1, g2, nb)
+    print(f"{name}: {fmt_agreement(ag)}")
+    assert ag["n"] > 20
+    assert_agreement(ag, name)
     tots = [g1[1:].sum()]
     for k in range(7):
         tots.append(_run(c, ph, name, nh, nb, precision=omc.FP64, form=omc.FORM_EVENT, seed=200 + k)[0][1:].sum())
@@ -81,7 +82,7 @@ def test_event_split_fp64_agrees_with_oracle(tmp_path, name):
     sd = float(np.std(tots, ddof=1))
     ta = s1[1:].sum()
     print(f"{name}: oracle total {ta:.1f}, GPU {np.mean(tots):.1f} +- {sd:.1f} per run")
-    assert abs(ta - np.mean(tots)) <= 4.0 * sd * np.sqrt(1 + 1 / len(tots))
+    assert abs(ta - np.mean(tots)) <= 3.0 * sd * np.sqrt(1 + 1 / len(tots))
 
 
 def test_event_split_matches_unsplit_and_lowers_variance():
@@ -95,14 +96,14 @@ def test_event_split_matches_unsplit_and_lowers_variance():
     b1, b2, sb = _run(c, ph, name, nh, nb, precision=omc.FP32, form=omc.FORM_EVENT, seed=32, nsplit=8)
     c.close()
     assert sa.errors == 0 and sb.errors == 0
-    f2, f3, n = _agreement(a1, a2, b1, b2, nb)
+    ag = _agreement(a1, a2, b1, b2, nb)
     ma, ua = _batch_stats(a1[1:], a2[1:], nb)
     mb, ub = _batch_stats(b1[1:], b2[1:], nb)
     sel = np.maximum(ma, mb) > 0.5 * max(ma.max(), mb.max())
     ratio = float(np.median(ub[sel] / np.maximum(ua[sel], 1e-300)))
-    print(f"nsplit 8 vs 1: {n} voxels, {f2:.3f} / {f3:.3f}; median uncertainty ratio {ratio:.3f} "
+    print(f"nsplit 8 vs 1: {fmt_agreement(ag)}; median uncertainty ratio {ratio:.3f} "
           f"(charged events {sb.events_charged / max(sa.events_charged, 1):.2f}x)")
-    assert f2 >= 0.88 and f3 >= 0.97, (f2, f3)
+    assert_agreement(ag)
     assert ratio < 0.85
     assert abs(mb[sel].sum() / ma[sel].sum() - 1.0) < 0.01
 
@@ -134,9 +135,9 @@ def test_event_split_largest_factor_fits_the_pools():
     h1, h2, sh = _run(c, ph, name, 20000, 10, precision=omc.FP32, form=omc.FORM_EVENT, seed=42, nsplit=1)
     c.close()
     assert st.errors == 0 and sh.errors == 0
-    f2, f3, n = _agreement(g1, g2, h1, h2, 10)
-    print(f"nsplit 64 vs 1: {n} voxels, {f2:.3f} / {f3:.3f}; peak in flight {st.peak_inflight}")
-    assert f2 >= 0.85 and f3 >= 0.95, (f2, f3)
+    ag = _agreement(g1, g2, h1, h2, 10)
+    print(f"nsplit 64 vs 1: {fmt_agreement(ag)}; peak in flight {st.peak_inflight}")
+    assert_agreement(ag)
 
 
 @pytest.mark.parametrize("ns", [4, 12])
@@ -158,11 +159,10 @@ def test_dij_split_event_matches_history_form(tmp_path, ns):
     pd_h, pv_h = plain[2]
     pd_e, pv_e = event[2]
     sel = pd_h > 0.5 * pd_h.max()
-    z = np.abs(pd_h[sel] - pd_e[sel]) / np.sqrt(pv_h[sel] + pv_e[sel])
-    print(f"Dij nsplit {ns}, history vs event plan dose: {sel.sum()} voxels, {np.mean(z <= 2):.3f} / "
-          f"{np.mean(z <= 3):.3f}")
-    assert sel.sum() > 20
-    assert np.mean(z <= 2.0) >= 0.85 and np.mean(z <= 3.0) >= 0.97
+    ag = z_agreement(pd_h[sel], np.sqrt(pv_h[sel]), pd_e[sel], np.sqrt(pv_e[sel]), 5)
+    print(f"Dij nsplit {ns}, history vs event plan dose: {fmt_agreement(ag)}")
+    assert ag["n"] > 20
+    assert_agreement(ag)
 
 
 def test_history_form_deep_splitting_matches_event_form():
@@ -180,6 +180,6 @@ def test_history_form_deep_splitting_matches_event_form():
             _run(c, ph, name, 100, 2, precision=omc.FP32, form=form, nsplit=65)
     c.close()
     assert sh.errors == 0 and se.errors == 0
-    f2, f3, n = _agreement(h1, h2, e1, e2, nb)
-    print(f"nsplit 20, history vs event: {n} voxels, {f2:.3f} / {f3:.3f}")
-    assert f2 >= 0.88 and f3 >= 0.97, (f2, f3)
+    ag = _agreement(h1, h2, e1, e2, nb)
+    print(f"nsplit 20, history vs event: {fmt_agreement(ag)}")
+    assert_agreement(ag)
