@@ -142,8 +142,9 @@ typedef struct omc_run_opts {
                                OMC_ERR_LIMIT (0 = 10^8) */
     int csc_on_device;      /* Dij: 1 = leave the matrices in device memory (omc_csc.device = 1) for a
                                consumer on the GPU; 0 = host arrays, as the mex interface needs */
-    int graph;              /* event form: 0 = each job's chunks of rounds replay a captured HIP graph
-                               (eager launches when count_events times the kernels); -1 = eager launches */
+    int graph;              /* event form: 0 = each job's chunks of rounds replay a HIP graph of its first
+                               chunk (eager launches when count_events times the kernels); -1 = eager
+                               launches */
 } omc_run_opts;
 
 /* Kernel classes of omc_run_stats.kernel_seconds / kernel_launches */
@@ -183,6 +184,9 @@ typedef struct omc_run_stats {
      * which at least one launch of the class ran (launches of concurrent
      * streams overlap, so kernel_seconds, their sum, can exceed it) */
     double kernel_busy_seconds[5];
+    /* event form: chunks of rounds enqueued by replaying a HIP graph, and
+     * jobs whose graph the runtime refused (they ran eager launches) */
+    uint64_t graph_replays, graph_fallbacks;
 } omc_run_stats;
 
 /* Defaults: FP32, event form, Philox, seed 1802, nsplit 1, automatic launch. */
@@ -318,6 +322,7 @@ typedef struct omc_result {
     omc_csc dij;                      /* OMC_REQ_DIJ */
     double submit_t, start_t, end_t;  /* seconds on the server clock */
     double histories;
+    char message[256];                /* OMC_REQ_FAILED: what failed (the worker thread's omc_last_error) */
 } omc_result;
 
 /* hbm_budget_bytes 0: 3/4 of the free device memory, at most 96 GiB */

@@ -76,7 +76,8 @@ class RunStats(C.Structure):
                 ("group_size", C.c_int64), ("nnz", C.c_int64), ("batches_per_job", C.c_int64),
                 ("streams", C.c_int64), ("e_source", C.c_double), ("e_escape", C.c_double),
                 ("e_lost", C.c_double), ("e_deposit", C.c_double),
-                ("kernel_busy_seconds", C.c_double * 5)]
+                ("kernel_busy_seconds", C.c_double * 5), ("graph_replays", C.c_uint64),
+                ("graph_fallbacks", C.c_uint64)]
 
     def energy_residual(self):
         """(deposit + escape + lost - source) / source of the run"""
@@ -125,7 +126,7 @@ class Request(C.Structure):
 class Result(C.Structure):
     _fields_ = [("state", C.c_int), ("error", C.c_int), ("sum", _d), ("sum2", _d), ("nreg", C.c_int64),
                 ("dij", CSC), ("submit_t", C.c_double), ("start_t", C.c_double), ("end_t", C.c_double),
-                ("histories", C.c_double)]
+                ("histories", C.c_double), ("message", C.c_char * 256)]
 
 
 _lib = None
@@ -551,7 +552,7 @@ class Server:
             if res.dij.jc:
                 out["dij"] = res.dij.to_numpy()
         else:
-            out["message"] = lib().omc_last_error().decode()
+            out["message"] = res.message.decode(errors="replace")
         lib().omc_result_free(C.byref(res))
         return out
 

@@ -132,6 +132,7 @@ struct JobStats {
     // time the device ran a class at all.
     hipEvent_t origin = nullptr;
     std::vector<std::pair<float, float>> iv[4];
+    uint64_t graph_replays = 0, graph_fallbacks = 0;
 };
 
 // Seconds covered by the union of the intervals of one kernel class over
@@ -160,7 +161,7 @@ double busy_seconds(const std::vector<const JobStats *> &jobs, int k) {
 // With `timing`, every kernel of every round is bracketed by events on the
 // job's stream and its duration is added to the job statistics when the
 // chunk has completed.  Without it and with L.graph, the first chunk's
-// kernels are captured into a HIP graph and every chunk of the job replays
+// kernels are built into a HIP graph and every chunk of the job replays
 // it: a round's kernels depend on the round only through its parity, and
 // chunks start at even rounds, so every chunk is the same sequence of
 // launches.  A replay is one runtime call instead of 4 K launches.
@@ -178,7 +179,7 @@ int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int 
             JobStats &js) {
     constexpr int K = 4;
     static_assert(K % 2 == 0, "chunks must start at even rounds");
-    const bool use_graph = L.graph && !timing;
+    bool use_graph = L.graph && !timing;
     GraphExec gx;
     // With photon splitting one split puts up to 2 x nsplit charged particles
     // into the queues: a round injects at most target / nsplit primaries, so
@@ -193,20 +194,25 @@ int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int 
     Events kev(timing ? 2 * K * 5 : 0);
     auto enqueue = [&](long r0, int slot) -> hipError_t {
         hipError_t x;
-        if (use_graph) {
-            if (!gx.x) {
-                // capture in this thread's mode: the lanes of a Dij run
-                // enqueue from threads of their own meanwhile
-                if ((x = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) return x;
-                hipError_t xl = hipSuccess;
-                for (int j = 0; j < K && xl == hipSuccess; j++)
-                    for (int part = 0; part < 4 && xl == hipSuccess; part++) xl = launch_round_part(L, r0 + j, part, count, st);
-                x = hipStreamEndCapture(st, &gx.g);
-                if (xl != hipSuccess) return xl;
-                if (x != hipSuccess) return x;
-                if ((x = hipGraphInstantiate(&gx.x, gx.g, nullptr, nullptr, 0)) != hipSuccess) return x;
+        if (use_graph && !gx.x) {
+            // the graph is built node by node rather than captured from the
+            // stream: the lanes of a Dij run and the requests of a server
+            // enqueue, copy and allocate from threads of their own meanwhile,
+            // and a capture invalidated by such a call (measured: the
+            // synchronous phantom upload of a concurrent request) leaves the
+            // stream unusable.  A graph the runtime refuses leaves the job to
+            // eager launches.
+            bool ok = build_rounds_graph(L, r0, K, count, &gx.g) == hipSuccess &&
+                      hipGraphInstantiate(&gx.x, gx.g, nullptr, nullptr, 0) == hipSuccess;
+            if (!ok) {
+                (void)hipGetLastError();
+                use_graph = false;
+                js.graph_fallbacks++;
             }
+        }
+        if (use_graph) {
             if ((x = hipGraphLaunch(gx.x, st)) != hipSuccess) return x;
+            js.graph_replays++;
         } else {
             for (int j = 0; j < K; j++) {
                 hipEvent_t *e = timing ? &kev.ev[(size_t)(slot * K + j) * 5] : nullptr;
@@ -354,6 +360,8 @@ void stats_from_job(omc_run_stats *st, const JobStats &js) {
     st->events_photon += js.ev_photon;
     st->events_charged += js.ev_charged;
     st->rounds += js.rounds;
+    st->graph_replays += js.graph_replays;
+    st->graph_fallbacks += js.graph_fallbacks;
     st->peak_inflight = std::max<uint64_t>(st->peak_inflight, js.peak);
 }
 
@@ -1387,6 +1395,7 @@ void run_request(omc_server *s, omc_server::Req *q) {
     if (st) (void)hipStreamDestroy(st);
     std::lock_guard<std::mutex> lk(s->mu);
     res.error = rc;
+    if (rc) snprintf(res.message, sizeof res.message, "%s", omc_host_last_error());
     res.state = rc ? OMC_REQ_FAILED : OMC_REQ_DONE;
     res.histories = stats.histories;
     res.end_t = s->now();

@@ -53,6 +53,7 @@ template hipError_t launch_history<float>(const HistArgs<float> &, int, bool, in
 template hipError_t launch_sampler<float>(const SampArgs<float> &, hipStream_t);
 template hipError_t launch_rounds<float>(const EvLaunch<float> &, long, int, bool, hipStream_t);
 template hipError_t launch_round_part<float>(const EvLaunch<float> &, long, int, bool, hipStream_t);
+template hipError_t build_rounds_graph<float>(const EvLaunch<float> &, long, int, bool, hipGraph_t *);
 
 hipError_t launch_job_start(EvCtl *c, long long nhist, long long target, long long cap, long long inj_max, int dest,
                             hipStream_t st) {

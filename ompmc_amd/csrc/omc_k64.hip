@@ -8,6 +8,7 @@ template hipError_t launch_history<double>(const HistArgs<double> &, int, bool, 
 template hipError_t launch_sampler<double>(const SampArgs<double> &, hipStream_t);
 template hipError_t launch_rounds<double>(const EvLaunch<double> &, long, int, bool, hipStream_t);
 template hipError_t launch_round_part<double>(const EvLaunch<double> &, long, int, bool, hipStream_t);
+template hipError_t build_rounds_graph<double>(const EvLaunch<double> &, long, int, bool, hipGraph_t *);
 
 }  // namespace omc
 

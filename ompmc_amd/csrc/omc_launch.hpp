@@ -36,7 +36,7 @@ template <class R> struct EvLaunch {
     int max_age;
     int delta;
     int nsplit;                  // photon splitting factor; > 1 transports photons with k_event_split
-    int graph;                   // 1: untimed chunks of rounds replay a captured HIP graph
+    int graph;                   // 1: untimed chunks of rounds replay a HIP graph
     SrcArgs<R> src;              // source of the job (ctl and out are set per round)
     int grid_event, grid_source;
 };
@@ -45,6 +45,9 @@ template <class R> struct EvLaunch {
 template <class R> hipError_t launch_rounds(const EvLaunch<R> &L, long r0, int nround, bool count, hipStream_t st);
 // One part of round r: 0 source, 1 photon events, 2 charged events, 3 round end.
 template <class R> hipError_t launch_round_part(const EvLaunch<R> &L, long r, int part, bool count, hipStream_t st);
+// The same kernels as a graph, nodes added one by one in stream order.
+template <class R> hipError_t build_rounds_graph(const EvLaunch<R> &L, long r0, int nround, bool count,
+                                                 hipGraph_t *out);
 hipError_t launch_job_start(EvCtl *c, long long nhist, long long target, long long cap, long long inj_max, int dest,
                             hipStream_t st);
 

@@ -3,6 +3,8 @@
 #pragma once
 #include "omc_launch.hpp"
 
+#include <cstring>
+
 namespace omc {
 
 template <class R> hipError_t launch_history(const HistArgs<R> &a, int rng, bool count, int grid, int block,
@@ -44,16 +46,42 @@ template <class R> hipError_t launch_sampler(const SampArgs<R> &a, hipStream_t s
     return hipGetLastError();
 }
 
-template <class R> hipError_t launch_round_part(const EvLaunch<R> &L, long r, int part, bool count, hipStream_t st) {
+// One kernel of round r: its function, shape and by-value arguments.
+// Part 0 samples the round's primaries, 1 transports photons, 2 charged
+// particles, 3 ends the round.  Arguments depend on r only through its
+// parity, which selects the input and output pools.
+template <class R> struct RoundCall {
+    const void *fn = nullptr;
+    dim3 grid, block;
+    SrcArgs<R> src;
+    EvArgs<R> ev;
+    EvCtl *ctl = nullptr;
+    int par = 0;
+    void *args[2] = {nullptr, nullptr};
+    int part = 0;
+    // kernelParams of a graph node: pointers to the argument values above
+    void **bind() {
+        if (part == 0) args[0] = &src;
+        else if (part == 3) { args[0] = &ctl; args[1] = &par; }
+        else args[0] = &ev;
+        return args;
+    }
+};
+
+template <class R> void round_call(const EvLaunch<R> &L, long r, int part, bool count, RoundCall<R> &c) {
     const int par = (int)(r & 1), nx = 1 - par;
+    c.part = part;
+    c.block = dim3(256);
     if (part == 0) {
-        SrcArgs<R> s = L.src;
+        SrcArgs<R> &s = c.src;
+        s = L.src;
         s.ctl = L.ctl;
         s.ebal = L.ebal;
         s.out = L.pool[par][L.src.dos.charge != 0 && !L.src.use_beamlets ? 1 : 0];
-        hipLaunchKernelGGL((k_source<R>), dim3(L.grid_source), dim3(256), 0, st, s);
+        c.fn = reinterpret_cast<const void *>(&k_source<R>);
+        c.grid = dim3(L.grid_source);
     } else if (part == 1 || part == 2) {
-        EvArgs<R> a;
+        EvArgs<R> &a = c.ev;
         a.t = L.t;
         a.g = L.g;
         a.seed = L.seed;
@@ -76,20 +104,80 @@ template <class R> hipError_t launch_round_part(const EvLaunch<R> &L, long r, in
         a.in = L.pool[par][part - 1];
         a.n_in = &L.ctl->cnt[par][part - 1];
         a.claim = &L.ctl->claim[par][part - 1];
-        if (part == 1 && a.nsplit > 1) {
-            if (count) hipLaunchKernelGGL((k_event_split<R, true>), dim3(L.grid_event), dim3(256), 0, st, a);
-            else hipLaunchKernelGGL((k_event_split<R, false>), dim3(L.grid_event), dim3(256), 0, st, a);
-        } else if (part == 1) {
-            if (count) hipLaunchKernelGGL((k_event<R, true, true>), dim3(L.grid_event), dim3(256), 0, st, a);
-            else hipLaunchKernelGGL((k_event<R, true, false>), dim3(L.grid_event), dim3(256), 0, st, a);
-        } else {
-            if (count) hipLaunchKernelGGL((k_event<R, false, true>), dim3(L.grid_event), dim3(256), 0, st, a);
-            else hipLaunchKernelGGL((k_event<R, false, false>), dim3(L.grid_event), dim3(256), 0, st, a);
-        }
+        if (part == 1 && a.nsplit > 1)
+            c.fn = count ? reinterpret_cast<const void *>(&k_event_split<R, true>)
+                         : reinterpret_cast<const void *>(&k_event_split<R, false>);
+        else if (part == 1)
+            c.fn = count ? reinterpret_cast<const void *>(&k_event<R, true, true>)
+                         : reinterpret_cast<const void *>(&k_event<R, true, false>);
+        else
+            c.fn = count ? reinterpret_cast<const void *>(&k_event<R, false, true>)
+                         : reinterpret_cast<const void *>(&k_event<R, false, false>);
+        c.grid = dim3(L.grid_event);
     } else {
-        hipLaunchKernelGGL(k_round_end, dim3(1), dim3(64), 0, st, L.ctl, par);
+        c.ctl = L.ctl;
+        c.par = par;
+        c.fn = reinterpret_cast<const void *>(&k_round_end);
+        c.grid = dim3(1);
+        c.block = dim3(64);
+    }
+}
+
+template <class R> hipError_t launch_round_part(const EvLaunch<R> &L, long r, int part, bool count, hipStream_t st) {
+    RoundCall<R> c;
+    round_call(L, r, part, count, c);
+    if (part == 0) {
+        hipLaunchKernelGGL((k_source<R>), c.grid, c.block, 0, st, c.src);
+    } else if (part == 1 || part == 2) {
+        if (c.fn == reinterpret_cast<const void *>(&k_event_split<R, true>))
+            hipLaunchKernelGGL((k_event_split<R, true>), c.grid, c.block, 0, st, c.ev);
+        else if (c.fn == reinterpret_cast<const void *>(&k_event_split<R, false>))
+            hipLaunchKernelGGL((k_event_split<R, false>), c.grid, c.block, 0, st, c.ev);
+        else if (c.fn == reinterpret_cast<const void *>(&k_event<R, true, true>))
+            hipLaunchKernelGGL((k_event<R, true, true>), c.grid, c.block, 0, st, c.ev);
+        else if (c.fn == reinterpret_cast<const void *>(&k_event<R, true, false>))
+            hipLaunchKernelGGL((k_event<R, true, false>), c.grid, c.block, 0, st, c.ev);
+        else if (c.fn == reinterpret_cast<const void *>(&k_event<R, false, true>))
+            hipLaunchKernelGGL((k_event<R, false, true>), c.grid, c.block, 0, st, c.ev);
+        else
+            hipLaunchKernelGGL((k_event<R, false, false>), c.grid, c.block, 0, st, c.ev);
+    } else {
+        hipLaunchKernelGGL(k_round_end, c.grid, c.block, 0, st, c.ctl, c.par);
     }
     return hipGetLastError();
+}
+
+// A graph of rounds r0 .. r0+nround-1, built node by node (no stream
+// capture, so enqueues and allocations of other threads cannot disturb it):
+// each kernel depends on the one before, as on a stream.
+template <class R> hipError_t build_rounds_graph(const EvLaunch<R> &L, long r0, int nround, bool count,
+                                                 hipGraph_t *out) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipGraphCreate(&g, 0);
+    if (e != hipSuccess) return e;
+    hipGraphNode_t prev = nullptr;
+    for (long r = r0; r < r0 + nround && e == hipSuccess; r++)
+        for (int part = 0; part < 4 && e == hipSuccess; part++) {
+            RoundCall<R> c;
+            round_call(L, r, part, count, c);
+            hipKernelNodeParams p;
+            memset(&p, 0, sizeof p);
+            p.func = const_cast<void *>(c.fn);
+            p.gridDim = c.grid;
+            p.blockDim = c.block;
+            p.sharedMemBytes = 0;
+            p.kernelParams = c.bind();
+            p.extra = nullptr;
+            hipGraphNode_t n = nullptr;
+            e = hipGraphAddKernelNode(&n, g, prev ? &prev : nullptr, prev ? 1 : 0, &p);
+            prev = n;
+        }
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return e;
+    }
+    *out = g;
+    return hipSuccess;
 }
 
 template <class R> hipError_t launch_rounds(const EvLaunch<R> &L, long r0, int nround, bool count, hipStream_t st) {

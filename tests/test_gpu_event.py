@@ -296,11 +296,11 @@ def test_unequal_voxel_spacing_agrees_with_oracle(tmp_path):
 
 def test_graph_replay_matches_eager_launches(ct):
     """Without kernel timing the event form enqueues each chunk of rounds by
-    replaying a HIP graph captured from the job's first chunk.  Several dose
+    replaying a HIP graph built from the job's first chunk.  Several dose
     and Dij runs in a row, graph and eager alternating (eager work and
     synchronisations between the replays), each graph run against the eager
     result to FP64 summation order; the Dij runs use three concurrent lanes,
-    each capturing on its own thread."""
+    each building and replaying its graphs on its own thread."""
     c, ph = ct
     spec = omc.Spectrum.read(omc.os.path.join(omc.DATA, "spectra", "mohan6.spectrum"))
     kw = dict(collimator=(-3, 3, -3, 3), precision=omc.FP32, form=omc.FORM_EVENT, seed=77, batches_per_job=2)
@@ -308,6 +308,7 @@ def test_graph_replay_matches_eager_launches(ct):
     for rep in range(3):
         g1, g2, st = c.run_dosxyz(ph, spec, 20000, 6, graph=True, **kw)
         assert st.errors == 0 and st.rounds > 4
+        assert st.graph_replays > 0 and st.graph_fallbacks == 0
         assert np.allclose(g1, e1, rtol=1e-9, atol=1e-12 * e1.max()), rep
         assert np.allclose(g2, e2, rtol=1e-9, atol=1e-12 * e2.max()), rep
         c.run_dosxyz(ph, spec, 5000, 2, graph=False, **kw)          # eager work between replays
@@ -317,5 +318,6 @@ def test_graph_replay_matches_eager_launches(ct):
     for rep in range(2):
         (jc2, ir2, val2), _, _, st = c.run_dij(ph, spec, bix, graph=True, **dk)
         assert st.errors == 0 and st.streams == 3
+        assert st.graph_replays > 0 and st.graph_fallbacks == 0
         assert np.array_equal(jc, jc2) and np.array_equal(ir, ir2), rep
         assert np.allclose(val, val2, rtol=1e-9), rep
