@@ -269,6 +269,24 @@ int omc_dij_assemble(omc_ctx *ctx, int phantom_id, int nbixels, int nbatch, cons
                      const double *endep, double rel_threshold, const omc_run_opts *opts, omc_csc *dij, omc_csc *var,
                      omc_run_stats *stats);
 
+/* No counterpart in the reference: omc_run_dosxyz in the history form for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Every buffer the run
+ * needs beyond its inputs and outputs (the batch tally, the source's inverse
+ * CDF, counters, error flags) lies in one fixed array of
+ * OMC_NOALLOC_SCRATCH_BYTES that the library holds on each device it runs on
+ * until the process ends; the call allocates nothing.  sum and sum2 are
+ * DEVICE arrays of nvox + 1 doubles on the context's device that receive the
+ * batch sums (as omc_run_dosxyz's host arrays do).  Limits: nvox + 1 <=
+ * OMC_NOALLOC_MAX_REGIONS, form OMC_FORM_HISTORY, Philox random numbers,
+ * nsplit <= 8; other requests fail with OMC_ERR_ARG.  Calls that overlap on
+ * one device share the array and race; the phantom and the context's tables
+ * are uploaded beforehand by the ordinary (allocating) calls. */
+#define OMC_NOALLOC_MAX_REGIONS 4194304
+#define OMC_NOALLOC_SCRATCH_BYTES 33574656
+int omc_run_dosxyz_noalloc(omc_ctx *ctx, int phantom_id, const omc_dosxyz_source_desc *src, int64_t nperbatch,
+                           int nbatch, const omc_run_opts *opts, double *sum, double *sum2, omc_run_stats *stats);
+
 /* Injected-random-number sampler evaluation for verification: ncase cases of
  * the golden-vector layout (kinds 1..16: compton, pair, photo, rayleigh,
  * brems, moller, bhabha, annih, rannih, msdist, mscat, sscat, drange, eloss,

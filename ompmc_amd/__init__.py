@@ -30,6 +30,7 @@ REPO = os.path.dirname(HERE)
 DATA = os.path.join(REPO, "data")
 
 FP32, FP64 = 32, 64
+NOALLOC_MAX_REGIONS = 4194304    # OMC_NOALLOC_MAX_REGIONS
 FORM_HISTORY, FORM_EVENT = 0, 1
 RNG_PHILOX, RNG_REPLAY = 0, 1
 INVDIM = 1000
@@ -160,6 +161,8 @@ def lib():
     L.omc_run_opts_default.argtypes = [C.POINTER(RunOpts)]
     L.omc_run_dosxyz.argtypes = [vp, C.c_int, C.POINTER(DosxyzSourceDesc), C.c_int64, C.c_int,
                                  C.POINTER(RunOpts), _d, _d, C.POINTER(RunStats)]
+    L.omc_run_dosxyz_noalloc.argtypes = [vp, C.c_int, C.POINTER(DosxyzSourceDesc), C.c_int64, C.c_int,
+                                         C.POINTER(RunOpts), vp, vp, C.POINTER(RunStats)]
     L.omc_dosxyz_dose.argtypes = [vp, C.c_int, _d, _d, C.c_int64, C.c_int, _d, _d]
     L.omc_write_3ddose_file.argtypes = [vp, C.c_int, C.c_char_p, _d, _d]
     L.omc_sampler_eval.argtypes = [vp, C.c_int, C.c_int, C.c_int, _i, _d, C.c_int, _u32, _i64, _d, C.c_int, _i64]
@@ -357,6 +360,25 @@ class Context:
             _check(rc)
         del keep
         return s, s2, st
+
+    def run_dosxyz_noalloc(self, phantom: Phantom, spectrum: Spectrum, nperbatch, nbatch, d_sum, d_sum2,
+                           collimator=(-5, 5, -5, 5), ssd=1.0e10, charge=0, **opts):
+        """omc_run_dosxyz_noalloc: the history-form dose run with no device
+        allocation; d_sum and d_sum2 are device addresses (int) of nreg
+        doubles on the context's device that receive the batch sums.
+        Returns the run statistics."""
+        src = DosxyzSourceDesc()
+        src.spec = spectrum.desc()
+        src.charge, src.ssd = charge, ssd
+        for k in range(4):
+            src.collimator[k] = collimator[k]
+        opts.setdefault("form", FORM_HISTORY)
+        o, keep = run_opts(**opts)
+        st = RunStats()
+        _check(lib().omc_run_dosxyz_noalloc(self.h, phantom.id, C.byref(src), nperbatch, nbatch, C.byref(o),
+                                            C.c_void_p(d_sum), C.c_void_p(d_sum2), C.byref(st)))
+        del keep
+        return st
 
     def dose(self, phantom: Phantom, s, s2, nperbatch, nbatch):
         nv = phantom.nreg - 1

@@ -26,6 +26,7 @@ HIP_UNITS = {
     "omc_k64.hip": ["-ffp-contract=off"],
     "omc_capi.hip": [],
     "omc_event.hip": [],
+    "omc_noalloc.hip": [],
 }
 C_UNITS = {"omc_host.c": ["-O2", "-fPIC", "-ffp-contract=off", "-std=gnu11", "-Wall"]}
 

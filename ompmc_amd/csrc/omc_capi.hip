@@ -292,6 +292,8 @@ extern "C" int omc_ctx_create(omc_ctx **out, int nmed, const char *const *media,
     omc_ctx *c = new omc_ctx();
     c->device = device;
     if (hipSetDevice(device) != hipSuccess) { delete c; return omc_seterr(OMC_ERR_DEVICE, "hipSetDevice(%d)", device); }
+    int rc0 = omc_noalloc_init(device);      // the fixed scratch array of omc_run_dosxyz_noalloc on this device
+    if (rc0) { delete c; return rc0; }
     char names[OMC_MXMED][60];
     for (int i = 0; i < nmed; i++) snprintf(names[i], 60, "%s", media[i]);
     omc_paths p = {pegs_file, pgs4form_file, data_folder};

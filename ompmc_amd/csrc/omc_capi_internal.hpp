@@ -98,6 +98,8 @@ template <> inline const omc::Geom<float> *geom_of<float>(const PhantomDev *p) {
 template <> inline const omc::Geom<double> *geom_of<double>(const PhantomDev *p) { return p->dg64; }
 
 int omc_seterr(int code, const char *fmt, ...);
+// device address of the no-allocation scratch array on `device` (current), once per device
+int omc_noalloc_init(int device);
 int omc_check_opts(const omc_run_opts *o);
 PhantomDev *omc_get_phantom(omc_ctx *c, int id);
 // cache_stream: the phantom's device memory comes from the block cache and is
