@@ -150,7 +150,7 @@ def serve_mix(ctx, rate_hist_per_s, nreq=12, load=0.8, seed=20261206, precision=
     ids = []
     t0 = time.perf_counter()
     t_next = t0
-    for (kind, ph, nh, kw), gap in zip(reqs, gaps):
+    for k, ((kind, ph, nh, kw), gap) in enumerate(zip(reqs, gaps)):
         t_next += gap
         dt = t_next - time.perf_counter()
         if dt > 0:
@@ -159,7 +159,13 @@ def serve_mix(ctx, rate_hist_per_s, nreq=12, load=0.8, seed=20261206, precision=
             ids.append(srv.submit_dose(ph, spec, max(1, nh // 10), 10, precision=precision, **kw))
         else:
             ids.append(srv.submit_dij(ph, spec, kw["bix"], nbatch=10, precision=precision))
-    res = [srv.wait(i) for i in ids]
+        if (k + 1) % 20 == 0:
+            log(f"served mix at load {load}: {k + 1} of {nreq} submitted")
+    res = []
+    for k, i in enumerate(ids):
+        res.append(srv.wait(i))
+        if (k + 1) % 20 == 0:
+            log(f"served mix at load {load}: {k + 1} of {nreq} done")
     wall = time.perf_counter() - t0
     srv.close()
     bad = [r for r in res if r["error"]]
@@ -219,7 +225,7 @@ def cpu_baseline_c3(ph, bix, spec, nbatch, seconds_target=12.0):
                     host={"cpu_model": cpu_model(), "nproc": os.cpu_count(), "threads_allowed": cores})
 
 
-def cpu_cross_check(cores, nhist=200000):
+def cpu_cross_check(cores, nhist=1000000):
     """The reference's own omc_dosxyz, compiled from its sources into
     oracle/_ref, against the oracle restatement on the same input: the 60^3
     water cube with 2 MeV photons in a 10x10 cm field, nhist histories in 10
@@ -406,22 +412,51 @@ def bench_c3(a):
                               "speedup_vs_plain": rb / rp,
                               "plan_seconds_batched": hist / a.steps * k / rb}
         log(f"at 1% unc (x{k} histories): batched {rb:.4g}, plain {rp:.4g}, ratio {rb / rp:.1f}")
+    # energy balance of the last timed step (FP64 tallies): primaries' energy
+    # against deposits in every region, photons that left and dropped energy
+    out["energy_balance"] = {"source_MeV": st_last.e_source, "deposit_MeV": st_last.e_deposit,
+                             "escape_MeV": st_last.e_escape, "lost_MeV": st_last.e_lost,
+                             "residual": st_last.energy_residual()}
+    # the same step without kernel timing, its rounds replayed from HIP graphs
+    # against eager launches (the timed steps above launch eagerly: the HIP
+    # events around every kernel cannot sit in a graph)
+    modes = {}
+    for g in (True, False, True, False):
+        t0 = time.perf_counter()
+        _, _, _, stg = ctx.run_dij(gph, spec, bix, nbatch=nbatch, rel_threshold=rel, graph=g, **opts)
+        modes.setdefault("graph" if g else "eager", []).append(stg.histories / (time.perf_counter() - t0))
+        if g:
+            modes.setdefault("graph_replays", []).append(int(stg.graph_replays))
+    out["launch_modes"] = {"graph_histories_per_s": max(modes["graph"]), "eager_histories_per_s": max(modes["eager"]),
+                           "graph_replays_per_step": modes["graph_replays"][-1],
+                           "what": "untimed steps, best of two each; graph: each job's chunks of four rounds "
+                                   "(16 kernels) replay one HIP graph"}
+    log(f"launch modes: {out['launch_modes']}")
     if not a.no_fp64 and prec == omc.FP32:
-        # FP32 against FP64 on every 10th beamlet: three independent seeds per
+        # FP32 against FP64 on the whole plan: three independent seeds per
         # precision; region (voxels above 20% of the plan maximum) fixed by an
-        # FP64 pilot run with its own seed
-        sub = subset(bix, 10)
+        # FP64 pilot run with its own seed.  The FP64 runs also give the FP64
+        # throughput of the step.
         kw = dict(nbatch=nbatch, rel_threshold=rel, want_plan=True, form=form, group=a.group, csc_on_device=True,
                   download=False)
-        pilot = ctx.run_dij(gph, spec, sub, precision=omc.FP64, seed=70, **kw)[2][0]
+        pilot = ctx.run_dij(gph, spec, bix, precision=omc.FP64, seed=70, **kw)[2][0]
         reg = pilot > 0.2 * pilot.max()
-        sums = {p: np.array([ctx.run_dij(gph, spec, sub, precision=p, seed=71 + 3 * i + (p == omc.FP64), **kw)[2][0][reg].sum()
-                             for i in range(3)]) for p in (omc.FP32, omc.FP64)}
+        sums, rates = {}, {}
+        for p in (omc.FP32, omc.FP64):
+            v = []
+            for i in range(3):
+                t0 = time.perf_counter()
+                _, _, pl_, st_ = ctx.run_dij(gph, spec, bix, precision=p, seed=71 + 3 * i + (p == omc.FP64), **kw)
+                rates.setdefault(p, []).append(st_.histories / (time.perf_counter() - t0))
+                v.append(pl_[0][reg].sum())
+            sums[p] = np.array(v)
         d = sums[omc.FP32].mean() / sums[omc.FP64].mean() - 1.0
         sig = math.hypot(sums[omc.FP32].std(ddof=1), sums[omc.FP64].std(ddof=1)) / math.sqrt(3) / sums[omc.FP64].mean()
-        out["fp64_plan_dose_rel_diff"] = {"value": d, "sigma": sig, "beamlets": int(sub.n), "region": "> 20% of max",
-                                          "seeds": 3}
-        log(f"fp32/fp64: {d:+.3e} +- {sig:.1e}")
+        out["fp64_plan_dose_rel_diff"] = {"value": d, "sigma": sig, "beamlets": int(bix.n), "region": "> 20% of max",
+                                          "seeds": 3, "voxels": int(reg.sum())}
+        out["fp64"] = {"histories_per_s": float(np.median(rates[omc.FP64])),
+                       "what": "the whole plan in FP64 transport, matrices in HBM, no kernel timing, median of 3"}
+        log(f"fp32/fp64: {d:+.3e} +- {sig:.1e}; fp64 {out['fp64']['histories_per_s']:.4g} histories/s")
     if not a.no_split and form == omc.FORM_EVENT:
         # photon splitting (the reference's nsplit): the same beamlets and
         # histories at several splitting factors; efficiency = 1 / (plan dose

@@ -306,7 +306,11 @@ template <class R, class G, class T> struct Engine {
     Counters cnt;
     bool count;
     int nsplit = 1;                 // photon splitting factor (vrt.nsplit)
-    int aborted = 0;                // set when a particle exceeded an iteration guard (it is dropped)
+    // set when a particle exceeded an iteration guard (it is dropped): 1
+    // photon flight, 2 electron pass, 3 electron step, 4 rejection loop (the
+    // source line of the loop in abort_site)
+    int aborted = 0;
+    int abort_site = 0;
     double esc = 0.0;               // wt * E of photons that left the phantom (energy balance)
     double lost = 0.0;              // energy dropped without a deposit (energy balance)
 
@@ -317,12 +321,37 @@ template <class R, class G, class T> struct Engine {
         return (double)p.wt * ((double)p.e + (p.iq > 0 ? RMD : -RMD));
     }
 
+    // Every rejection loop is bounded: past 2^20 trials of one loop (every
+    // sampler here accepts a trial with a probability of a few per cent at
+    // least) the particle counts as stuck; `aborted` is set, every loop then
+    // ends at its next test, and the caller drops the particle and reports it
+    // like the step guards.  No kernel can spin forever on a sampler.
+    static constexpr int MAX_TRIALS = 1 << 20;
+    __device__ __forceinline__ bool stuck(int &n, int site = __builtin_LINE()) {
+        if (++n > MAX_TRIALS && !aborted) {
+            aborted = 4;
+            abort_site = site;
+        }
+        return aborted != 0;
+    }
+
     __device__ __forceinline__ R rnd() {
         if (count) cnt.nrn++;
         return rng.template next<R>();
     }
 
     static __device__ __forceinline__ int pwlf_i(R lv, R c1, R c0) { return (int)(lv * c1 + c0); }
+    // Photon table interval of a log energy, at least the first: a photon
+    // just above PCUT = AP sits at the lower edge of the tables, where
+    // float rounding of lv * c1 + c0 can give 0.99999994 instead of 1 and
+    // the reference's formula an interval of -1 -- a read before the table
+    // (measured: one photon in ~6e7 FP32 histories, whose garbage Rayleigh
+    // fraction then kept it scattering until a guard dropped it).  In FP64
+    // the formula stays at 0 there, so the clamp changes nothing.
+    static __device__ __forceinline__ int photon_interval(R gle, const Med<R> &m) {
+        const int i = pwlf_i(gle, m.ge1, m.ge0) - 1;
+        return i < 0 ? 0 : i;
+    }
     static __device__ __forceinline__ R pwlf(const PW<R> &q, R lv) { return lv * q.c1 + q.c0; }
     static __device__ __forceinline__ R pwlf(const PW<R> *tab, R lv) {
         PW<R> q = gld(tab);
@@ -410,6 +439,7 @@ template <class R, class G, class T> struct Engine {
     // box-rejection azimuth (src/ompmc.c:102-122)
     __device__ __forceinline__ void azimuth(R &c, R &sn) {
         R xphi, xphi2, yphi, yphi2, r2;
+        int trials = 0;
         do {
             xphi = rnd();
             xphi = (R)2.0 * xphi - (R)1.0;
@@ -417,7 +447,7 @@ template <class R, class G, class T> struct Engine {
             yphi = rnd();
             yphi2 = yphi * yphi;
             r2 = xphi2 + yphi2;
-        } while (r2 > (R)1.0);
+        } while ((r2 > (R)1.0) && !stuck(trials));
         r2 = (R)1 / r2;
         c = (xphi2 - yphi2) * r2;
         sn = (R)2.0 * xphi * yphi * r2;
@@ -468,8 +498,10 @@ template <class R, class G, class T> struct Engine {
         R pmax = pwlf(t.pmax + (imed * MXGE + lgle), gle);
         R xmax = (R)80.65506856998 * eig;
         R xv, costhe, csqthe, r0, r1;
+        int trials = 0;
         do {
             r1 = rnd();
+            int trials_in = 0;
             do {
                 r0 = rnd();
                 r0 *= pmax;
@@ -478,11 +510,11 @@ template <class R, class G, class T> struct Engine {
                     while (r0 >= t.ray_fcum[ib + 1]) ib++;
                 r0 = (r0 - t.ray_fcum[ib]) * t.ray_c[ib];
                 xv = t.ray_xgrid[ib] * M<R>::exp(M<R>::log((R)1.0 + r0) * t.ray_b[ib]);
-            } while (xv >= xmax);
+            } while ((xv >= xmax) && !stuck(trials_in));
             xv /= eig;
             costhe = (R)1.0 - (R)0.000307444456 * (xv * xv);
             csqthe = costhe * costhe;
-        } while ((R)2.0 * r1 >= ((R)1.0 + csqthe));
+        } while (((R)2.0 * r1 >= ((R)1.0 + csqthe)) && !stuck(trials));
         R sinthe = M<R>::sqrt((R)1.0 - csqthe);
         Uphi q;
         uphi21(p, q, costhe, sinthe);
@@ -532,6 +564,7 @@ template <class R, class G, class T> struct Engine {
                 aux = m.bpar1 * ((R)1.0 - m.bpar0 * RM / eig);
             }
             R eavail = eig - (R)2.0 * RM, rejf, rejmax, r4;
+            int trials = 0;
             do {
                 R br, r0 = rnd(), r1 = rnd();
                 r4 = rnd();
@@ -550,7 +583,7 @@ template <class R, class G, class T> struct Engine {
                 delta = (eig * m.delcm) / (ese2 * ese1);
                 if (delta < (R)1.0) rejf = m.dl1[l1] + delta * (m.dl2[l1] + delta * m.dl3[l1]);
                 else rejf = m.dl4[l1] + m.dl5[l1] * M<R>::log(delta + m.dl6[l1]);
-            } while (r4 * rejmax > rejf);
+            } while ((r4 * rejmax > rejf) && !stuck(trials));
             ese1 = eig - ese2;
             r4 = rnd();
             if (r4 < (R)0.5) { iq1 = -1; iq2 = 1; } else { iq1 = 1; iq2 = -1; }
@@ -578,13 +611,14 @@ template <class R, class G, class T> struct Engine {
             R rejtop = (R)1.0 * M<R>::fmax(rejmin, rejmid);
             R theta, rtest, rejfactor;
             // accepted when either test passes (reference quirk Q2)
+            int trials = 0;
             do {
                 R xitst = rnd();
                 R rejtst = pair_rej(m, xitst, esedei, eseder, tteig);
                 rtest = rnd();
                 theta = M<R>::sqrt((R)1.0 / xitst - (R)1.0) / ttese;
                 rejfactor = rejtst / rejtop;
-            } while ((rtest > rejfactor) && (theta >= (R)3.14159265358979323846));
+            } while (((rtest > rejfactor) && (theta >= (R)3.14159265358979323846)) && !stuck(trials));
             R sinthe = M<R>::sin(theta), costhe = M<R>::cos(theta);
             if (j == 0) {
                 uphi21(p0, q, costhe, sinthe);
@@ -604,6 +638,7 @@ template <class R, class G, class T> struct Engine {
         bool first = true;
         R sinthe = (R)0, costhe, br, aux, rejf3, temp = (R)0;
         R alph1 = (R)0, alph2 = (R)0, alpha = (R)0, rejmax = (R)0;
+        int trials = 0;
         do {
             if (ko > (R)2.0) {
                 if (first) {
@@ -612,6 +647,7 @@ template <class R, class G, class T> struct Engine {
                     alpha = alph1 + alph2;
                 }
                 R r3;
+                int trials_in = 0;
                 do {
                     R r1 = rnd(), r2 = rnd();
                     if (r1 * alpha < alph1) br = M<R>::exp(alph1 * r2) * bro;
@@ -621,10 +657,11 @@ template <class R, class G, class T> struct Engine {
                     aux = (R)1.0 + br * br;
                     rejf3 = aux - br * sinthe;
                     r3 = rnd();
-                } while (r3 * aux > rejf3);
+                } while ((r3 * aux > rejf3) && !stuck(trials_in));
             } else {
                 if (first) rejmax = broi + bro;
                 R r2;
+                int trials_in = 0;
                 do {
                     R r1 = rnd();
                     r2 = rnd();
@@ -632,10 +669,10 @@ template <class R, class G, class T> struct Engine {
                     temp = ((R)1.0 - br) / (ko * br);
                     sinthe = M<R>::fmax((R)0.0, temp * ((R)2.0 - temp));
                     rejf3 = (R)1.0 + br * br - br * sinthe;
-                } while (r2 * br * rejmax > rejf3);
+                } while ((r2 * br * rejmax > rejf3) && !stuck(trials_in));
             }
             first = false;
-        } while ((br < bro) || (br > (R)1));
+        } while (((br < bro) || (br > (R)1)) && !stuck(trials));
         costhe = (R)1.0 - temp;
         sinthe = M<R>::sqrt(sinthe);
         R esg = br * eig, ese = eig - esg + RM;
@@ -667,6 +704,7 @@ template <class R, class G, class T> struct Engine {
             R gamma = eelec / RM;
             R alpha = (R)0.5 * gamma - (R)0.5 + (R)1.0 / gamma;
             R ratio = beta / alpha, costhe, sinth2, xi, r2;
+            int trials = 0;
             do {
                 R rn = rnd();
                 rn = (R)2.0 * rn - (R)1.0;
@@ -688,7 +726,7 @@ template <class R, class G, class T> struct Engine {
                 }
                 sinth2 = M<R>::fmax(((R)1.0 - costhe) * ((R)1.0 + costhe), (R)0.0);
                 r2 = rnd();
-            } while (r2 > (R)0.5 * ((R)1.0 + gamma) * sinth2 * xi / gamma);
+            } while ((r2 > (R)0.5 * ((R)1.0 + gamma) * sinth2 * xi / gamma) && !stuck(trials));
             Uphi q;
             uphi21(p, q, costhe, M<R>::sqrt(sinth2));
         }
@@ -718,7 +756,7 @@ template <class R, class G, class T> struct Engine {
             R tstep;
             if (imed != -1) {
                 const Med<R> &m = t.med[imed];
-                lgle = pwlf_i(gle, m.ge1, m.ge0) - 1;
+                lgle = photon_interval(gle, m);
                 R gmfpr0 = pwlf(t.gmfp + (imed * MXGE + lgle), gle);
                 gmfp = gmfpr0 / rhof_of(p);
                 cohfac = pwlf(t.cohe + (imed * MXGE + lgle), gle);
@@ -831,7 +869,7 @@ template <class R, class G, class T> struct Engine {
         for (int m = 0; m < MXMED; m++) {
             const int mc = m < t.nmed ? m : 0;
             const Med<R> &md = t.med[mc];
-            const int lg = pwlf_i(gle, md.ge1, md.ge0) - 1;
+            const int lg = photon_interval(gle, md);
             const R mr = (R)1.0 / (pwlf(t.gmfp + (mc * MXGE + lg), gle) * pwlf(t.cohe + (mc * MXGE + lg), gle));
             const R rm = g.rhof_max[m];
             const bool use = m < t.nmed && rm > (R)0;
@@ -847,6 +885,7 @@ template <class R, class G, class T> struct Engine {
         for (int iters = 0;; iters++) {
             if (iters > (1 << 24)) {     // no progress: dropped, reported through `aborted`
                 aborted = 1;
+                abort_site = __LINE__;
                 p.ix = -1;
                 lost += (double)p.wt * (double)eig;
                 return -1;
@@ -886,7 +925,7 @@ template <class R, class G, class T> struct Engine {
         // wave ran the interaction code once per trial in which any lane
         // accepted).  Each lane's operations and draws are the same.
         const Med<R> &md = t.med[imed];
-        int lgle = pwlf_i(gle, md.ge1, md.ge0) - 1;
+        int lgle = photon_interval(gle, md);
         R cohfac = pwlf(t.cohe + (imed * MXGE + lgle), gle);
         return interact(p, q, imed, eig, gle, lgle, cohfac);
     }
@@ -901,7 +940,7 @@ template <class R, class G, class T> struct Engine {
         for (int m = 0; m < MXMED; m++) {
             const int mc = m < t.nmed ? m : 0;
             const Med<R> &md = t.med[mc];
-            const int lg = pwlf_i(gle, md.ge1, md.ge0) - 1;
+            const int lg = photon_interval(gle, md);
             xs.lgle[m] = lg;
             xs.gmfp0[m] = pwlf(t.gmfp + (mc * MXGE + lg), gle);
             xs.coh[m] = pwlf(t.cohe + (mc * MXGE + lg), gle);
@@ -1144,13 +1183,14 @@ template <class R, class G, class T> struct Engine {
         bool spin_index = true;
         Spinr sr = {0, 0};
         R xi, rejf, rn;
+        int trials = 0;
         do {
             xi = rnd();
             xi = (R)2.0 * chia2 * xi / ((R)1.0 - xi + chia2);
             cost = (R)1.0 - xi;
             rejf = spin_rejection(imed, qel, elke, beta2, (R)0, cost, spin_index, true, sr);
             rn = rnd();
-        } while (rn > rejf);
+        } while ((rn > rejf) && !stuck(trials));
         sint = M<R>::sqrt(xi * ((R)2.0 - xi));
     }
 
@@ -1166,13 +1206,14 @@ template <class R, class G, class T> struct Engine {
             if (sprob < explambda) { cost = (R)1.0; sint = (R)0.0; return; }
             R wsum = ((R)1.0 + lambda) * explambda;
             if (sprob < wsum) {
+                int trials = 0;
                 do {
                     xi = rnd();
                     xi = (R)2.0 * chia2 * xi / ((R)1.0 - xi + chia2);
                     cost = (R)1.0 - xi;
                     rejf = spin_rejection(imed, qel, elke, beta2, q1, cost, spin_index, false, sr);
                     rn = rnd();
-                } while (rn > rejf);
+                } while ((rn > rejf) && !stuck(trials));
                 sint = M<R>::sqrt(xi * ((R)2.0 - xi));
                 return;
             }
@@ -1182,18 +1223,20 @@ template <class R, class G, class T> struct Engine {
                 wsum = explambda;
                 cost = (R)1.0;
                 sint = (R)0.0;
+                int trials = 0;
                 do {
                     icount += 1;
                     if (icount > 20) break;
                     wprob = wprob * lambda / (R)icount;
                     wsum = wsum + wprob;
+                    int trials_in = 0;
                     do {
                         xi = rnd();
                         xi = (R)2.0 * chia2 * xi / ((R)1.0 - xi + chia2);
                         cosz = (R)1.0 - xi;
                         rejf = spin_rejection(imed, qel, elke, beta2, q1, cosz, spin_index, false, sr);
                         rn = rnd();
-                    } while (rn > rejf);
+                    } while ((rn > rejf) && !stuck(trials_in));
                     sinz = xi * ((R)2.0 - xi);
                     if (sinz > (R)1.0E-20) {
                         sinz = M<R>::sqrt(sinz);
@@ -1202,7 +1245,7 @@ template <class R, class G, class T> struct Engine {
                         cost = cost * cosz - sint * sinz * M<R>::cos(phi);
                         sint = M<R>::sqrt(M<R>::fmax((R)0.0, ((R)1.0 - cost) * ((R)1.0 + cost)));
                     }
-                } while (wsum <= sprob);
+                } while ((wsum <= sprob) && !stuck(trials));
                 return;
             }
         }
@@ -1234,6 +1277,7 @@ template <class R, class G, class T> struct Engine {
                 find_index = false;
             }
             const MsBin<R> *row = t.ms + (ms.i * (MXQ_MS + 1) + ms.j) * (MXU_MS + 1);
+            int trials = 0;
             do {
                 xi = rnd();
                 R ak = xi * (R)MXU_MS;
@@ -1256,7 +1300,7 @@ template <class R, class G, class T> struct Engine {
                 cost = (R)1.0 - xi;
                 rejf = spin_rejection(imed, qel, elke, beta2, q1, cost, spin_index, false, sr);
                 rn = rnd();
-            } while (rn > rejf);
+            } while ((rn > rejf) && !stuck(trials));
             sint = M<R>::sqrt(xi * ((R)2.0 - xi));
         }
     }
@@ -1476,6 +1520,7 @@ template <class R, class G, class T> struct Engine {
         R z2max = y2max + (R)1.0;
         R z2maxi = M<R>::sqrt(z2max);
         R aux, br, delta, r7, rejf, ese, esg, phi1, phi2;
+        int trials = 0;
         do {
             R r6 = rnd();
             r7 = rnd();
@@ -1492,7 +1537,7 @@ template <class R, class G, class T> struct Engine {
                 phi2 = phi1;
             }
             rejf = ((R)1.0 + aux * aux) * phi1 - (R)2.0 * aux * phi2 / (R)3.0;
-        } while (r7 >= rejf);
+        } while ((r7 >= rejf) && !stuck(trials));
         pg.e = esg;
         pg.iq = 0;
         transfer(pg, pe);
@@ -1505,7 +1550,8 @@ template <class R, class G, class T> struct Engine {
         if (aux1 > (R)10.0) rjarg3 = -M<R>::log(m.zbrang) + ((R)1.0 - aux1) / (aux1 * aux1);
         else rjarg3 = M<R>::log(aux / ((R)1.0 + aux1));
         R rejmax = rjarg1 * rjarg3 - rjarg2;
-        while (rtest >= rejtst) {
+        int trials_ang = 0;
+        while (rtest >= rejtst && !stuck(trials_ang)) {
             y2tst = rnd();
             rtest = rnd();
             R aux3 = z2maxi / (y2tst + ((R)1.0 - y2tst) * z2maxi);
@@ -1547,6 +1593,7 @@ template <class R, class G, class T> struct Engine {
         R t0 = ekin / RM, e0 = t0 + (R)1.0, extrae = eie - m.thmoll;
         R g2 = (t0 * t0) / (e0 * e0), g3 = ((R)2.0 * t0 + (R)1.0) / (e0 * e0);
         R br, gmax = ((R)1.0 + (R)1.25 * g2), rejf4, r28;
+        int trials = 0;
         do {
             R r27 = rnd();
             br = m.te / (ekin - extrae * r27);
@@ -1554,7 +1601,7 @@ template <class R, class G, class T> struct Engine {
             r28 = rnd();
             rejf4 = ((R)1.0 + g2 * (br * br) + r * (r - g3));
             r28 *= gmax;
-        } while (r28 > rejf4);
+        } while ((r28 > rejf4) && !stuck(trials));
         R ekse2 = br * ekin, ese1 = eie - ekse2, ese2 = ekse2 + RM;
         p0.e = ese1;
         p1.e = ese2;
@@ -1581,12 +1628,13 @@ template <class R, class G, class T> struct Engine {
         R b2 = yp * ((R)3.0 + yy * yy);
         R b1 = (R)2.0 - yy * yy;
         R br, rejf2, r4;
+        int trials = 0;
         do {
             R r3 = rnd();
             br = ep0 / ((R)1.0 - ep0c * r3);
             r4 = rnd();
             rejf2 = ((R)1.0 - beta2 * br * (b1 - br * (b2 - br * (b3 - br * b4))));
-        } while (r4 > rejf2);
+        } while ((r4 > rejf2) && !stuck(trials));
         p1.iq = 1;
         if (br < (R)0.5) {
             p1.iq = -1;
@@ -1621,13 +1669,14 @@ template <class R, class G, class T> struct Engine {
             cosdel = aa / sinpsi;
         }
         R ep, rejf, r2;
+        int trials = 0;
         do {
             R r1 = rnd();
             ep = ep0 * M<R>::exp(r1 * wsamp);
             r2 = rnd();
             R d = ep * a - (R)1.0;
             rejf = (R)1.0 - d * d / (ep * (a * a - (R)2.0));
-        } while (r2 > rejf);
+        } while ((r2 > rejf) && !stuck(trials));
         R esg1 = avip * ep;
         p0.e = esg1;
         p0.iq = 0;
@@ -1735,8 +1784,11 @@ template <class R, class G, class T> struct Engine {
                     es.phase = 0;
                     return 0;
                 }
-                if (++iters > (1 << 24)) {
-                    aborted = 1;
+                if (++iters > (1 << 24) || aborted) {     // or a sampler of the last pass was stuck
+                    if (!aborted) {
+                        aborted = 2;
+                        abort_site = __LINE__;
+                    }
                     p.e = eie;
                     lost += charged_energy(p);
                     return -1;
@@ -1952,8 +2004,11 @@ template <class R, class G, class T> struct Engine {
                 R tt0 = total_tstep;
                 total_tstep -= tvstep * rhof;
                 if (total_tstep < M<R>::reach_tol(tt0)) demfp = (R)0.0;
-                if (++iters > (1 << 24)) {
-                    aborted = 1;
+                if (++iters > (1 << 24) || aborted) {     // or a sampler of this step was stuck
+                    if (!aborted) {
+                        aborted = 3;
+                        abort_site = __LINE__;
+                    }
                     lost += charged_energy(p);
                     return -1;
                 }

@@ -263,8 +263,16 @@ int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int 
     js.ev_charged += done->events[1];
     js.rounds += (uint64_t)(done->done_round >= 0 ? done->done_round : done->rounds);
     js.peak = std::max<uint64_t>(js.peak, done->peak);
-    if (done->err & 2)
-        return omc_seterr(OMC_ERR_LIMIT, "a particle made no progress within its iteration guard and was dropped");
+    if (done->err & 2) {
+        static const char *why[] = {"?", "photon flight", "electron pass", "electron step", "rejection loop",
+                                    "lane trips", "requeues"};
+        const int w = done->drop_why >= 0 && done->drop_why <= 6 ? done->drop_why : 0;
+        return omc_seterr(OMC_ERR_LIMIT,
+                          "a particle made no progress within its iteration guard and was dropped (first: %s guard, "
+                          "source line %d; iq %d, E %.6g MeV, voxel (%d, %d, %d), phase %d, requeues %d)",
+                          why[w], done->drop_site, done->drop_iq, (double)done->drop_e, done->drop_ix, done->drop_iy,
+                          done->drop_iz, done->drop_phase, done->drop_age);
+    }
     if (done->err & 1) {
         omc_seterr(OMC_ERR_STACK, "event pool overflow (target %lld, capacity %lld)", S.target, S.cap);
         return RC_POOL_OVERFLOW;

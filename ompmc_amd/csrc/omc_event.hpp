@@ -74,7 +74,26 @@ struct EvCtl {
     unsigned long long claim[2][2];   // [round parity][photon, charged] queue entries claimed by lanes
     int inj_dest;                     // pool of the primaries: 0 photons, 1 charged
     int err;                          // bit 0: pool overflow; bit 1: a particle exceeded an iteration guard
+    // the first particle dropped by a guard: why (Engine::aborted, 5 lane
+    // trips, 6 requeues), the source line of the guard, and its state
+    int drop_why, drop_site, drop_iq, drop_ix, drop_iy, drop_iz, drop_phase, drop_age;
+    float drop_e;
 };
+
+// Records the first particle a guard dropped (one lane of the whole job wins).
+template <class R>
+__device__ __forceinline__ void note_drop(EvCtl *c, int why, int site, const Part<R> &p, int phase, int age) {
+    if (atomicCAS(&c->drop_why, 0, why) == 0) {
+        c->drop_site = site;
+        c->drop_iq = p.iq;
+        c->drop_ix = p.ix;
+        c->drop_iy = p.iy;
+        c->drop_iz = p.iz;
+        c->drop_phase = phase;
+        c->drop_age = age;
+        c->drop_e = (float)p.e;
+    }
+}
 
 template <class R> struct EvArgs {
     const Tables<R> *t;
@@ -98,6 +117,7 @@ template <class R> struct EvArgs {
     int max_age;                      // electrons: requeues after which a particle counts as stuck (dropped, err bit 1)
     int delta;                        // photons: 1 delta tracking, 0 voxel-by-voxel flight
     int nsplit;                       // photon splitting factor (vrt.nsplit); > 1 runs k_event_split for photons
+    EvCtl *ctl;                       // the job's control block (guard reports)
 };
 
 template <class R> __device__ __forceinline__ void load(const Pool<R> &P, long i, Part<R> &p) {
@@ -253,6 +273,7 @@ k_event(EvArgs<R> a) {
             // and reported (EvCtl::err bit 1), never transported forever
             if (e.aborted || ++iters > (1 << 22)) {
                 atomicOr(a.err, 2);
+                note_drop(a.ctl, e.aborted ? e.aborted : 5, e.abort_site, p, phase, age);
                 d = -1;
             }
             // d: -1 particle gone, 0 particle continues, +1 continues with a secondary
@@ -261,6 +282,7 @@ k_event(EvArgs<R> a) {
                 done = true;
                 if (++age > a.max_age) {
                     atomicOr(a.err, 2);
+                    note_drop(a.ctl, 6, 0, p, phase, age);
                     d = -1;
                 }
             }
@@ -417,6 +439,7 @@ k_event_split(EvArgs<R> a) {
         }
         if (have && e.aborted) {
             atomicOr(a.err, 2);
+            note_drop(a.ctl, e.aborted, e.abort_site, p, 0, 0);
             if (surv) e.lost += (double)p.wt * (double)p.e;
             surv = false;
         }
@@ -539,6 +562,7 @@ static __global__ void k_job_start(EvCtl *c, long long nhist, long long target, 
     c->peak = 0;
     c->inj_dest = dest;
     c->err = 0;
+    c->drop_why = c->drop_site = 0;
     round_end(c, 1);
 }
 

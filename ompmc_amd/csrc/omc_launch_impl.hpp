@@ -96,6 +96,7 @@ template <class R> void round_call(const EvLaunch<R> &L, long r, int part, bool 
         a.ebal = L.ebal;
         a.counters = L.counters ? L.counters + (part == 1 ? 0 : 8) : nullptr;
         a.err = &L.ctl->err;
+        a.ctl = L.ctl;
         a.budget = L.budget;
         a.turns = L.turns;
         a.max_age = L.max_age;
