@@ -432,6 +432,26 @@ def bench_c3(a):
                            "what": "untimed steps, best of two each; graph: each job's chunks of four rounds "
                                    "(16 kernels) replay one HIP graph"}
     log(f"launch modes: {out['launch_modes']}")
+    if a.tune and form == omc.FORM_EVENT:
+        # the autotuner's choice on a 200-beamlet sample of this plan, timed on
+        # the whole plan against the engine's defaults (untimed kernels)
+        from ompmc_amd import tune
+        t0 = time.perf_counter()
+        best = tune.tune_dij(ctx, gph, spec, bix, nbatch=nbatch, precision=prec, use_cache=False)
+        t_tune = time.perf_counter() - t0
+        kn = {k: best[k] for k in ("event_target", "electron_budget", "electron_turns", "streams")}
+        rates = {}
+        for name, kw in (("tuned", kn), ("default", {}), ("tuned", kn), ("default", {})):
+            o2 = dict(opts)
+            o2.update(kw)
+            t0 = time.perf_counter()
+            _, _, _, stt = ctx.run_dij(gph, spec, bix, nbatch=nbatch, rel_threshold=rel, **o2)
+            rates.setdefault(name, []).append(stt.histories / (time.perf_counter() - t0))
+        out["tuner"] = {"choice": kn, "tuning_seconds": t_tune, "tuned_histories_per_s": max(rates["tuned"]),
+                        "default_histories_per_s": max(rates["default"]),
+                        "what": "coordinate descent on a 200-beamlet sample; both settings on the whole plan, "
+                                "best of two untimed steps each"}
+        log(f"tuner: {out['tuner']}")
     if not a.no_fp64 and prec == omc.FP32:
         # FP32 against FP64 on the whole plan: three independent seeds per
         # precision; region (voxels above 20% of the plan maximum) fixed by an
@@ -686,6 +706,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-split", action="store_true", help="c3: skip the photon-splitting measurement")
+    ap.add_argument("--tune", action="store_true", help="c3: also run the autotuner and time its choice")
     ap.add_argument("--subset", type=int, default=1, help="c3: every n-th beamlet only (profiling runs)")
     ap.add_argument("--serve-loads", default="0.5,0.8,1.0",
                     help="offered loads of the served mix, as fractions of the batched throughput (e.g. 0.5,0.8,1.0)")

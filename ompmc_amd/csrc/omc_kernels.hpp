@@ -52,6 +52,7 @@ template <class R, class E> __device__ void init_history(E &e, const SrcDosxyz<R
         p.w = s.ssd / rxyz;
     } else {
         R fw, r3;
+        int trials = 0;
         do {
             r3 = e.rnd();
             p.x = r3 * s.xsize + s.xinl;
@@ -61,7 +62,7 @@ template <class R, class E> __device__ void init_history(E &e, const SrcDosxyz<R
             rxyz = M<R>::sqrt(s.ssd * s.ssd + p.x * p.x + p.y * p.y);
             p.w = s.ssd / rxyz;
             fw = p.w * p.w * p.w;
-        } while (r3 >= fw);
+        } while (r3 >= fw && !e.stuck(trials));
     }
     p.z = g.zb[0];
     p.u = p.x / rxyz;
@@ -81,6 +82,21 @@ template <class R, class E> __device__ void init_history(E &e, const SrcDosxyz<R
     }
     p.ix = ix; p.iy = iy; p.iz = 0;
     p.wt = (R)1.0;
+}
+
+// The reference's entry voxel along one axis (omc_matrad.c:1391-1406): a
+// linear scan that stops at the first interior boundary b[i + 1] >= x, i.e.
+// the number of interior boundaries b[1..n-1] below x.  Counted by
+// bisection with the same comparison, so the index is the scan's, in
+// log2(n) dependent loads instead of up to n.
+template <class R> __device__ __forceinline__ int entry_index(const R *b, int n, R x) {
+    int lo = 0, hi = n - 1;            // the answer lies in [lo, hi]
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b[mid] < x) lo = mid;      // b[1..mid] are all below x (b increases)
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 template <class R, class E> __device__ void init_history(E &e, const SrcBeamlets<R> &s, int ib, Part<R> &p) {
@@ -126,11 +142,9 @@ template <class R, class E> __device__ void init_history(E &e, const SrcBeamlets
     if (p.y > g.yb[g.ny]) p.y = g.yb[g.ny] - tiny;
     if (p.z < g.zb[0]) p.z = g.yb[0] + tiny;          // reference quirk Q12
     if (p.z > g.zb[g.nz]) p.z = g.zb[g.nz] - tiny;
-    int ix = 0, iy = 0, iz = 0;
-    while (ix < g.nx - 1 && g.xb[ix + 1] < p.x) ix++;
-    while (iy < g.ny - 1 && g.yb[iy + 1] < p.y) iy++;
-    while (iz < g.nz - 1 && g.zb[iz + 1] < p.z) iz++;
-    p.ix = ix; p.iy = iy; p.iz = iz;
+    p.ix = entry_index(g.xb, g.nx, p.x);
+    p.iy = entry_index(g.yb, g.ny, p.y);
+    p.iz = entry_index(g.zb, g.nz, p.z);
     p.wt = (R)1.0;
 }
 

@@ -147,14 +147,18 @@ def test_autotuner_picks_a_measured_setting(tmp_path, monkeypatch):
     c = omc.Context(ph_h.media)
     ph = c.phantom(ph_h, ecut=0.7, pcut=0.01)
     spec = omc.Spectrum.mono(2.0)
-    best = tune.tune_dij(c, ph, spec, bix, nbatch=4, sample=8, targets=(4096, 65536), budgets=(2, 8), turns=(3, -1))
+    best = tune.tune_dij(c, ph, spec, bix, nbatch=4, sample=8, targets=(4096, 65536), budgets=(2, 8), turns=(3, -1),
+                         streams=(1, 2), max_evals=6)
     assert best["event_target"] in (4096, 65536) and best["electron_budget"] in (2, 8)
-    assert best["electron_turns"] in (3, -1)
+    assert best["electron_turns"] in (3, -1) and best["streams"] in (1, 2)
     assert (tmp_path / "tune.json").exists()
+    import json
+    table = json.load(open(tmp_path / "tune.json")).popitem()[1]["table"]
+    assert 2 <= len(table) <= 6                              # the descent stops within its budget
     assert tune.tune_dij(c, ph, spec, bix, nbatch=4, sample=8) == best          # from the cache
     # a split plan is tuned with splitting, under its own cache entry
     bs = tune.tune_dij(c, ph, spec, bix, nbatch=4, sample=8, targets=(4096, 65536), budgets=(8,), turns=(24,),
-                       nsplit=4)
+                       streams=(1,), nsplit=4)
     assert bs["nsplit"] == 4 and best["nsplit"] == 1
     assert tune.tune_dij(c, ph, spec, bix, nbatch=4, sample=8, nsplit=4) == bs
     a = c.run_dij(ph, spec, bix, nbatch=4, seed=5, **best)[0]
