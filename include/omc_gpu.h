@@ -145,6 +145,10 @@ typedef struct omc_run_opts {
     int graph;              /* event form: 0 = each job's chunks of rounds replay a HIP graph of its first
                                chunk (eager launches when count_events times the kernels); -1 = eager
                                launches */
+    int lds_tables;         /* event form, FP32: 0 = the charged-particle kernel copies the electron
+                               coefficients of each energy bin (up to 64 KiB) into LDS once per workgroup
+                               and reads them there; -1 = reads them from the tables in device memory.
+                               Same values either way */
 } omc_run_opts;
 
 /* Kernel classes of omc_run_stats.kernel_seconds / kernel_launches */
@@ -187,6 +191,9 @@ typedef struct omc_run_stats {
     /* event form: chunks of rounds enqueued by replaying a HIP graph, and
      * jobs whose graph the runtime refused (they ran eager launches) */
     uint64_t graph_replays, graph_fallbacks;
+    /* event form: bytes of electron coefficients the charged-particle kernel copied into LDS per
+     * workgroup (0: it read them from device memory; see omc_run_opts.lds_tables) */
+    uint64_t lds_table_bytes;
 } omc_run_stats;
 
 /* Defaults: FP32, event form, Philox, seed 1802, nsplit 1, automatic launch. */

@@ -62,7 +62,7 @@ class RunOpts(C.Structure):
                 ("electron_budget", C.c_int), ("group", C.c_int), ("hbm_budget", C.c_int64),
                 ("voxel_walk", C.c_int), ("electron_turns", C.c_int), ("batches_per_job", C.c_int),
                 ("streams", C.c_int), ("max_particle_steps", C.c_int64), ("csc_on_device", C.c_int),
-                ("graph", C.c_int)]
+                ("graph", C.c_int), ("lds_tables", C.c_int)]
 
 
 class RunStats(C.Structure):
@@ -78,7 +78,7 @@ class RunStats(C.Structure):
                 ("streams", C.c_int64), ("e_source", C.c_double), ("e_escape", C.c_double),
                 ("e_lost", C.c_double), ("e_deposit", C.c_double),
                 ("kernel_busy_seconds", C.c_double * 5), ("graph_replays", C.c_uint64),
-                ("graph_fallbacks", C.c_uint64)]
+                ("graph_fallbacks", C.c_uint64), ("lds_table_bytes", C.c_uint64)]
 
     def energy_residual(self):
         """(deposit + escape + lost - source) / source of the run"""
@@ -205,7 +205,7 @@ def default_paths():
 def run_opts(precision=FP32, form=FORM_EVENT, seed=1802, rng=RNG_PHILOX, replay=None, replay_offsets=None,
              grid=0, block=0, count_events=False, nsplit=1, event_target=0, electron_budget=0, group=0,
              hbm_budget=0, voxel_walk=False, electron_turns=0, batches_per_job=0, streams=0, max_particle_steps=0,
-             csc_on_device=False, graph=True):
+             csc_on_device=False, graph=True, lds_tables=True):
     o = RunOpts()
     lib().omc_run_opts_default(C.byref(o))
     o.precision, o.form, o.seed, o.rng, o.nsplit = precision, form, seed, rng, nsplit
@@ -217,6 +217,7 @@ def run_opts(precision=FP32, form=FORM_EVENT, seed=1802, rng=RNG_PHILOX, replay=
     o.max_particle_steps = max_particle_steps
     o.csc_on_device = int(bool(csc_on_device))
     o.graph = 0 if graph else -1
+    o.lds_tables = 0 if lds_tables else -1
     keep = []
     if replay is not None:
         rp = np.ascontiguousarray(replay, np.uint32)

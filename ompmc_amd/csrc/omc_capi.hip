@@ -3,6 +3,7 @@
 // serving entry points live in omc_dij.hip and omc_server.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -272,6 +273,32 @@ template <class R> static int upload_tables(omc_ctx *c, DevArena &A, Tables<R> &
     T.range_ep = vec(h.range_ep, 2 * nek);
     T.e_array = vec(h.e_array, nek);
     T.spin_rej = vec(h.spin_rej, (size_t)nm * OMC_NSPIN_MED + 1);   // + the zero entry of quirk Q17
+    // the coefficients of each charged bin interleaved, [2][nmed][stride]
+    // with stride = the largest MEKE + 1 (bins 0 .. MEKE of every medium)
+    int stride = 0;
+    for (int i = 0; i < nm; i++) stride = std::max(stride, std::min(h.meke[i] + 1, OMC_MXEKE));
+    std::vector<CBin<R>> cb((size_t)2 * nm * stride);
+    for (int qel = 0; qel < 2; qel++)
+        for (int i = 0; i < nm; i++)
+            for (int k = 0; k < stride; k++) {
+                const size_t s = (size_t)i * OMC_MXEKE + k;
+                CBin<R> &b = cb[((size_t)qel * nm + i) * stride + k];
+                memset(&b, 0, sizeof b);
+                auto set = [&](PW<R> &q, const double *c0, const double *c1) { q.c1 = (R)c1[s]; q.c0 = (R)c0[s]; };
+                set(b.dedx, qel ? h.pdedx0 : h.ededx0, qel ? h.pdedx1 : h.ededx1);
+                set(b.sig, qel ? h.psig0 : h.esig0, qel ? h.psig1 : h.esig1);
+                set(b.tmxs, h.tmxs0, h.tmxs1);
+                set(b.eta, qel ? h.etap_ms0 : h.etae_ms0, qel ? h.etap_ms1 : h.etae_ms1);
+                set(b.blcc, h.blcce0, h.blcce1);
+                set(b.q1c, qel ? h.q1cp_ms0 : h.q1ce_ms0, qel ? h.q1cp_ms1 : h.q1ce_ms1);
+                set(b.q2c, qel ? h.q2cp_ms0 : h.q2ce_ms0, qel ? h.q2cp_ms1 : h.q2ce_ms1);
+                set(b.br1, qel ? h.pbr10 : h.ebr10, qel ? h.pbr11 : h.ebr11);
+                if (qel) set(b.br2, h.pbr20, h.pbr21);
+                b.e_lo = (R)h.e_array[s];
+                b.range = (R)h.range_ep[(size_t)qel * nek + s];
+            }
+    T.cbin = A.upload(cb);
+    T.cb_stride = stride;
     std::vector<MsBin<R>> msb(OMC_NMS);
     for (size_t n = 0; n < (size_t)OMC_NMS; n++) msb[n] = MsBin<R>{(R)h.wms[n], h.ims[n], (R)h.fms[n], (R)h.ums[n]};
     T.ms = A.upload(msb);
@@ -556,6 +583,8 @@ int omc_check_opts(const omc_run_opts *o) {
     if (o->electron_turns < -1) return omc_seterr(OMC_ERR_ARG, "electron_turns %d (-1 no limit, 0 automatic)", o->electron_turns);
     if (o->streams < 0 || o->streams > 4) return omc_seterr(OMC_ERR_ARG, "streams %d (0 automatic, 1..4)", o->streams);
     if (o->graph != 0 && o->graph != -1) return omc_seterr(OMC_ERR_ARG, "graph %d (0 graphs, -1 eager launches)", o->graph);
+    if (o->lds_tables != 0 && o->lds_tables != -1)
+        return omc_seterr(OMC_ERR_ARG, "lds_tables %d (0 staged in LDS, -1 read from device memory)", o->lds_tables);
     return OMC_OK;
 }
 

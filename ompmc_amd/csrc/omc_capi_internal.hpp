@@ -93,6 +93,10 @@ struct omc_ctx {
 template <class R> const omc::Tables<R> *tables_of(const omc_ctx *c);
 template <> inline const omc::Tables<float> *tables_of<float>(const omc_ctx *c) { return c->dt32; }
 template <> inline const omc::Tables<double> *tables_of<double>(const omc_ctx *c) { return c->dt64; }
+// the host copies of the same descriptors (device pointers inside)
+template <class R> const omc::Tables<R> &host_tables_of(const omc_ctx *c);
+template <> inline const omc::Tables<float> &host_tables_of<float>(const omc_ctx *c) { return c->t32; }
+template <> inline const omc::Tables<double> &host_tables_of<double>(const omc_ctx *c) { return c->t64; }
 template <class R> const omc::Geom<R> *geom_of(const PhantomDev *p);
 template <> inline const omc::Geom<float> *geom_of<float>(const PhantomDev *p) { return p->dg32; }
 template <> inline const omc::Geom<double> *geom_of<double>(const PhantomDev *p) { return p->dg64; }

@@ -131,6 +131,19 @@ template <class R> __device__ __forceinline__ PW<R> gld(const PW<R> *p) {
     return r;
 }
 
+// The coefficients of one energy bin of one charged species in one medium,
+// interleaved (80 bytes in FP32): what a condensed-history step, its
+// multiple scattering and its discrete interaction read at the particle's
+// own bin, so a step touches one or two cache lines instead of eight tables.
+// Tables::cbin holds them as [2][nmed][cb_stride] (electrons, positrons;
+// cb_stride = the largest MEKE + 1); the per-quantity arrays stay for reads
+// at other bins (the range walk of eloss, the end of a step's interval).
+template <class R> struct CBin {
+    PW<R> dedx, sig, tmxs, eta, blcc, q1c, q2c, br1, br2;  // br1/br2: ebr1 (electrons), pbr1/pbr2 (positrons)
+    R e_lo, range;                                          // e_array and range_ep at the bin
+};
+static_assert(sizeof(CBin<float>) % 16 == 0 && sizeof(CBin<double>) % 16 == 0, "copied in 16-byte words");
+
 template <class R> struct Med {
     R ge0, ge1, eke0, eke1;
     R rho, te, thmoll, ap, delcm, zbrang, bpar0, bpar1;
@@ -155,6 +168,8 @@ template <class R> struct Tables {
     const R *spin_rej;                                          // [nmed][NSPIN_MED]
     const MsBin<R> *ms;                                         // [NMS] q^(2+) surface bins
     R b2spin_min, dbeta2i, espml, dleneri, dqq1i, dllambi, dqmsi;
+    const CBin<R> *cbin;                                        // [2][nmed][cb_stride] charged bins
+    int cb_stride;
 };
 
 // Rectilinear voxel phantom resident in HBM: boundaries plus one byte of
@@ -185,8 +200,11 @@ template <class R> struct Part {
 struct Philox {
     // Philox4x32-10 (Salmon et al., SC'11): counter-based, keyed per stream.
     uint32_t k0, k1, c0, c1, c2, c3;
-    uint32_t buf[4];
-    int pos;
+    // the outputs of the last round not yet drawn, the next one in b0: each
+    // draw shifts them down, so no access needs a run-time index (an indexed
+    // buffer can be moved to scratch memory when registers are short)
+    uint32_t b0, b1, b2, b3;
+    int pos;                        // outputs of the last round drawn (4: none left)
     __device__ __forceinline__ void init(uint64_t seed, uint64_t stream) {
         k0 = (uint32_t)seed; k1 = (uint32_t)(seed >> 32);
         c0 = (uint32_t)stream; c1 = (uint32_t)(stream >> 32); c2 = 0; c3 = 0;
@@ -202,16 +220,25 @@ struct Philox {
             x0 = y0; x1 = y1; x2 = y2; x3 = y3;
             a += 0x9E3779B9u; b += 0xBB67AE85u;
         }
-        buf[0] = x0; buf[1] = x1; buf[2] = x2; buf[3] = x3;
+        b0 = x0; b1 = x1; b2 = x2; b3 = x3;
         if (++c2 == 0) ++c3;
         pos = 0;
     }
+    __device__ __forceinline__ uint32_t take() {
+        uint32_t v = b0;
+        b0 = b1; b1 = b2; b2 = b3;
+        pos++;
+        return v;
+    }
+    // drops the first k (0..3) outputs of a fresh round
+    __device__ __forceinline__ void skip(int k) {
+        if (k > 0) take();
+        if (k > 1) take();
+        if (k > 2) take();
+    }
     template <class R> __device__ __forceinline__ R next() {
         if (pos >= 4) round4();
-        // selects instead of a dynamic index keep buf in registers
-        uint32_t v = pos == 0 ? buf[0] : pos == 1 ? buf[1] : pos == 2 ? buf[2] : buf[3];
-        pos++;
-        return (R)(v >> 8) * (R)(1.0 / 16777216.0);
+        return (R)(take() >> 8) * (R)(1.0 / 16777216.0);
     }
 };
 
@@ -298,7 +325,11 @@ struct Counters {
 };
 
 // ------------------------------------------------------------------ transport engine
-template <class R, class G, class T> struct Engine {
+// LDS: the charged-particle event kernel's engine, whose electron bins are
+// read through `eb` -- the kernel's LDS copy of them, or their place in
+// device memory when they are not staged -- with generic loads; otherwise
+// every bin comes from Tables::cbin with global loads.
+template <class R, class G, class T, bool LDS = false> struct Engine {
     const Tables<R> &t;
     const Geom<R> &g;
     G &rng;
@@ -311,6 +342,26 @@ template <class R, class G, class T> struct Engine {
     // source line of the loop in abort_site)
     int aborted = 0;
     int abort_site = 0;
+    const CBin<R> *eb = nullptr;    // LDS engines: electron bins (LDS or device memory, same layout)
+
+    // the interleaved coefficients of bin l of species qel (0 electrons,
+    // 1 positrons) in medium imed, and loads of their fields
+    __device__ __forceinline__ const CBin<R> *cbin(int qel, int imed, int l) const {
+        // bins past MEKE are never reached below the tables' upper energy;
+        // the clamp keeps a stray index inside the array
+        l = min(max(l, 0), t.cb_stride - 1);
+        if (LDS && qel == 0) return eb + (imed * t.cb_stride + l);
+        return t.cbin + (((size_t)qel * t.nmed + imed) * t.cb_stride + l);
+    }
+    __device__ __forceinline__ PW<R> cld(const PW<R> *p) const {
+        if (LDS) return *p;
+        return gld(p);
+    }
+    __device__ __forceinline__ R cld(const R *p) const {
+        if (LDS) return *p;
+        return gld(p);
+    }
+    __device__ __forceinline__ R cpw(const PW<R> *p, R lv) const { return pwlf(cld(p), lv); }
     double esc = 0.0;               // wt * E of photons that left the phantom (energy balance)
     double lost = 0.0;              // energy dropped without a deposit (energy balance)
 
@@ -1330,18 +1381,11 @@ template <class R, class G, class T> struct Engine {
             lelke = 0;
             elke = ((R)1.0 - m.eke0) / m.eke1;
         }
-        int ix = MXEKE * imed + lelke;
-        R etap, xi_corr, gamma;
-        if (qel == 0) {
-            etap = pwlf(t.etae + (ix), elke);
-            xi_corr = pwlf(t.q1ce + (ix), elke);
-            gamma = pwlf(t.q2ce + (ix), elke);
-        } else {
-            etap = pwlf(t.etap + (ix), elke);
-            xi_corr = pwlf(t.q1cp + (ix), elke);
-            gamma = pwlf(t.q2cp + (ix), elke);
-        }
-        R ms_corr = pwlf(t.blcce + (ix), elke);
+        const CBin<R> *cb = cbin(qel, imed, lelke);
+        R etap = cpw(&cb->eta, elke);
+        R xi_corr = cpw(&cb->q1c, elke);
+        R gamma = cpw(&cb->q2c, elke);
+        R ms_corr = cpw(&cb->blcc, elke);
         chia2 *= etap;
         lambda /= (etap * ((R)1.0 + chia2));
         lambda *= ms_corr;
@@ -1406,7 +1450,7 @@ template <class R, class G, class T> struct Engine {
 
     // CSDA path length inside one PWLF interval (src/ompmc.c:3997-4032)
     __device__ OMC_INL R drange(int imed, int iq, int lelke, R ekei, R ekef, R elkei, R elkef) {
-        return drange(gld((iq < 0 ? t.ededx : t.pdedx) + (MXEKE * imed + lelke)), ekei, ekef, elkei, elkef);
+        return drange(cld(&cbin((1 + iq) / 2, imed, lelke)->dedx), ekei, ekef, elkei, elkef);
     }
     // the same with the interval's stopping-power entry already loaded
     __device__ OMC_INL R drange(const PW<R> &q, R ekei, R ekef, R elkei, R elkef) {
@@ -1420,10 +1464,8 @@ template <class R, class G, class T> struct Engine {
 
     // sub-threshold energy loss over a path tustep (src/ompmc.c:4034-4126)
     __device__ OMC_INL R eloss(int imed, int iq, R rhof, R tustep, R range, R eke, R elke, int lelke) {
-        const int qel = (1 + iq) / 2;
-        const R *rep = t.range_ep + ((size_t)qel * t.nmed + imed) * MXEKE;
-        return eloss(imed, iq, rhof, tustep, range, eke, elke, lelke,
-                     gld((iq < 0 ? t.ededx : t.pdedx) + (imed * MXEKE + lelke)), gld(rep + (lelke)));
+        const CBin<R> *cb = cbin((1 + iq) / 2, imed, lelke);
+        return eloss(imed, iq, rhof, tustep, range, eke, elke, lelke, cld(&cb->dedx), cld(&cb->range));
     }
     // the same with the stopping-power entry and the range at the lower edge
     // of the energy bin already loaded
@@ -1808,10 +1850,9 @@ template <class R, class G, class T> struct Engine {
                 demfp = M<R>::fmax(-M<R>::log(rn), (R)1.0E-5);
                 elke = M<R>::log(eke);
                 lelke = pwlf_i(elke, m.eke1, m.eke0) - 1;
-                int ix = imed * MXEKE + lelke;
                 if (m.sig_mono[qel]) {
-                    if (iq < 0) sig0 = pwlf(t.esig + (ix), elke) / pwlf(t.ededx + (ix), elke);
-                    else sig0 = pwlf(t.psig + (ix), elke) / pwlf(t.pdedx + (ix), elke);
+                    const CBin<R> *cb = cbin(qel, imed, lelke);
+                    sig0 = cpw(&cb->sig, elke) / cpw(&cb->dedx, elke);
                 } else {
                     sig0 = iq < 0 ? m.esig_e : m.psig_e;
                 }
@@ -1833,13 +1874,13 @@ template <class R, class G, class T> struct Engine {
                     // this step's table entries at (medium, energy bin), loaded
                     // together ahead of the branches below, whose outcome they
                     // do not depend on (same values as loading them where used)
-                    const int ix = imed * MXEKE + lelke;
-                    q_dedx = gld((iq < 0 ? t.ededx : t.pdedx) + ix);
-                    const PW<R> q_tmxs = gld(t.tmxs + ix);
-                    const PW<R> q_eta = gld((iq < 0 ? t.etae : t.etap) + ix);
-                    const PW<R> q_msc = gld(t.blcce + ix);
-                    const R e_lo = gld(t.e_array + ix);
-                    rep_lo = gld(rep + (lelke));
+                    const CBin<R> *cb = cbin(qel, imed, lelke);
+                    q_dedx = cld(&cb->dedx);
+                    const PW<R> q_tmxs = cld(&cb->tmxs);
+                    const PW<R> q_eta = cld(&cb->eta);
+                    const PW<R> q_msc = cld(&cb->blcc);
+                    const R e_lo = cld(&cb->e_lo);
+                    rep_lo = cld(&cb->range);
                     rhof = rhof_voxel;           // the reference reloads the density here, not at the move
                     if (sig0 <= (R)0.0) {
                         tstep = (R)10.0E8;
@@ -2020,15 +2061,14 @@ template <class R, class G, class T> struct Engine {
                 }
             } while (demfp >= (R)1.0E-5);
             if ((imed != medold) || (ustep == (R)0.0) || (imed == -1)) continue;
-            int ix = imed * MXEKE + lelke;
-            R sigf = iq < 0 ? pwlf(t.esig + (ix), elke) / pwlf(t.ededx + (ix), elke)
-                            : pwlf(t.psig + (ix), elke) / pwlf(t.pdedx + (ix), elke);
+            const CBin<R> *cbf = cbin(qel, imed, lelke);
+            R sigf = cpw(&cbf->sig, elke) / cpw(&cbf->dedx, elke);
             sigratio = sigf / sig0;
             rfict = rnd();
         } while (rfict >= sigratio);
-        int ix = imed * MXEKE + lelke;
+        const CBin<R> *cbi = cbin(qel, imed, lelke);
         if (iq < 0) {
-            R ebr1 = pwlf(t.ebr1 + (ix), elke);
+            R ebr1 = cpw(&cbi->br1, elke);
             R rn = rnd();
             if (rn <= ebr1) {
                 brems(p, q);
@@ -2041,13 +2081,13 @@ template <class R, class G, class T> struct Engine {
             }
             return moller(p, q);
         }
-        R pbr1 = pwlf(t.pbr1 + (ix), elke);
+        R pbr1 = cpw(&cbi->br1, elke);
         R rn = rnd();
         if (rn < pbr1) {
             brems(p, q);
             return 1;
         }
-        R pbr2 = pwlf(t.pbr2 + (ix), elke);
+        R pbr2 = cpw(&cbi->br2, elke);
         if (rn < pbr2) bhabha(p, q);
         else annih(p, q);
         return 1;

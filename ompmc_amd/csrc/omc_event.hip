@@ -62,6 +62,10 @@ constexpr int DEFAULT_BUDGET = 8;            // condensed-history steps between 
 // Measured on the 1960-beamlet plan (1 M in flight): 12 -> 12.0, 16 -> 12.3,
 // 24 -> 12.5, 64 -> 11.2, no limit -> 9.7 M histories/s.
 constexpr int DEFAULT_TURNS = 24;
+// LDS a charged-kernel workgroup may give to the electron bins (12 KB per
+// medium at 150 bins; two workgroups share a CU's 160 KB).  Plans with more
+// media read them from device memory.
+constexpr size_t MAX_LDS_TABLES = 64 * 1024;
 // Round guard of a job: the rounds its injection needs plus 2^16 -- with the
 // requeue after DEFAULT_TURNS x DEFAULT_BUDGET steps that is ~1.7e7 steps for
 // one particle, thousands of times the longest physical track.  A job that
@@ -133,6 +137,7 @@ struct JobStats {
     hipEvent_t origin = nullptr;
     std::vector<std::pair<float, float>> iv[4];
     uint64_t graph_replays = 0, graph_fallbacks = 0;
+    uint64_t lds_bytes = 0;          // electron bins the charged kernel staged in LDS per workgroup
 };
 
 // Seconds covered by the union of the intervals of one kernel class over
@@ -180,6 +185,7 @@ int run_job(hipStream_t st, EvState<R> &S, EvLaunch<R> &L, long long nhist, int 
     constexpr int K = 4;
     static_assert(K % 2 == 0, "chunks must start at even rounds");
     bool use_graph = L.graph && !timing;
+    js.lds_bytes = (uint64_t)L.eb_words * 16u;
     GraphExec gx;
     // With photon splitting one split puts up to 2 x nsplit charged particles
     // into the queues: a round injects at most target / nsplit primaries, so
@@ -316,6 +322,13 @@ template <class R> void fill_launch(omc_ctx *c, const PhantomDev *pd, EvState<R>
     L.delta = o->voxel_walk ? 0 : 1;
     L.nsplit = o->nsplit;
     L.graph = o->graph >= 0 ? 1 : 0;
+    {
+        const Tables<R> &ht = host_tables_of<R>(c);
+        L.ebin = ht.cbin;
+        const size_t bytes = (size_t)ht.nmed * ht.cb_stride * sizeof(CBin<R>);
+        if (std::is_same<R, float>::value && o->lds_tables >= 0 && bytes <= MAX_LDS_TABLES)
+            L.eb_words = (int)(bytes / 16);
+    }
     int cus = device_cus(c->device);
     L.grid_event = o->grid > 0 ? o->grid : cus * 8;
     L.grid_source = cus * 4;
@@ -370,6 +383,7 @@ void stats_from_job(omc_run_stats *st, const JobStats &js) {
     st->rounds += js.rounds;
     st->graph_replays += js.graph_replays;
     st->graph_fallbacks += js.graph_fallbacks;
+    st->lds_table_bytes = std::max<uint64_t>(st->lds_table_bytes, js.lds_bytes);
     st->peak_inflight = std::max<uint64_t>(st->peak_inflight, js.peak);
 }
 

@@ -44,7 +44,7 @@ struct PhiloxAt {
         n = pos;
         if (pos & 3u) {
             g.round4();
-            g.pos = (int)(pos & 3u);
+            g.skip((int)(pos & 3u));
         }
     }
     template <class R> __device__ __forceinline__ R next() {
@@ -118,6 +118,8 @@ template <class R> struct EvArgs {
     int delta;                        // photons: 1 delta tracking, 0 voxel-by-voxel flight
     int nsplit;                       // photon splitting factor (vrt.nsplit); > 1 runs k_event_split for photons
     EvCtl *ctl;                       // the job's control block (guard reports)
+    const CBin<R> *ebin;              // charged kernel: the electron half of Tables::cbin
+    int eb_words;                     // its size in 16-byte words when copied to LDS (0: read in place)
 };
 
 template <class R> __device__ __forceinline__ void load(const Pool<R> &P, long i, Part<R> &p) {
@@ -215,6 +217,19 @@ __device__ __forceinline__ void flush_counters(unsigned long long *dst, const Co
 template <class R, bool PHOTON, bool COUNT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PHOTON ? 1 : OMC_CHARGED_WAVES_PER_EU)))
 k_event(EvArgs<R> a) {
+    // Charged kernel: the electron bins of every medium (nmed x (MEKE + 1)
+    // records of 80 bytes, at most 64 KiB; launched with that much dynamic
+    // LDS) are copied to LDS once per workgroup, and the step loop reads its
+    // per-bin coefficients there instead of from device memory.
+    extern __shared__ __align__(16) unsigned char omc_lds[];
+    const CBin<R> *eb = a.ebin;
+    if (!PHOTON && a.eb_words > 0) {
+        uint4 *dst = reinterpret_cast<uint4 *>(omc_lds);
+        const uint4 *src = reinterpret_cast<const uint4 *>(a.ebin);
+        for (int i = threadIdx.x; i < a.eb_words; i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+        eb = reinterpret_cast<const CBin<R> *>(omc_lds);
+    }
     const long long n = (long long)*a.n_in;
     Counters tot = {0, 0, 0, 0, 0, 0, 0};
     double eb_esc = 0.0;             // photons that left the phantom (photon kernel only)
@@ -254,12 +269,14 @@ k_event(EvArgs<R> a) {
         bool done = false;
         uint32_t qpos = 0;
         if (have) {
-            Engine<R, PhiloxAt, TallyAtomic> e{*a.t, *a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
+            Engine<R, PhiloxAt, TallyAtomic, !PHOTON && sizeof(R) == 4> e{*a.t, *a.g, rng, tally,
+                                                                          {0, 0, 0, 0, 0, 0, 0}, COUNT};
             e.nsplit = a.nsplit;          // roulette of the photons that charged particles produce
+            e.eb = eb;
             if (PHOTON) {
                 d = a.delta ? e.photon_delta(p, q) : e.photon(p, q);
             } else {
-                typename Engine<R, PhiloxAt, TallyAtomic>::EState es;
+                typename decltype(e)::EState es;
                 es.demfp = demfp;
                 es.sig0 = sig0;
                 es.phase = phase;

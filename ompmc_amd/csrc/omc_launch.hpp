@@ -37,6 +37,8 @@ template <class R> struct EvLaunch {
     int delta;
     int nsplit;                  // photon splitting factor; > 1 transports photons with k_event_split
     int graph;                   // 1: untimed chunks of rounds replay a HIP graph
+    const CBin<R> *ebin;         // the electron half of the charged bins (Tables::cbin)
+    int eb_words;                // > 0: the charged kernel copies it to LDS (16-byte words)
     SrcArgs<R> src;              // source of the job (ctl and out are set per round)
     int grid_event, grid_source;
 };

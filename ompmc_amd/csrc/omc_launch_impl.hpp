@@ -59,6 +59,7 @@ template <class R> struct RoundCall {
     int par = 0;
     void *args[2] = {nullptr, nullptr};
     int part = 0;
+    unsigned shmem = 0;                  // dynamic LDS bytes per workgroup
     // kernelParams of a graph node: pointers to the argument values above
     void **bind() {
         if (part == 0) args[0] = &src;
@@ -102,6 +103,9 @@ template <class R> void round_call(const EvLaunch<R> &L, long r, int part, bool 
         a.max_age = L.max_age;
         a.delta = L.delta;
         a.nsplit = L.nsplit > 1 ? L.nsplit : 1;
+        a.ebin = L.ebin;
+        a.eb_words = part == 2 ? L.eb_words : 0;
+        c.shmem = (unsigned)a.eb_words * 16u;
         a.in = L.pool[par][part - 1];
         a.n_in = &L.ctl->cnt[par][part - 1];
         a.claim = &L.ctl->claim[par][part - 1];
@@ -139,9 +143,9 @@ template <class R> hipError_t launch_round_part(const EvLaunch<R> &L, long r, in
         else if (c.fn == reinterpret_cast<const void *>(&k_event<R, true, false>))
             hipLaunchKernelGGL((k_event<R, true, false>), c.grid, c.block, 0, st, c.ev);
         else if (c.fn == reinterpret_cast<const void *>(&k_event<R, false, true>))
-            hipLaunchKernelGGL((k_event<R, false, true>), c.grid, c.block, 0, st, c.ev);
+            hipLaunchKernelGGL((k_event<R, false, true>), c.grid, c.block, c.shmem, st, c.ev);
         else
-            hipLaunchKernelGGL((k_event<R, false, false>), c.grid, c.block, 0, st, c.ev);
+            hipLaunchKernelGGL((k_event<R, false, false>), c.grid, c.block, c.shmem, st, c.ev);
     } else {
         hipLaunchKernelGGL(k_round_end, c.grid, c.block, 0, st, c.ctl, c.par);
     }
@@ -166,7 +170,7 @@ template <class R> hipError_t build_rounds_graph(const EvLaunch<R> &L, long r0, 
             p.func = const_cast<void *>(c.fn);
             p.gridDim = c.grid;
             p.blockDim = c.block;
-            p.sharedMemBytes = 0;
+            p.sharedMemBytes = c.shmem;
             p.kernelParams = c.bind();
             p.extra = nullptr;
             hipGraphNode_t n = nullptr;

@@ -321,3 +321,30 @@ def test_graph_replay_matches_eager_launches(ct):
         assert st.graph_replays > 0 and st.graph_fallbacks == 0
         assert np.array_equal(jc, jc2) and np.array_equal(ir, ir2), rep
         assert np.allclose(val, val2, rtol=1e-9), rep
+
+
+def test_lds_staged_bins_match_device_memory_reads(ct):
+    """The FP32 charged-particle kernel reads the electron coefficients of a
+    particle's energy bin from a copy in LDS (lds_tables on, the default) or
+    from device memory (off): the same values, so the same seed gives the
+    same dose up to FP64 summation order, for dose and Dij runs."""
+    c, ph = ct
+    spec = omc.Spectrum.read(omc.os.path.join(omc.DATA, "spectra", "mohan6.spectrum"))
+    kw = dict(collimator=(-3, 3, -3, 3), precision=omc.FP32, form=omc.FORM_EVENT, seed=91)
+    a1, a2, sa = c.run_dosxyz(ph, spec, 20000, 4, lds_tables=True, **kw)
+    b1, b2, sb = c.run_dosxyz(ph, spec, 20000, 4, lds_tables=False, **kw)
+    assert sa.errors == 0 and sb.errors == 0
+    assert sa.lds_table_bytes > 0 and sb.lds_table_bytes == 0
+    assert sa.lds_table_bytes <= 64 * 1024 and sa.lds_table_bytes % 80 == 0
+    assert np.allclose(a1, b1, rtol=1e-9, atol=1e-12 * b1.max())
+    assert np.allclose(a2, b2, rtol=1e-9, atol=1e-12 * b2.max())
+    # FP64 transport reads device memory whatever the option says
+    _, _, s64 = c.run_dosxyz(ph, spec, 2000, 2, lds_tables=True, **dict(kw, precision=omc.FP64))
+    assert s64.lds_table_bytes == 0
+    bix = P.beamlets(nbeams=2, radius=1.5, hist_median=800, hist_range=(400, 1600), seed=19)
+    dk = dict(nbatch=4, precision=omc.FP32, form=omc.FORM_EVENT, streams=2, seed=92)
+    (jc, ir, val), _, _, st1 = c.run_dij(ph, spec, bix, lds_tables=True, **dk)
+    (jc2, ir2, val2), _, _, st2 = c.run_dij(ph, spec, bix, lds_tables=False, **dk)
+    assert st1.lds_table_bytes > 0 and st2.lds_table_bytes == 0
+    assert np.array_equal(jc, jc2) and np.array_equal(ir, ir2)
+    assert np.allclose(val, val2, rtol=1e-9)
