@@ -623,10 +623,13 @@ def bench_c5(a):
                                "threshold 1e-4", "beamlets": int(bix.n), "histories_per_step": st32.histories,
                    "group": int(st32.group_size), "groups": int(st32.groups)},
         "fp64": {"histories_per_s": st64.histories / w64, "seconds": w64, "nnz": int(st64.nnz)},
-        "fp64_plan_dose_rel_diff": {"value": d, "sigma": sig, "region": "> 20% of the FP64 plan maximum",
+        "fp64_plan_dose_rel_diff": {"value": d, "sigma_independent_voxels": sig,
+                                    "region": "> 20% of the FP64 plan maximum",
                                     "voxels": int(reg.sum()), "frac_within_2sigma": float(np.mean(z <= 2)),
-                                    "what": "one FP32 and one FP64 run of the whole plan, different seeds; sigma "
-                                            "from the plan-dose batch variances summed over the region"},
+                                    "what": "one FP32 and one FP64 run of the whole plan, different seeds; "
+                                            "sigma_independent_voxels sums the per-voxel batch variances over "
+                                            "the region, which understates the sum's uncertainty (voxels share "
+                                            "histories); frac_within_2sigma compares voxel by voxel"},
         "plan_dose_rel_unc_gt50": unc50,
         "nnz": int(st32.nnz),
         "energy_residual": {"fp32": st32.energy_residual(), "fp64": st64.energy_residual()},
