@@ -348,3 +348,40 @@ def test_lds_staged_bins_match_device_memory_reads(ct):
     assert st1.lds_table_bytes > 0 and st2.lds_table_bytes == 0
     assert np.array_equal(jc, jc2) and np.array_equal(ir, ir2)
     assert np.allclose(val, val2, rtol=1e-9)
+
+
+def test_nine_media_read_bins_from_device_memory():
+    """Nine media, the most a context takes: their electron bins (9 x 151
+    records of 80 bytes) exceed the 64 KiB the charged kernel may stage in
+    LDS, so it reads them in place.  Nine slabs along the beam, 6 MV: FP32
+    transport against FP64 per voxel above 10% of the maximum and in the
+    deposited energy (1e6 histories each, independent seeds)."""
+    media = [P.WATER, P.LUNG, P.BONE, P.AIR, "AL521ICRU", "PMMA521ICRU", "POLYSTY521ICRU", "CU521ICRU",
+             "ICRUTISSUE521ICRU"]
+    dens = [1.0, 0.26, 1.85, 0.0012, 2.7, 1.19, 1.06, 8.96, 1.0]
+    n, nz, vox = 24, 36, 0.5
+    zc = (np.arange(nz) + 0.5) * vox
+    lay = np.minimum((zc // 2.0).astype(np.int32), 8)         # 2 cm slabs, the last one to the end
+    med = np.broadcast_to((lay + 1)[:, None, None], (nz, n, n)).astype(np.int32).copy()
+    rho = np.asarray(dens)[med - 1]
+    half = n * vox / 2.0
+    edges = -half + vox * np.arange(n + 1, dtype=np.float64)
+    ph_h = P.Phantom(media, edges, edges.copy(), vox * np.arange(nz + 1, dtype=np.float64), med, rho, "nine")
+    c = omc.Context(media)
+    ph = c.phantom(ph_h, ecut=0.7, pcut=0.01)
+    spec = omc.Spectrum.read(omc.os.path.join(omc.DATA, "spectra", "mohan6.spectrum"))
+    nb = 10
+    kw = dict(collimator=(-4, 4, -4, 4), form=omc.FORM_EVENT)
+    a1, a2, sa = c.run_dosxyz(ph, spec, 100000, nb, precision=omc.FP32, seed=301, **kw)
+    b1, b2, sb = c.run_dosxyz(ph, spec, 100000, nb, precision=omc.FP64, seed=302, **kw)
+    c.close()
+    assert sa.errors == 0 and sb.errors == 0
+    assert sa.lds_table_bytes == 0
+    assert abs(sa.energy_residual()) < 1e-6 and abs(sb.energy_residual()) < 1e-9
+    ag = _agreement(a1, a2, b1, b2, nb)
+    ea, eb = a1[1:].sum(), b1[1:].sum()
+    rel = ea / eb - 1.0
+    print(f"nine media, FP32 vs FP64: {fmt_agreement(ag)}; deposit {rel:+.2e}")
+    assert ag["n"] > 1000
+    assert_agreement(ag)
+    assert abs(rel) < 5e-3
