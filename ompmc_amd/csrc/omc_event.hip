@@ -57,7 +57,11 @@ int device_cus(int dev) {
 }
 
 // ------------------------------------------------------------------ event-form state
-constexpr int DEFAULT_BUDGET = 8;            // condensed-history steps between lane refills
+// Condensed-history steps between lane refills.  Whole 1960-beamlet plan,
+// interleaved-bin build, best of three: 8 -> 17.2, 12 -> 18.2, 16 -> 17.7 M
+// histories/s (with 1 M or 2 M in flight, and 16, 24 or 32 turns, all
+// within 1% of each other at 12).
+constexpr int DEFAULT_BUDGET = 12;
 // Refills an electron may hold its lane in one round before it is requeued.
 // Measured on the 1960-beamlet plan (1 M in flight): 12 -> 12.0, 16 -> 12.3,
 // 24 -> 12.5, 64 -> 11.2, no limit -> 9.7 M histories/s.
@@ -67,7 +71,7 @@ constexpr int DEFAULT_TURNS = 24;
 // media read them from device memory.
 constexpr size_t MAX_LDS_TABLES = 64 * 1024;
 // Round guard of a job: the rounds its injection needs plus 2^16 -- with the
-// requeue after DEFAULT_TURNS x DEFAULT_BUDGET steps that is ~1.7e7 steps for
+// requeue after DEFAULT_TURNS x DEFAULT_BUDGET steps that is ~1.9e7 steps for
 // one particle, thousands of times the longest physical track.  A job that
 // has not drained by then is reported as a step-guard error.
 long max_rounds(long long nhist, long long target) { return 65536L + 8L * (long)(nhist / std::max(target, 1LL)); }
