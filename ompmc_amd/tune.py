@@ -31,7 +31,7 @@ CACHE = os.environ.get("OMC_TUNE_CACHE", os.path.join(os.path.expanduser("~"), "
 # Ladders of the knobs.  The search starts at the engine's defaults (marked)
 # and moves along each ladder one rung at a time.
 TARGETS = (262144, 524288, 1048576, 2097152, 4194304)     # default 4096 per CU = 1048576 on 256 CUs
-BUDGETS = (4, 8, 12, 16)                                  # default 8
+BUDGETS = (4, 8, 12, 16, 24)                              # default 12
 TURNS = (12, 24, 48)                                      # default 24
 STREAMS = (1, 2, 3, 4)                                    # default 3 when the run has several groups
 
@@ -93,7 +93,7 @@ def tune_dij(ctx, phantom, spectrum, bix, nbatch=10, precision=FP32, sample=200,
                 nsplit=nsplit)
     ladders = [tuple(targets), tuple(budgets), tuple(turns), tuple(streams)]
     names = ("event_target", "electron_budget", "electron_turns", "streams")
-    pos = [_start(ladders[0], 1048576 * (4 if nsplit > 1 else 1)), _start(ladders[1], 8), _start(ladders[2], 24),
+    pos = [_start(ladders[0], 1048576 * (4 if nsplit > 1 else 1)), _start(ladders[1], 12), _start(ladders[2], 24),
            _start(ladders[3], 3)]
     timed = {}
 
