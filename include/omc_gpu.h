@@ -48,9 +48,12 @@ extern "C" {
 /* execution form: one lane per history (plain), or event queues (batched) */
 #define OMC_FORM_HISTORY 0
 #define OMC_FORM_EVENT 1
-/* random numbers: counter-based Philox streams keyed by (seed, history), or a
- * replayed 24-bit stream (RANMAR output) with per-history offsets */
-#define OMC_RNG_PHILOX 0
+/* random numbers: keyed streams -- in the history form one Philox4x32-10
+ * stream per history, keyed by (seed, history); in the event form one
+ * xoshiro128** stream per particle, seeded by splitmix64 from a key of
+ * (seed, history) or of the parent's key and draw count -- or a replayed
+ * 24-bit stream (RANMAR output) with per-history offsets */
+#define OMC_RNG_STREAMS 0
 #define OMC_RNG_REPLAY 1
 
 typedef struct omc_ctx omc_ctx;
@@ -110,8 +113,8 @@ typedef struct omc_dosxyz_source_desc {
 typedef struct omc_run_opts {
     int precision;          /* OMC_FP32 | OMC_FP64 */
     int form;               /* OMC_FORM_HISTORY | OMC_FORM_EVENT */
-    int rng;                /* OMC_RNG_PHILOX | OMC_RNG_REPLAY */
-    uint64_t seed;          /* Philox key */
+    int rng;                /* OMC_RNG_STREAMS | OMC_RNG_REPLAY */
+    uint64_t seed;          /* run seed: part of every stream's key */
     int nsplit;             /* photon splitting factor (vrt.nsplit, 1..64); both forms.  In the event
                                form split photons fly voxel by voxel (no delta tracking) */
     const uint32_t *replay;       /* OMC_RNG_REPLAY: draws (24-bit integers) */
@@ -196,7 +199,7 @@ typedef struct omc_run_stats {
     uint64_t lds_table_bytes;
 } omc_run_stats;
 
-/* Defaults: FP32, event form, Philox, seed 1802, nsplit 1, automatic launch. */
+/* Defaults: FP32, event form, keyed streams, seed 1802, nsplit 1, automatic launch. */
 void omc_run_opts_default(omc_run_opts *o);
 
 /* Dose run with omc_dosxyz semantics (main, omc_dosxyz.c:1206-1263):
@@ -285,7 +288,7 @@ int omc_dij_assemble(omc_ctx *ctx, int phantom_id, int nbixels, int nbatch, cons
  * until the process ends; the call allocates nothing.  sum and sum2 are
  * DEVICE arrays of nvox + 1 doubles on the context's device that receive the
  * batch sums (as omc_run_dosxyz's host arrays do).  Limits: nvox + 1 <=
- * OMC_NOALLOC_MAX_REGIONS, form OMC_FORM_HISTORY, Philox random numbers,
+ * OMC_NOALLOC_MAX_REGIONS, form OMC_FORM_HISTORY, keyed random streams,
  * nsplit <= 8; other requests fail with OMC_ERR_ARG.  Calls that overlap on
  * one device share the array and race; the phantom and the context's tables
  * are uploaded beforehand by the ordinary (allocating) calls. */

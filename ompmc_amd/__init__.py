@@ -32,7 +32,7 @@ DATA = os.path.join(REPO, "data")
 FP32, FP64 = 32, 64
 NOALLOC_MAX_REGIONS = 4194304    # OMC_NOALLOC_MAX_REGIONS
 FORM_HISTORY, FORM_EVENT = 0, 1
-RNG_PHILOX, RNG_REPLAY = 0, 1
+RNG_STREAMS, RNG_REPLAY = 0, 1
 INVDIM = 1000
 
 _d = C.POINTER(C.c_double)
@@ -202,7 +202,7 @@ def default_paths():
             DATA + "/")
 
 
-def run_opts(precision=FP32, form=FORM_EVENT, seed=1802, rng=RNG_PHILOX, replay=None, replay_offsets=None,
+def run_opts(precision=FP32, form=FORM_EVENT, seed=1802, rng=RNG_STREAMS, replay=None, replay_offsets=None,
              grid=0, block=0, count_events=False, nsplit=1, event_target=0, electron_budget=0, group=0,
              hbm_budget=0, voxel_walk=False, electron_turns=0, batches_per_job=0, streams=0, max_particle_steps=0,
              csc_on_device=False, graph=True, lds_tables=True):

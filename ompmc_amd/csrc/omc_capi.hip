@@ -563,7 +563,7 @@ extern "C" void omc_run_opts_default(omc_run_opts *o) {
     memset(o, 0, sizeof *o);
     o->precision = OMC_FP32;
     o->form = OMC_FORM_EVENT;
-    o->rng = OMC_RNG_PHILOX;
+    o->rng = OMC_RNG_STREAMS;
     o->seed = 1802;
     o->nsplit = 1;
 }
@@ -660,7 +660,7 @@ static int run_dosxyz_history(omc_ctx *c, hipStream_t stream, PhantomDev *pd, co
         int grid = o->grid > 0 ? o->grid : (int)(want < 2048 ? want : 2048);
         if (grid > max_grid) grid = max_grid;
         if (o->count_events) HIPCHK(hipEventRecord(kev[2 * b], stream));
-        HIPCHK(launch_history<R>(a, o->rng == OMC_RNG_REPLAY ? RNG_REPLAY : RNG_PHILOX, (o->count_events & 1) != 0,
+        HIPCHK(launch_history<R>(a, o->rng == OMC_RNG_REPLAY ? RNG_REPLAY : RNG_STREAMS, (o->count_events & 1) != 0,
                                  grid, block, stream));
         if (o->count_events) HIPCHK(hipEventRecord(kev[2 * b + 1], stream));
         HIPCHK(launch_accumulate(d_e, d_s, d_s2, nreg, 1.0, d_ebal + EB_DEPOSIT, stream));

@@ -76,7 +76,7 @@ constexpr size_t MAX_LDS_TABLES = 64 * 1024;
 // has not drained by then is reported as a step-guard error.
 long max_rounds(long long nhist, long long target) { return 65536L + 8L * (long)(nhist / std::max(target, 1LL)); }
 
-template <class R> size_t pool_entry_bytes() { return sizeof(R) * 10 + sizeof(int) * 6 + 4 + 8; }
+template <class R> size_t pool_entry_bytes() { return sizeof(R) * 10 + sizeof(int) * 6 + 4 + 8 + 16; }
 
 template <class R> size_t event_bytes(long long target) {
     return (size_t)4 * (size_t)(8 * target) * pool_entry_bytes<R>() + sizeof(EvCtl);
@@ -122,6 +122,7 @@ template <class R> struct EvState {
                 for (int **f : fi) *f = A.alloc_n<int>(cap);
                 P.ctr = A.alloc_n<uint32_t>(cap);
                 P.stream = A.alloc_n<uint64_t>(cap);
+                P.rs = A.alloc_n<uint4>(cap);
             }
         ctl = A.alloc_n<EvCtl>(1);
         if (!A.ok) return omc_seterr(OMC_ERR_NOMEM, "event pools for %lld particles do not fit", cap);
@@ -873,7 +874,7 @@ template <class R> int dij_lane_group(DijRun<R> &D, DijLane<R> &l, int gi) {
                 int grid = o->grid > 0 ? o->grid : (int)std::min<long>(want, 2048);
                 if (grid > l.max_grid) grid = l.max_grid;
                 if (D.timing) CK(hipEventRecord(ev.ev[0], stream));
-                CK(launch_history<R>(H, D.replay ? RNG_REPLAY : RNG_PHILOX, D.count, grid, block, stream));
+                CK(launch_history<R>(H, D.replay ? RNG_REPLAY : RNG_STREAMS, D.count, grid, block, stream));
                 if (D.timing) {
                     CK(hipEventRecord(ev.ev[1], stream));
                     CK(hipEventSynchronize(ev.ev[1]));
@@ -1242,7 +1243,7 @@ extern "C" int omc_dij_assemble(omc_ctx *c, int phantom_id, int nbixels, int nba
     omc_run_opts o;
     omc_run_opts_default(&o);
     if (opts) o = *opts;
-    o.rng = OMC_RNG_PHILOX;
+    o.rng = OMC_RNG_STREAMS;
     o.count_events = 0;
     int rc = omc_check_opts(&o);
     if (rc) return rc;

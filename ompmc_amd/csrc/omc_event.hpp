@@ -15,7 +15,7 @@
 // without host round trips; the host reads EvCtl now and then to see whether
 // the job has drained.  Surviving particles and the secondaries they produced
 // are appended to the output pools with one atomic per wavefront (ballot +
-// popcount for the lane offsets).  Every particle carries its own Philox
+// popcount for the lane offsets).  Every particle carries its own random
 // stream: primaries are keyed like the plain history form, secondaries by a
 // hash of the parent's stream and draw position, so a result does not
 // depend on scheduling, grid size or the target.
@@ -30,23 +30,29 @@ template <class R> struct Pool {
     int *ix, *iy, *iz, *iq;
     int *phase;                  // electron_run phase (bits 0-7) and the particle's requeue count (bits 8-31)
     int *slot;                   // tally slot (beamlet of a Dij group); 0 for dose runs
-    uint32_t *ctr;               // next draw of the particle's stream
-    uint64_t *stream;
+    uint32_t *ctr;               // draws taken from the particle's stream
+    uint64_t *stream;            // the stream's key
+    uint4 *rs;                   // the stream's generator state, where ctr > 0
 };
 
-// Philox positioned at draw `pos` of stream `stream`
-struct PhiloxAt {
-    Philox g;
+// A particle's random stream: xoshiro128** keyed by the stream's 64-bit key,
+// with the count of draws taken from it (the position that keys the
+// streams of the particle's secondaries).  A particle that has drawn travels
+// through the queues with its generator state; one that has not (a new
+// secondary) starts from its key.
+struct StreamAt {
+    Xoshiro g;
     uint32_t n;
-    __device__ __forceinline__ void at(uint64_t seed, uint64_t stream, uint32_t pos) {
-        g.init(seed, stream);
-        g.c2 = pos >> 2;
+    __device__ __forceinline__ void at(uint64_t seed, uint64_t stream, uint32_t pos, const uint4 *saved) {
         n = pos;
-        if (pos & 3u) {
-            g.round4();
-            g.skip((int)(pos & 3u));
+        if (pos == 0) {
+            g.init(seed, stream);
+        } else {
+            const uint4 s = *saved;
+            g.s0 = s.x; g.s1 = s.y; g.s2 = s.z; g.s3 = s.w;
         }
     }
+    __device__ __forceinline__ uint4 state() const { return make_uint4(g.s0, g.s1, g.s2, g.s3); }
     template <class R> __device__ __forceinline__ R next() {
         n++;
         return g.template next<R>();
@@ -130,12 +136,13 @@ template <class R> __device__ __forceinline__ void load(const Pool<R> &P, long i
 
 template <class R>
 __device__ __forceinline__ void store(const Pool<R> &P, long i, const Part<R> &p, R demfp, R sig0, int phase,
-                                      uint64_t stream, uint32_t ctr, int slot) {
+                                      uint64_t stream, uint32_t ctr, uint4 rs, int slot) {
     P.e[i] = p.e; P.x[i] = p.x; P.y[i] = p.y; P.z[i] = p.z;
     P.u[i] = p.u; P.v[i] = p.v; P.w[i] = p.w; P.wt[i] = p.wt;
     P.ix[i] = p.ix; P.iy[i] = p.iy; P.iz[i] = p.iz; P.iq[i] = p.iq;
     P.demfp[i] = demfp; P.sig0[i] = sig0; P.phase[i] = phase;
     P.stream[i] = stream; P.ctr[i] = ctr; P.slot[i] = slot;
+    if (ctr != 0u) P.rs[i] = rs;                 // a new stream (ctr 0) starts from its key
 }
 
 // Reserves an output slot for every lane with `want`, one atomic per wave.
@@ -157,16 +164,16 @@ __device__ __forceinline__ long long wave_reserve(unsigned long long *counter, b
 // nothing and is not queued.
 template <class R>
 __device__ __forceinline__ void emit(const EvArgs<R> &a, bool want, const Part<R> &p, R demfp, R sig0, int phase,
-                                     uint64_t stream, uint32_t ctr, int slot) {
+                                     uint64_t stream, uint32_t ctr, uint4 rs, int slot) {
     bool ph = want && p.iq == 0 && p.wt != (R)0, ch = want && p.iq != 0;
     long long ip = wave_reserve(a.n_out + 0, ph);
     long long ie = wave_reserve(a.n_out + 1, ch);
     if (ph) {
-        if (ip < a.cap) store(a.outp, ip, p, demfp, sig0, phase, stream, ctr, slot);
+        if (ip < a.cap) store(a.outp, ip, p, demfp, sig0, phase, stream, ctr, rs, slot);
         else atomicOr(a.err, 1);
     }
     if (ch) {
-        if (ie < a.cap) store(a.oute, ie, p, demfp, sig0, phase, stream, ctr, slot);
+        if (ie < a.cap) store(a.oute, ie, p, demfp, sig0, phase, stream, ctr, rs, slot);
         else atomicOr(a.err, 1);
     }
 }
@@ -238,7 +245,7 @@ k_event(EvArgs<R> a) {
     uint64_t stream = 0;
     int slot = 0, phase = 0;
     R demfp = (R)0, sig0 = (R)0;
-    PhiloxAt rng;
+    StreamAt rng;
     TallyAtomic tally;
     tally.init(a.endep);
     int iters = 0, turns = 0, age = 0;
@@ -259,7 +266,7 @@ k_event(EvArgs<R> a) {
             phase = a.in.phase[idx];
             age = phase >> 8;
             phase &= 0xff;
-            rng.at(a.seed, stream, a.in.ctr[idx]);
+            rng.at(a.seed, stream, a.in.ctr[idx], a.in.rs + idx);
             tally.init(a.endep + (size_t)slot * a.slot_stride,
                        a.touched ? a.touched + (size_t)slot * a.touch_stride : nullptr);
             have = true;
@@ -269,7 +276,7 @@ k_event(EvArgs<R> a) {
         bool done = false;
         uint32_t qpos = 0;
         if (have) {
-            Engine<R, PhiloxAt, TallyAtomic, !PHOTON && sizeof(R) == 4> e{*a.t, *a.g, rng, tally,
+            Engine<R, StreamAt, TallyAtomic, !PHOTON && sizeof(R) == 4> e{*a.t, *a.g, rng, tally,
                                                                           {0, 0, 0, 0, 0, 0, 0}, COUNT};
             e.nsplit = a.nsplit;          // roulette of the photons that charged particles produce
             e.eb = eb;
@@ -313,8 +320,8 @@ k_event(EvArgs<R> a) {
                 tot.nelectron += e.cnt.nelectron;
             }
         }
-        emit(a, have && done && d >= 0, p, demfp, sig0, phase | (age << 8), stream, qpos, slot);
-        emit(a, have && d == 1, q, (R)0, (R)0, 0, child_stream(stream, qpos), 0u, slot);
+        emit(a, have && done && d >= 0, p, demfp, sig0, phase | (age << 8), stream, qpos, rng.state(), slot);
+        emit(a, have && d == 1, q, (R)0, (R)0, 0, child_stream(stream, qpos), 0u, uint4{}, slot);
         if (done) have = false;
     }
     if (PHOTON && a.ebal) wave_add(a.ebal + EB_ESCAPE, eb_esc);
@@ -356,18 +363,18 @@ k_event_split(EvArgs<R> a) {
         Part<R> p;
         uint64_t stream = 0;
         int slot = 0;
-        PhiloxAt rng;
+        StreamAt rng;
         TallyAtomic tally;
         tally.init(a.endep);
         if (have) {
             load(a.in, idx, p);
             stream = a.in.stream[idx];
             slot = a.in.slot[idx];
-            rng.at(a.seed, stream, a.in.ctr[idx]);
+            rng.at(a.seed, stream, a.in.ctr[idx], a.in.rs + idx);
             tally.init(a.endep + (size_t)slot * a.slot_stride,
                        a.touched ? a.touched + (size_t)slot * a.touch_stride : nullptr);
         }
-        Engine<R, PhiloxAt, TallyAtomic> e{*a.t, *a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
+        Engine<R, StreamAt, TallyAtomic> e{*a.t, *a.g, rng, tally, {0, 0, 0, 0, 0, 0, 0}, COUNT};
         e.nsplit = ns;
         // set-up of the split (src/ompmc.c:1900-1940)
         bool live = have, surv = false;
@@ -375,7 +382,7 @@ k_event_split(EvArgs<R> a) {
         int i_survive = -1, imed = -1, lgle = 0;
         R cohfac = (R)0, gmfp = (R)0;
         Part<R> save = p;
-        typename Engine<R, PhiloxAt, TallyAtomic>::SplitXs xs;
+        typename Engine<R, StreamAt, TallyAtomic>::SplitXs xs;
         if (live) {
             if (COUNT) e.cnt.nphoton++;
             imed = e.medium(p);
@@ -451,8 +458,8 @@ k_event_split(EvArgs<R> a) {
                     }
                 }
             }
-            emit(a, w0, c0, (R)0, (R)0, 0, child_stream(stream, cpos), 0u, slot);
-            emit(a, w1, c1, (R)0, (R)0, 0, child_stream(~stream, cpos), 0u, slot);
+            emit(a, w0, c0, (R)0, (R)0, 0, child_stream(stream, cpos), 0u, uint4{}, slot);
+            emit(a, w1, c1, (R)0, (R)0, 0, child_stream(~stream, cpos), 0u, uint4{}, slot);
         }
         if (have && e.aborted) {
             atomicOr(a.err, 2);
@@ -464,7 +471,7 @@ k_event_split(EvArgs<R> a) {
         eb_esc += e.esc;
         eb_lost += e.lost;
         // the survivor continues from its place in the photon's stream
-        emit(a, have && surv, p, (R)0, (R)0, 0, stream, rng.n, slot);
+        emit(a, have && surv, p, (R)0, (R)0, 0, stream, rng.n, rng.state(), slot);
         if (COUNT && have) {
             tot.nhowfar += e.cnt.nhowfar; tot.nhownear += e.cnt.nhownear; tot.ncross += e.cnt.ncross;
             tot.ndep += e.cnt.ndep; tot.nrn += e.cnt.nrn; tot.nphoton += e.cnt.nphoton;
@@ -511,14 +518,14 @@ __global__ void __launch_bounds__(256) k_source(SrcArgs<R> a) {
             beamlet = a.slot_beamlet[slot];
             stream = beamlet_stream(beamlet, a.slot_batch ? a.slot_batch[slot] : a.batch, h - a.slot_off[slot]);
         }
-        PhiloxAt rng;
-        rng.at(a.seed, stream, 0);
+        StreamAt rng;
+        rng.at(a.seed, stream, 0, nullptr);
         TallyNone tn;
-        Engine<R, PhiloxAt, TallyNone> e{*a.t, *a.g, rng, tn, {0, 0, 0, 0, 0, 0, 0}, false};
+        Engine<R, StreamAt, TallyNone> e{*a.t, *a.g, rng, tn, {0, 0, 0, 0, 0, 0, 0}, false};
         Part<R> p;
         if (a.use_beamlets) init_history(e, a.bix, beamlet, p);
         else init_history(e, a.dos, p);
-        store(a.out, at + k, p, (R)0, (R)0, 0, stream, rng.n, slot);
+        store(a.out, at + k, p, (R)0, (R)0, 0, stream, rng.n, rng.state(), slot);
         eb_src += source_energy(p);
     }
     if (a.ebal) wave_add(a.ebal + EB_SOURCE, eb_src);

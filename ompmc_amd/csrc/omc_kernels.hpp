@@ -151,7 +151,7 @@ template <class R, class E> __device__ void init_history(E &e, const SrcBeamlets
 // ------------------------------------------------------------------ history kernel
 constexpr int HIST_STACK = 40;        // deepest LIFO depth below the two register slots
 
-enum RngMode { RNG_PHILOX = 0, RNG_REPLAY = 1 };
+enum RngMode { RNG_STREAMS = 0, RNG_REPLAY = 1 };
 
 template <class R> struct HistArgs {
     const Tables<R> *t;          // device copies (global memory), shared by all lanes

@@ -133,7 +133,7 @@ int run_noalloc(omc_ctx *c, PhantomDev *pd, const omc_dosxyz_source_desc *src, i
         const int block = o->block > 0 ? o->block : 256;
         const long want = (nper + block - 1) / block;
         const int grid = o->grid > 0 ? o->grid : (int)(want < 2048 ? want : 2048);
-        HIPCHK(launch_history<R>(a, RNG_PHILOX, (o->count_events & 1) != 0, grid, block, stream));
+        HIPCHK(launch_history<R>(a, RNG_STREAMS, (o->count_events & 1) != 0, grid, block, stream));
         HIPCHK(launch_accumulate(d_e, d_sum, d_sum2, nreg, 1.0, d_ebal + EB_DEPOSIT, stream));
     }
     HIPCHK(hipStreamSynchronize(stream));
@@ -198,8 +198,8 @@ extern "C" int omc_run_dosxyz_noalloc(omc_ctx *c, int phantom_id, const omc_dosx
     if (o->nsplit > HISTORY_SCRATCH_NSPLIT)
         return omc_seterr(OMC_ERR_ARG, "omc_run_dosxyz_noalloc: nsplit %d (at most %d: deeper stacks need device "
                           "memory)", o->nsplit, HISTORY_SCRATCH_NSPLIT);
-    if (o->rng != OMC_RNG_PHILOX)
-        return omc_seterr(OMC_ERR_ARG, "omc_run_dosxyz_noalloc: Philox random numbers only");
+    if (o->rng != OMC_RNG_STREAMS)
+        return omc_seterr(OMC_ERR_ARG, "omc_run_dosxyz_noalloc: keyed random streams only");
     if (o->form != OMC_FORM_HISTORY)
         return omc_seterr(OMC_ERR_ARG, "omc_run_dosxyz_noalloc: the history form only (the event form's pools "
                           "are sized per run)");

@@ -56,7 +56,7 @@ def slab(tmp_path_factory):
 
 @pytest.mark.parametrize("name", ["slab", "electron", "positron", "vacuum"])
 def test_event_fp64_agrees_with_oracle(tmp_path, name):
-    """Event form (FP64, Philox) against the oracle (FP64, RANMAR) on the same
+    """Event form (FP64, keyed streams) against the oracle (FP64, RANMAR) on the same
     phantom, source and history count."""
     nh, nb = 20000, 20
     stem = write_case_inp(tmp_path, name)

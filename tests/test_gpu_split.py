@@ -62,7 +62,7 @@ def _ctx(name):
 
 @pytest.mark.parametrize("name", ["split", "split_slab", "electron_split", "positron_split"])
 def test_event_split_fp64_agrees_with_oracle(tmp_path, name):
-    """Event form with splitting (FP64, Philox) against the oracle with the
+    """Event form with splitting (FP64, keyed streams) against the oracle with the
     same nsplit (FP64, RANMAR): per voxel, and the total energy against the
     spread of the GPU total over independent seeds."""
     nh, nb = 200000, 20
