@@ -128,14 +128,28 @@ def test_event_split_result_independent_of_schedule():
 def test_event_split_largest_factor_fits_the_pools():
     """nsplit 64 (the largest accepted) with the default queue target: the
     injection is scaled down so that the charged products of one photon
-    launch fit the pools without a rerun."""
+    launch fit the pools without a rerun.  Two runs at nsplit 64 against four
+    unsplit ones, independent seeds, each side summed over its runs.  (One
+    unsplit run of this size carries a fluctuation common to the voxels,
+    which share histories; against a single such run the mean z of an
+    unbiased comparison came out at +0.16 to +0.25.)"""
     name = "split"
     c, ph = _ctx(name)
-    g1, g2, st = _run(c, ph, name, 20000, 10, precision=omc.FP32, form=omc.FORM_EVENT, seed=41, nsplit=64)
-    h1, h2, sh = _run(c, ph, name, 20000, 10, precision=omc.FP32, form=omc.FORM_EVENT, seed=42, nsplit=1)
+
+    def runs(ns, seeds):
+        s1 = s2 = 0.0
+        for sd in seeds:
+            a1, a2, st = _run(c, ph, name, 20000, 10, precision=omc.FP32, form=omc.FORM_EVENT, seed=sd, nsplit=ns)
+            assert st.errors == 0, (ns, sd)
+            s1, s2 = s1 + a1, s2 + a2
+        return s1, s2, 10 * len(seeds), st
+    g1, g2, na, st = runs(64, (41, 43))
+    h1, h2, nb, _ = runs(1, (42, 44, 46, 48))
     c.close()
-    assert st.errors == 0 and sh.errors == 0
-    ag = _agreement(g1, g2, h1, h2, 10)
+    ma, ua = _batch_stats(g1[1:], g2[1:], na)
+    mb, ub = _batch_stats(h1[1:], h2[1:], nb)
+    sel = np.maximum(ma, mb) > 0.1 * max(ma.max(), mb.max())
+    ag = z_agreement(ma[sel], ua[sel], mb[sel], ub[sel], na, nb)
     print(f"nsplit 64 vs 1: {fmt_agreement(ag)}; peak in flight {st.peak_inflight}")
     assert_agreement(ag)
 
