@@ -66,6 +66,17 @@ constexpr int DEFAULT_BUDGET = 12;
 // Measured on the 1960-beamlet plan (1 M in flight): 12 -> 12.0, 16 -> 12.3,
 // 24 -> 12.5, 64 -> 11.2, no limit -> 9.7 M histories/s.
 constexpr int DEFAULT_TURNS = 24;
+// Workgroups per compute unit of the persistent event kernels.  Whole c3
+// plan, best of three (M histories/s), charged x photon: 8 x 8 21.9,
+// 2 x 8 22.0, 2 x 4 22.2, 4 x 4 21.9 -- within run-to-run noise.
+#ifndef OMC_CHARGED_BLOCKS_PER_CU
+#define OMC_CHARGED_BLOCKS_PER_CU 8
+#endif
+#ifndef OMC_PHOTON_BLOCKS_PER_CU
+#define OMC_PHOTON_BLOCKS_PER_CU 8
+#endif
+constexpr int CHARGED_BLOCKS_PER_CU = OMC_CHARGED_BLOCKS_PER_CU;
+constexpr int PHOTON_BLOCKS_PER_CU = OMC_PHOTON_BLOCKS_PER_CU;
 // LDS a charged-kernel workgroup may give to the electron bins (12 KB per
 // medium at 150 bins; two workgroups share a CU's 160 KB).  Plans with more
 // media read them from device memory.
@@ -335,7 +346,8 @@ template <class R> void fill_launch(omc_ctx *c, const PhantomDev *pd, EvState<R>
             L.eb_words = (int)(bytes / 16);
     }
     int cus = device_cus(c->device);
-    L.grid_event = o->grid > 0 ? o->grid : cus * 8;
+    L.grid_event = o->grid > 0 ? o->grid : cus * PHOTON_BLOCKS_PER_CU;
+    L.grid_charged = o->grid > 0 ? o->grid : cus * CHARGED_BLOCKS_PER_CU;
     L.grid_source = cus * 4;
     L.src.t = L.t;
     L.src.g = L.g;

@@ -40,7 +40,7 @@ template <class R> struct EvLaunch {
     const CBin<R> *ebin;         // the electron half of the charged bins (Tables::cbin)
     int eb_words;                // > 0: the charged kernel copies it to LDS (16-byte words)
     SrcArgs<R> src;              // source of the job (ctl and out are set per round)
-    int grid_event, grid_source;
+    int grid_event, grid_charged, grid_source;   // workgroups: photon kernels, charged kernel, source
 };
 
 // Enqueues rounds r0 .. r0+nround-1 of the job whose control block is L.ctl.

@@ -118,7 +118,7 @@ template <class R> void round_call(const EvLaunch<R> &L, long r, int part, bool 
         else
             c.fn = count ? reinterpret_cast<const void *>(&k_event<R, false, true>)
                          : reinterpret_cast<const void *>(&k_event<R, false, false>);
-        c.grid = dim3(L.grid_event);
+        c.grid = dim3(part == 2 ? L.grid_charged : L.grid_event);
     } else {
         c.ctl = L.ctl;
         c.par = par;
