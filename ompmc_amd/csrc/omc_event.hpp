@@ -190,10 +190,11 @@ __device__ __forceinline__ void flush_counters(unsigned long long *dst, const Co
     }
 }
 
-// The photon kernel asks for 4 waves per SIMD (128 VGPRs, 12 bytes of
-// spills): on the c3 plan its busy time fell from 0.30 to 0.24 s per step
-// against 3 waves, the plan's rate unchanged (the photons overlap the charged
-// launches of the other streams).
+// The photon kernel takes the registers it needs (149 VGPRs, 3 waves per
+// SIMD).  Capped at 4 waves (128 VGPRs, 12 bytes of spills) its busy time on
+// the c3 plan fell by a fifth with the plan's rate unchanged, but single
+// dose runs, where it has a stream to itself, ran 1% slower (c2 16.8 against
+// 17.0 M histories/s).
 // Occupancy hint of the charged-particle kernel (waves per SIMD).  With the
 // whole engine inlined the kernel keeps its state in registers; capping it at
 // 4 or 3 waves spills that state to scratch, and on gfx950 2 waves measured
@@ -226,7 +227,7 @@ __device__ __forceinline__ void flush_counters(unsigned long long *dst, const Co
 // would make no progress is dropped and reported after a.max_age requeues,
 // instead of cycling through rounds until the job's round guard ends it.
 template <class R, bool PHOTON, bool COUNT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PHOTON ? 4 : OMC_CHARGED_WAVES_PER_EU)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PHOTON ? 1 : OMC_CHARGED_WAVES_PER_EU)))
 k_event(EvArgs<R> a) {
     // Charged kernel: the electron bins of every medium (nmed x (MEKE + 1)
     // records of 80 bytes, at most 64 KiB; launched with that much dynamic
