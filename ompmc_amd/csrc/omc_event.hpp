@@ -69,7 +69,7 @@ __device__ __forceinline__ uint64_t child_stream(uint64_t s, uint32_t n) {
 
 // Device-resident control block of one job (one batch of histories).
 struct EvCtl {
-    unsigned long long cnt[2][2];     // [round parity][photon, charged] queued particles
+    unsigned long long cnt[2][2];     // [round parity][photon, charged] queued particles (charged: two bands, see emit)
     long long nhist, next_hist;       // histories of the job / injected so far
     long long inj_h0, inj_n, inj_at;  // injection of the coming round
     long long target, cap;
@@ -159,22 +159,51 @@ __device__ __forceinline__ long long wave_reserve(unsigned long long *counter, b
     return want ? (long long)(base + __popcll(below)) : -1;
 }
 
+// Total energy (MeV) above which a charged particle joins the high band of
+// the charged queue.  The queue holds two bands in one pool: particles below
+// fill it from index 0 up, those above from the last index down, and the
+// charged kernel takes the low band first.  A wave's lanes then hold
+// particles of similar energy, whose steps branch alike: on the c3 plan
+// 22.8 M histories/s against 21.6 M with one unsorted queue (thresholds 1.2
+// and 1.5 alike, 2.5 slightly slower).  Where a particle sits in the queue
+// does not change its track.
+constexpr float CHARGED_BAND_E = 1.5f;
+
 // Appends a particle to the photon or the charged output pool.  A photon of
 // zero weight (one that lost the roulette of photon splitting) would deposit
-// nothing and is not queued.
+// nothing and is not queued.  The charged pool's counter packs the low band's
+// length (bits 0-31) and the high band's (bits 32-63); one atomic per wave
+// reserves both, and a reservation that would make the bands meet writes
+// nothing and reports an overflow (the job is rerun with larger pools).
 template <class R>
 __device__ __forceinline__ void emit(const EvArgs<R> &a, bool want, const Part<R> &p, R demfp, R sig0, int phase,
                                      uint64_t stream, uint32_t ctr, uint4 rs, int slot) {
     bool ph = want && p.iq == 0 && p.wt != (R)0, ch = want && p.iq != 0;
     long long ip = wave_reserve(a.n_out + 0, ph);
-    long long ie = wave_reserve(a.n_out + 1, ch);
     if (ph) {
         if (ip < a.cap) store(a.outp, ip, p, demfp, sig0, phase, stream, ctr, rs, slot);
         else atomicOr(a.err, 1);
     }
-    if (ch) {
-        if (ie < a.cap) store(a.oute, ie, p, demfp, sig0, phase, stream, ctr, rs, slot);
-        else atomicOr(a.err, 1);
+    const bool hi = p.e > (R)CHARGED_BAND_E;
+    const unsigned long long mlo = __ballot(ch && !hi), mhi = __ballot(ch && hi);
+    if (mlo | mhi) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((unsigned long long)(mlo | mhi)) - 1;
+        const unsigned long long nlo = (unsigned long long)__popcll(mlo), nhi = (unsigned long long)__popcll(mhi);
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(a.n_out + 1, nlo | (nhi << 32));
+        base = __shfl(base, leader);
+        const unsigned long long lo0 = base & 0xffffffffull, hi0 = base >> 32;
+        const unsigned long long below = (1ull << lane) - 1ull;
+        if (ch) {
+            if ((long long)(lo0 + nlo + hi0 + nhi) <= a.cap) {
+                const long long ie = hi ? a.cap - 1 - (long long)(hi0 + __popcll(mhi & below))
+                                        : (long long)(lo0 + __popcll(mlo & below));
+                store(a.oute, ie, p, demfp, sig0, phase, stream, ctr, rs, slot);
+            } else {
+                atomicOr(a.err, 1);
+            }
+        }
     }
 }
 
@@ -229,6 +258,8 @@ __device__ __forceinline__ void flush_counters(unsigned long long *dst, const Co
 template <class R, bool PHOTON, bool COUNT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PHOTON ? 1 : OMC_CHARGED_WAVES_PER_EU)))
 k_event(EvArgs<R> a) {
+    // the charged queue: low band [0, n_lo), then the high band from cap - 1 down
+    const unsigned long long pk = *a.n_in;
     // Charged kernel: the electron bins of every medium (nmed x (MEKE + 1)
     // records of 80 bytes, at most 64 KiB; launched with that much dynamic
     // LDS) are copied to LDS once per workgroup, and the step loop reads its
@@ -242,7 +273,8 @@ k_event(EvArgs<R> a) {
         __syncthreads();
         eb = reinterpret_cast<const CBin<R> *>(omc_lds);
     }
-    const long long n = (long long)*a.n_in;
+    const long long n_lo = PHOTON ? (long long)pk : (long long)(pk & 0xffffffffull);
+    const long long n = PHOTON ? (long long)pk : n_lo + (long long)(pk >> 32);
     Counters tot = {0, 0, 0, 0, 0, 0, 0};
     double eb_esc = 0.0;             // photons that left the phantom (photon kernel only)
     bool have = false;
@@ -263,6 +295,7 @@ k_event(EvArgs<R> a) {
         if (!have && idx >= 0 && idx < n) {
             iters = 0;
             turns = 0;
+            if (!PHOTON && idx >= n_lo) idx = a.cap - 1 - (idx - n_lo);
             load(a.in, idx, p);
             stream = a.in.stream[idx];
             slot = a.in.slot[idx];
@@ -540,28 +573,37 @@ __global__ void __launch_bounds__(256) k_source(SrcArgs<R> a) {
 // injection of the next round (pools of parity 1 - par).  One thread.
 __device__ __forceinline__ void round_end(EvCtl *c, int par) {
     c->events[0] += c->cnt[par][0];
-    c->events[1] += c->cnt[par][1];
+    c->events[1] += (c->cnt[par][1] & 0xffffffffull) + (c->cnt[par][1] >> 32);
     c->cnt[par][0] = 0;
     c->cnt[par][1] = 0;
     c->claim[par][0] = 0;
     c->claim[par][1] = 0;
     int nx = 1 - par;
-    for (int k = 0; k < 2; k++)
-        if ((long long)c->cnt[nx][k] > c->cap) {
+    if ((long long)c->cnt[nx][0] > c->cap) {
+        c->err |= 1;
+        c->cnt[nx][0] = (unsigned long long)c->cap;
+    }
+    {
+        // an overflowing charged queue holds unwritten entries in both bands:
+        // it is dropped (the job is rerun with larger pools)
+        const unsigned long long pkc = c->cnt[nx][1];
+        if ((long long)((pkc & 0xffffffffull) + (pkc >> 32)) > c->cap) {
             c->err |= 1;
-            c->cnt[nx][k] = (unsigned long long)c->cap;
+            c->cnt[nx][1] = 0;
         }
-    long long inflight = (long long)(c->cnt[nx][0] + c->cnt[nx][1]);
+    }
+    const long long nch = (long long)((c->cnt[nx][1] & 0xffffffffull) + (c->cnt[nx][1] >> 32));
+    long long inflight = (long long)c->cnt[nx][0] + nch;
     long long room = c->target - inflight;
     long long left = c->nhist - c->next_hist;
-    long long space = c->cap - (long long)c->cnt[nx][c->inj_dest];
+    long long space = c->cap - (c->inj_dest ? nch : (long long)c->cnt[nx][0]);
     long long n = left < room ? left : room;
     if (n > space) n = space;
     if (c->inj_max > 0 && n > c->inj_max) n = c->inj_max;
     if (n < 0) n = 0;
     c->inj_h0 = c->next_hist;
     c->inj_n = n;
-    c->inj_at = (long long)c->cnt[nx][c->inj_dest];
+    c->inj_at = c->inj_dest ? (long long)(c->cnt[nx][1] & 0xffffffffull) : (long long)c->cnt[nx][0];
     c->cnt[nx][c->inj_dest] += (unsigned long long)n;
     c->next_hist += n;
     if ((unsigned long long)(inflight + n) > c->peak) c->peak = (unsigned long long)(inflight + n);
