@@ -124,6 +124,9 @@ template <class R> struct EvState {
     int alloc(long long tgt) {
         target = tgt < 64 ? 64 : tgt;
         cap = capmul * target;
+        // the charged pool's counter holds each band's length in 32 bits
+        if (cap > (1ll << 31))
+            return omc_seterr(OMC_ERR_NOMEM, "event pools for %lld particles exceed 2^31 entries", cap);
         for (int p = 0; p < 2; p++)
             for (int k = 0; k < 2; k++) {
                 Pool<R> &P = pool[p][k];
