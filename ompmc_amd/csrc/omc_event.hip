@@ -60,7 +60,9 @@ int device_cus(int dev) {
 // Condensed-history steps between lane refills.  Whole 1960-beamlet plan,
 // interleaved-bin build, best of three: 8 -> 17.2, 12 -> 18.2, 16 -> 17.7 M
 // histories/s (with 1 M or 2 M in flight, and 16, 24 or 32 turns, all
-// within 1% of each other at 12).
+// within 1% of each other at 12).  With the two-band charged queue, 16 against
+// 12: 1960-beamlet plan FP32 22.9 against 22.5 M, FP64 12.4 against 12.6 M;
+// 10,044-beamlet plan at 3e8 histories FP32 14.2 against 15.2 M.
 constexpr int DEFAULT_BUDGET = 12;
 // Refills an electron may hold its lane in one round before it is requeued.
 // Measured on the 1960-beamlet plan (1 M in flight): 12 -> 12.0, 16 -> 12.3,
