@@ -83,6 +83,12 @@ constexpr int PHOTON_BLOCKS_PER_CU = OMC_PHOTON_BLOCKS_PER_CU;
 // medium at 150 bins; two workgroups share a CU's 160 KB).  Plans with more
 // media read them from device memory.
 constexpr size_t MAX_LDS_TABLES = 64 * 1024;
+// Entries of a charged-kernel lane's deposit list (TallyList), 16 bytes each
+// for the 256 lanes of a workgroup; the lists are used where bins and lists
+// together leave room for two workgroups per CU.
+constexpr int TALLY_LIST = 6;
+constexpr size_t TALLY_LIST_BYTES = (size_t)TALLY_LIST * 256 * 16;
+constexpr size_t MAX_LDS_PER_GROUP = 80 * 1024;
 // Round guard of a job: the rounds its injection needs plus 2^16 -- with the
 // requeue after DEFAULT_TURNS x DEFAULT_BUDGET steps that is ~1.9e7 steps for
 // one particle, thousands of times the longest physical track.  A job that
@@ -349,6 +355,9 @@ template <class R> void fill_launch(omc_ctx *c, const PhantomDev *pd, EvState<R>
         const size_t bytes = (size_t)ht.nmed * ht.cb_stride * sizeof(CBin<R>);
         if (std::is_same<R, float>::value && o->lds_tables >= 0 && bytes <= MAX_LDS_TABLES)
             L.eb_words = (int)(bytes / 16);
+        // FP32: the deposit lists, where they fit beside the bins in two workgroups per CU
+        if (std::is_same<R, float>::value && (size_t)L.eb_words * 16 + TALLY_LIST_BYTES <= MAX_LDS_PER_GROUP)
+            L.tlist = TALLY_LIST;
     }
     int cus = device_cus(c->device);
     L.grid_event = o->grid > 0 ? o->grid : cus * PHOTON_BLOCKS_PER_CU;

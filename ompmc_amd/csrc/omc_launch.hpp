@@ -39,6 +39,7 @@ template <class R> struct EvLaunch {
     int graph;                   // 1: untimed chunks of rounds replay a HIP graph
     const CBin<R> *ebin;         // the electron half of the charged bins (Tables::cbin)
     int eb_words;                // > 0: the charged kernel copies it to LDS (16-byte words)
+    int tlist;                   // deposit-list entries per charged-kernel lane in LDS (0: direct atomics)
     SrcArgs<R> src;              // source of the job (ctl and out are set per round)
     int grid_event, grid_charged, grid_source;   // workgroups: photon kernels, charged kernel, source
 };

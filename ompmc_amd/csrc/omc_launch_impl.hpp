@@ -105,7 +105,8 @@ template <class R> void round_call(const EvLaunch<R> &L, long r, int part, bool 
         a.nsplit = L.nsplit > 1 ? L.nsplit : 1;
         a.ebin = L.ebin;
         a.eb_words = part == 2 ? L.eb_words : 0;
-        c.shmem = (unsigned)a.eb_words * 16u;
+        a.tlist = part == 2 ? L.tlist : 0;
+        c.shmem = (unsigned)a.eb_words * 16u + (unsigned)a.tlist * c.block.x * 16u;
         a.in = L.pool[par][part - 1];
         a.n_in = &L.ctl->cnt[par][part - 1];
         a.claim = &L.ctl->claim[par][part - 1];
